@@ -1,0 +1,264 @@
+// fgl_bin.hip -- sort-middle binning.  Every raster triangle becomes one
+// (tile, triangle) pair per tile its box can reach, written in submission
+// order (input index, then fan / line-quad order).  A stable LSD radix sort
+// on the tile id then leaves each tile's triangles contiguous and still in
+// submission order, which is the order the reference draws them in
+// (context.go:413-433 with one worker).
+#include "fgl_internal.h"
+#include "fgl_tiles.h"
+
+namespace fgl {
+
+// ---------------------------------------------------------------------------
+// per-draw reset
+
+__global__ void k_zero(DrawArgs a) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        DrawCounters z = {};
+        *a.ctr = z;
+    }
+    for (int64_t t = i; t < a.ntiles; t += (int64_t)gridDim.x * blockDim.x) {
+        a.tstart[t] = 0;
+        a.tend[t] = 0;
+    }
+}
+
+hipError_t launch_zero(const DrawArgs &a, hipStream_t s) {
+    int blocks = (a.ntiles + 255) / 256;
+    if (blocks < 1) blocks = 1;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(k_zero, dim3(blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan of the vertex-stage block sums (one workgroup)
+
+constexpr int SCAN_THREADS = 1024;
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(DrawArgs a) {
+    __shared__ uint64_t part[SCAN_THREADS];
+    const int n = a.nvsblocks;
+    const int tid = threadIdx.x;
+    const int per = (n + SCAN_THREADS - 1) / SCAN_THREADS;
+    const int lo = tid * per;
+    const int hi = lo + per < n ? lo + per : n;
+    uint64_t s = 0;
+    for (int k = lo; k < hi; k++) s += a.blocksum[k];
+    part[tid] = s;
+    __syncthreads();
+    for (int off = 1; off < SCAN_THREADS; off <<= 1) {
+        uint64_t v = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    uint64_t run = part[tid] - s;  // exclusive
+    for (int k = lo; k < hi; k++) {
+        uint64_t v = a.blocksum[k];
+        a.blockoff[k] = run;
+        run += v;
+    }
+    if (tid == SCAN_THREADS - 1) {
+        uint64_t total = part[tid];
+        uint32_t ovf = a.ctr->overflow;
+        if (total > a.pair_cap) ovf |= 4u;
+        a.ctr->npairs = total > a.pair_cap ? a.pair_cap : (uint32_t)total;
+        a.ctr->pairs_lo = (uint32_t)total;
+        a.ctr->pairs_hi = (uint32_t)(total >> 32);
+        a.ctr->overflow = ovf;
+    }
+}
+
+hipError_t launch_scan_top(const DrawArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_THREADS), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// pair emission
+
+__global__ __launch_bounds__(VS_THREADS) void k_emit(DrawArgs a) {
+    __shared__ uint64_t wsum[VS_THREADS / 64];
+    if (a.ctr->overflow) return;
+    int64_t i = (int64_t)blockIdx.x * VS_THREADS + threadIdx.x;
+    uint32_t nt = i < a.in.n ? a.ntl[i] : 0u;
+    // block-exclusive scan of nt
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint64_t v = nt;
+    for (int off = 1; off < 64; off <<= 1) {
+        uint64_t y = __shfl_up(v, off);
+        if (lane >= off) v += y;
+    }
+    if (lane == 63) wsum[wave] = v;
+    __syncthreads();
+    uint64_t wbase = 0;
+    for (int w = 0; w < wave; w++) wbase += wsum[w];
+    uint64_t pos = a.blockoff[blockIdx.x] + wbase + v - nt;
+    if (i >= a.in.n || nt == 0) return;
+    uint32_t nr = a.nrast[i];
+    bool side = (nr & REF_SIDE) != 0;
+    uint32_t cnt = nr & ~REF_SIDE;
+    uint32_t *keys = a.keys[0];
+    uint32_t *vals = a.vals[0];
+    for (uint32_t j = 0; j < cnt; j++) {
+        const PrimRec *rec;
+        uint32_t ref;
+        if (side) {
+            ref = REF_SIDE | (a.sstart[i] + j);
+            rec = a.side_rec + (a.sstart[i] + j);
+        } else {
+            ref = (uint32_t)i;
+            rec = a.rec_main + i;
+        }
+        TileRect r;
+        if (!prim_tile_rect(rec->x0, rec->y0, rec->x1, rec->y1, a.W, a.H, r)) continue;
+        for (int32_t ty = r.ty0; ty <= r.ty1; ty++)
+            for (int32_t tx = r.tx0; tx <= r.tx1; tx++) {
+                keys[pos] = (uint32_t)(ty * a.ntx + tx);
+                vals[pos] = ref;
+                pos++;
+            }
+    }
+}
+
+hipError_t launch_emit(const DrawArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_emit, dim3(a.nvsblocks), dim3(VS_THREADS), 0, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// stable LSD radix sort of (tile id, ref) pairs, 8 bits per pass
+
+__global__ __launch_bounds__(RS_THREADS) void k_radix_hist(DrawArgs a, int pass) {
+    __shared__ uint32_t h[256];
+    if (a.ctr->overflow) return;
+    const uint32_t E = a.ctr->npairs;
+    const uint32_t *keys = a.keys[pass & 1];
+    const int shift = 8 * pass;
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * RS_BLOCK;
+    for (int it = 0; it < RS_ITEMS; it++) {
+        uint64_t idx = base + (uint64_t)it * RS_THREADS + threadIdx.x;
+        if (idx < E) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    a.hist[(uint64_t)threadIdx.x * a.nrsblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_radix_scan(DrawArgs a) {
+    __shared__ uint32_t part[SCAN_THREADS];
+    if (a.ctr->overflow) return;
+    const int64_t n = (int64_t)256 * a.nrsblocks;
+    const int tid = threadIdx.x;
+    const int64_t per = (n + SCAN_THREADS - 1) / SCAN_THREADS;
+    const int64_t lo = tid * per;
+    const int64_t hi = lo + per < n ? lo + per : n;
+    uint32_t s = 0;
+    for (int64_t k = lo; k < hi; k++) s += a.hist[k];
+    part[tid] = s;
+    __syncthreads();
+    for (int off = 1; off < SCAN_THREADS; off <<= 1) {
+        uint32_t v = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[tid] - s;
+    for (int64_t k = lo; k < hi; k++) {
+        uint32_t v = a.hist[k];
+        a.hist[k] = run;
+        run += v;
+    }
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(DrawArgs a, int pass) {
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[RS_THREADS / 64][256];
+    if (a.ctr->overflow) return;
+    const uint32_t E = a.ctr->npairs;
+    const uint32_t *kin = a.keys[pass & 1];
+    const uint32_t *vin = a.vals[pass & 1];
+    uint32_t *kout = a.keys[(pass + 1) & 1];
+    uint32_t *vout = a.vals[(pass + 1) & 1];
+    const int shift = 8 * pass;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint64_t ltmask = (1ull << lane) - 1ull;
+    run[tid] = a.hist[(uint64_t)tid * a.nrsblocks + blockIdx.x];
+    const uint64_t base = (uint64_t)blockIdx.x * RS_BLOCK;
+    for (int it = 0; it < RS_ITEMS; it++) {
+        for (int w = 0; w < RS_THREADS / 64; w++) wcnt[w][tid] = 0;
+        __syncthreads();
+        uint64_t idx = base + (uint64_t)it * RS_THREADS + tid;
+        bool valid = idx < E;
+        uint32_t key = valid ? kin[idx] : 0u;
+        uint32_t val = valid ? vin[idx] : 0u;
+        uint32_t digit = (key >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            bool bit = (digit >> b) & 1u;
+            uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        uint32_t rank = (uint32_t)__popcll(peers & ltmask);
+        if (valid && rank == 0) wcnt[wave][digit] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        {
+            uint32_t r = run[tid];
+            for (int w = 0; w < RS_THREADS / 64; w++) {
+                uint32_t c = wcnt[w][tid];
+                wcnt[w][tid] = r;
+                r += c;
+            }
+            run[tid] = r;
+        }
+        __syncthreads();
+        if (valid) {
+            uint32_t p = wcnt[wave][digit] + rank;
+            kout[p] = key;
+            vout[p] = val;
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_radix_hist(const DrawArgs &a, int pass, hipStream_t s) {
+    hipLaunchKernelGGL(k_radix_hist, dim3(a.nrsblocks), dim3(RS_THREADS), 0, s, a, pass);
+    return hipGetLastError();
+}
+hipError_t launch_radix_scan(const DrawArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(SCAN_THREADS), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_radix_scatter(const DrawArgs &a, int pass, hipStream_t s) {
+    hipLaunchKernelGGL(k_radix_scatter, dim3(a.nrsblocks), dim3(RS_THREADS), 0, s, a, pass);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// per-tile ranges in the sorted pairs
+
+__global__ void k_ranges(DrawArgs a) {
+    if (a.ctr->overflow) return;
+    const uint32_t E = a.ctr->npairs;
+    const uint32_t *keys = a.keys[a.sorted_buf];
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < E;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t t = keys[i];
+        if (i == 0 || keys[i - 1] != t) a.tstart[t] = (uint32_t)i;
+        if (i + 1 == E || keys[i + 1] != t) a.tend[t] = (uint32_t)(i + 1);
+    }
+}
+
+hipError_t launch_ranges(const DrawArgs &a, hipStream_t s) {
+    uint64_t blocks = ((uint64_t)a.pair_cap + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_ranges, dim3((unsigned)blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace fgl

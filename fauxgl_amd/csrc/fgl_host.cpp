@@ -1,0 +1,909 @@
+// fgl_host.cpp -- the C ABI (include/fgl.h): contexts, device buffers,
+// deferred clears, the per-draw kernel sequence and its capacity retries.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/fgl.h"
+#include "fgl_internal.h"
+
+using namespace fgl;
+
+namespace {
+
+thread_local std::string g_err = "";
+thread_local int g_device = -1;
+
+void set_err(std::string &dst, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    dst = buf;
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t ensure(size_t need) {
+        if (need <= bytes && p) return hipSuccess;
+        if (p) {
+            (void)hipFree(p);
+            p = nullptr;
+            bytes = 0;
+        }
+        size_t want = need < 256 ? 256 : need;
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        bytes = want;
+        return hipSuccess;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct Tex {
+    void *d = nullptr;
+    int32_t w = 0, h = 0, kind = 0;
+};
+
+struct KStat {
+    uint64_t launches = 0;
+    double ms = 0;
+};
+
+struct EvPair {
+    std::string name;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct fgl_ctx {
+    int device = 0;
+    int32_t W = 0, H = 0;
+    hipStream_t stream = nullptr;
+    DevBuf color, depth;
+    bool pend_color = false, pend_depth = false;
+    uint32_t clear_color = 0;
+    double clear_depth = 0;
+    std::string err;
+    std::map<int64_t, Tex> tex;
+    int64_t next_tex = 1;
+
+    DevBuf ctr, ntl, nrast, sstart, blocksum, blockoff, rec_main, side_rec, side_vtx;
+    DevBuf keys0, vals0, keys1, vals1, hist, tstart, tend;
+    DevBuf in_pos, in_nrm, in_uv, in_col;
+    DevBuf rs_tmp, rs_wx, rs_ox, rs_wy, rs_oy;
+    DevBuf dpart, dimg;
+    uint32_t side_rec_cap = 0, side_vtx_cap = 0, pair_cap = 0;
+
+    bool prof = false;
+    std::map<std::string, KStat> kstats;
+    std::vector<EvPair> pending;
+    std::vector<hipEvent_t> pool;
+
+    ~fgl_ctx() {
+        for (auto &kv : tex)
+            if (kv.second.d) (void)hipFree(kv.second.d);
+        for (auto &p : pending) {
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+struct fgl_mesh {
+    fgl_ctx *ctx = nullptr;
+    int64_t ntri = 0, nline = 0;
+    DevBuf tpos, tnrm, tuv, tcol, lpos, lnrm, luv, lcol;
+    bool t_has[4] = {false, false, false, false};
+    bool l_has[4] = {false, false, false, false};
+};
+
+namespace {
+
+#define HIPCHK(ctx, call)                                                                        \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess) {                                                                  \
+            set_err((ctx)->err, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                    __LINE__);                                                                   \
+            return FGL_E_DEVICE;                                                                 \
+        }                                                                                        \
+    } while (0)
+
+hipEvent_t ev_get(fgl_ctx *c) {
+    if (!c->pool.empty()) {
+        hipEvent_t e = c->pool.back();
+        c->pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Collects the durations of finished profiled launches.
+void prof_collect(fgl_ctx *c) {
+    for (auto &p : c->pending) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            KStat &k = c->kstats[p.name];
+            k.launches++;
+            k.ms += ms;
+        }
+        c->pool.push_back(p.a);
+        c->pool.push_back(p.b);
+    }
+    c->pending.clear();
+}
+
+// Runs one launch, bracketed by events when profiling.
+template <class F>
+int launch(fgl_ctx *c, const char *name, F &&f) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (c->prof) {
+        a = ev_get(c);
+        b = ev_get(c);
+        if (a) (void)hipEventRecord(a, c->stream);
+    }
+    hipError_t e = f();
+    if (e != hipSuccess) {
+        set_err(c->err, "launch of %s failed: %s", name, hipGetErrorString(e));
+        return FGL_E_DEVICE;
+    }
+    if (c->prof && a && b) {
+        (void)hipEventRecord(b, c->stream);
+        c->pending.push_back(EvPair{name, a, b});
+    }
+    return FGL_OK;
+}
+
+int select_device(std::string &err) {
+    if (g_device >= 0) {
+        if (hipSetDevice(g_device) != hipSuccess) {
+            set_err(err, "hipSetDevice(%d) failed", g_device);
+            return FGL_E_DEVICE;
+        }
+        return FGL_OK;
+    }
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n < 1) {
+        set_err(err, "no HIP device available (%s)", hipGetErrorString(e));
+        return FGL_E_DEVICE;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) {
+        set_err(err, "hipGetDeviceProperties failed");
+        return FGL_E_DEVICE;
+    }
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        set_err(err, "device 0 is %s; this library is built for gfx950", prop.gcnArchName);
+        return FGL_E_DEVICE;
+    }
+    g_device = 0;
+    if (hipSetDevice(0) != hipSuccess) {
+        set_err(err, "hipSetDevice(0) failed");
+        return FGL_E_DEVICE;
+    }
+    return FGL_OK;
+}
+
+// Screen(w, h) (matrix.go:119-128)
+void screen_matrix(int32_t w, int32_t h, double *S) {
+    double w2 = (double)w / 2;
+    double h2 = (double)h / 2;
+    const double m[16] = {w2, 0, 0, w2, 0, -h2, 0, h2, 0, 0, 0.5, 0.5, 0, 0, 0, 1};
+    std::memcpy(S, m, sizeof m);
+}
+
+int materialize(fgl_ctx *c) {
+    int64_t n = (int64_t)c->W * c->H;
+    if (c->pend_color) {
+        int r = launch(c, "fill_color", [&] { return launch_fill_color(c->color.as<uint32_t>(), n, c->clear_color, c->stream); });
+        if (r) return r;
+        c->pend_color = false;
+    }
+    if (c->pend_depth) {
+        int r = launch(c, "fill_depth", [&] { return launch_fill_depth(c->depth.as<double>(), n, c->clear_depth, c->stream); });
+        if (r) return r;
+        c->pend_depth = false;
+    }
+    return FGL_OK;
+}
+
+int key_bits_for(int32_t ntiles) {
+    int b = 1;
+    while ((1ll << b) < ntiles) b++;
+    return b;
+}
+
+int fill_shader(fgl_ctx *c, const fgl_shader *sh, DevShader &d) {
+    std::memset(&d, 0, sizeof d);
+    d.kind = sh->kind;
+    if (sh->kind != FGL_SHADER_SOLID && sh->kind != FGL_SHADER_TEXTURE && sh->kind != FGL_SHADER_PHONG) {
+        set_err(c->err, "unknown shader kind %d (custom shaders run on the CPU path)", sh->kind);
+        return FGL_E_UNSUPPORTED;
+    }
+    std::memcpy(d.M, sh->matrix, sizeof d.M);
+    std::memcpy(d.color, sh->color, sizeof d.color);
+    std::memcpy(d.light, sh->light_dir, sizeof d.light);
+    std::memcpy(d.cam, sh->camera_pos, sizeof d.cam);
+    std::memcpy(d.obj, sh->object_color, sizeof d.obj);
+    std::memcpy(d.amb, sh->ambient, sizeof d.amb);
+    std::memcpy(d.dif, sh->diffuse, sizeof d.dif);
+    std::memcpy(d.spc, sh->specular, sizeof d.spc);
+    d.power = sh->specular_power;
+    // ObjectColor != Discard compares all four channels with 0 (shader.go:76)
+    d.obj_is_discard = (sh->object_color[0] == 0 && sh->object_color[1] == 0 && sh->object_color[2] == 0 &&
+                        sh->object_color[3] == 0);
+    bool want_tex = sh->kind == FGL_SHADER_TEXTURE || (sh->kind == FGL_SHADER_PHONG && sh->texture > 0);
+    if (want_tex) {
+        auto it = c->tex.find(sh->texture);
+        if (it == c->tex.end()) {
+            set_err(c->err, "texture handle %lld is not a texture of this context", (long long)sh->texture);
+            return FGL_E_INVALID;
+        }
+        d.has_tex = 1;
+        d.tex_kind = it->second.kind;
+        d.tex_w = it->second.w;
+        d.tex_h = it->second.h;
+        d.tex_pix = it->second.d;
+    }
+    return FGL_OK;
+}
+
+constexpr int64_t kMaxPrims = (int64_t)1 << 26;
+
+int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, const DevPrims &in, uint64_t info[2]) {
+    if (info) info[0] = info[1] = 0;
+    if (in.n < 0 || in.n > kMaxPrims) {
+        set_err(c->err, "primitive count %lld outside [0, %lld]", (long long)in.n, (long long)kMaxPrims);
+        return FGL_E_INVALID;
+    }
+    DrawArgs a;
+    std::memset(&a, 0, sizeof a);
+    int r = fill_shader(c, sh, a.sh);
+    if (r) return r;
+    if (in.n == 0) return FGL_OK;
+    if (hipSetDevice(c->device) != hipSuccess) {
+        set_err(c->err, "hipSetDevice failed");
+        return FGL_E_DEVICE;
+    }
+
+    a.W = c->W;
+    a.H = c->H;
+    a.ntx = (c->W + TILE_W - 1) / TILE_W;
+    a.nty = (c->H + TILE_H - 1) / TILE_H;
+    a.ntiles = a.ntx * a.nty;
+    a.kind = kind;
+    a.in = in;
+    a.st.read_depth = st->read_depth != 0;
+    a.st.write_depth = st->write_depth != 0;
+    a.st.write_color = st->write_color != 0;
+    a.st.alpha_blend = st->alpha_blend != 0;
+    a.st.wireframe = st->wireframe != 0;
+    a.st.front_face = st->front_face;
+    a.st.cull = st->cull;
+    a.st.line_width = st->line_width;
+    a.st.depth_bias = st->depth_bias;
+    // Rejecting bz > depth before shading equals the reference's locked test
+    // only while stored depth cannot increase.
+    a.st.early_z = a.st.read_depth && (!a.st.write_depth || st->depth_bias >= 0);
+    screen_matrix(c->W, c->H, a.S);
+    a.color = c->color.as<uint32_t>();
+    a.depth = c->depth.as<double>();
+
+    const int64_t n = in.n;
+    a.nvsblocks = (int32_t)((n + VS_THREADS - 1) / VS_THREADS);
+    a.key_bits = key_bits_for(a.ntiles);
+    const int npass = (a.key_bits + 7) / 8;
+    a.sorted_buf = npass & 1;
+
+    // starting capacities; grown on overflow
+    uint64_t want_side = kind == PRIM_LINE ? (uint64_t)2 * n : (st->wireframe ? (uint64_t)6 * n : (uint64_t)n / 8);
+    want_side += 4096;
+    uint64_t want_vtx = kind == PRIM_LINE ? 256 : (uint64_t)n / 4 + 4096;
+    uint64_t want_pairs = (uint64_t)2 * n + (uint64_t)a.ntiles + 65536;
+    if (c->side_rec_cap < want_side) c->side_rec_cap = (uint32_t)std::min<uint64_t>(want_side, 0x7fffffffu);
+    if (c->side_vtx_cap < want_vtx) c->side_vtx_cap = (uint32_t)std::min<uint64_t>(want_vtx, 0x7fffffffu);
+    if (c->pair_cap < want_pairs) c->pair_cap = (uint32_t)std::min<uint64_t>(want_pairs, 0xfffff000u);
+
+    HIPCHK(c, c->ctr.ensure(sizeof(DrawCounters)));
+    HIPCHK(c, c->ntl.ensure((size_t)n * 4));
+    HIPCHK(c, c->nrast.ensure((size_t)n * 4));
+    HIPCHK(c, c->sstart.ensure((size_t)n * 4));
+    HIPCHK(c, c->blocksum.ensure((size_t)a.nvsblocks * 8));
+    HIPCHK(c, c->blockoff.ensure((size_t)a.nvsblocks * 8));
+    if (kind == PRIM_TRI) HIPCHK(c, c->rec_main.ensure((size_t)n * sizeof(PrimRec)));
+    HIPCHK(c, c->tstart.ensure((size_t)a.ntiles * 4));
+    HIPCHK(c, c->tend.ensure((size_t)a.ntiles * 4));
+
+    for (int attempt = 0; attempt < 6; attempt++) {
+        HIPCHK(c, c->side_rec.ensure((size_t)c->side_rec_cap * sizeof(PrimRec)));
+        HIPCHK(c, c->side_vtx.ensure((size_t)c->side_vtx_cap * SIDE_VTX_DOUBLES * 8));
+        HIPCHK(c, c->keys0.ensure((size_t)c->pair_cap * 4));
+        HIPCHK(c, c->vals0.ensure((size_t)c->pair_cap * 4));
+        HIPCHK(c, c->keys1.ensure((size_t)c->pair_cap * 4));
+        HIPCHK(c, c->vals1.ensure((size_t)c->pair_cap * 4));
+        a.nrsblocks = (int32_t)(((uint64_t)c->pair_cap + RS_BLOCK - 1) / RS_BLOCK);
+        HIPCHK(c, c->hist.ensure((size_t)256 * a.nrsblocks * 4));
+
+        a.rec_main = c->rec_main.as<PrimRec>();
+        a.side_rec = c->side_rec.as<PrimRec>();
+        a.side_rec_cap = c->side_rec_cap;
+        a.side_vtx = c->side_vtx.as<double>();
+        a.side_vtx_cap = c->side_vtx_cap;
+        a.ntl = c->ntl.as<uint32_t>();
+        a.nrast = c->nrast.as<uint32_t>();
+        a.sstart = c->sstart.as<uint32_t>();
+        a.blocksum = c->blocksum.as<uint64_t>();
+        a.blockoff = c->blockoff.as<uint64_t>();
+        a.keys[0] = c->keys0.as<uint32_t>();
+        a.vals[0] = c->vals0.as<uint32_t>();
+        a.keys[1] = c->keys1.as<uint32_t>();
+        a.vals[1] = c->vals1.as<uint32_t>();
+        a.pair_cap = c->pair_cap;
+        a.hist = c->hist.as<uint32_t>();
+        a.tstart = c->tstart.as<uint32_t>();
+        a.tend = c->tend.as<uint32_t>();
+        a.clear_color_pending = c->pend_color;
+        a.clear_depth_pending = c->pend_depth;
+        a.clear_color = c->clear_color;
+        a.clear_depth = c->clear_depth;
+        a.ctr = c->ctr.as<DrawCounters>();
+
+        if ((r = launch(c, "zero", [&] { return launch_zero(a, c->stream); }))) return r;
+        if ((r = launch(c, "vertex", [&] { return launch_vs(a, c->stream); }))) return r;
+        if ((r = launch(c, "scan", [&] { return launch_scan_top(a, c->stream); }))) return r;
+        if ((r = launch(c, "emit", [&] { return launch_emit(a, c->stream); }))) return r;
+        for (int p = 0; p < npass; p++) {
+            if ((r = launch(c, "radix_hist", [&] { return launch_radix_hist(a, p, c->stream); }))) return r;
+            if ((r = launch(c, "radix_scan", [&] { return launch_radix_scan(a, c->stream); }))) return r;
+            if ((r = launch(c, "radix_scatter", [&] { return launch_radix_scatter(a, p, c->stream); }))) return r;
+        }
+        if ((r = launch(c, "ranges", [&] { return launch_ranges(a, c->stream); }))) return r;
+        if ((r = launch(c, "tile", [&] { return launch_tile(a, c->stream); }))) return r;
+
+        DrawCounters h;
+        HIPCHK(c, hipMemcpyAsync(&h, a.ctr, sizeof h, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (c->prof) prof_collect(c);
+        if (h.overflow == 0) {
+            if (info) {
+                info[0] = h.total;
+                info[1] = h.updated;
+            }
+            c->pend_color = false;
+            c->pend_depth = false;
+            return FGL_OK;
+        }
+        if (h.overflow & 8u) {
+            set_err(c->err, "internal error: tile fragment buffer overflow");
+            return FGL_E_DEVICE;
+        }
+        if (h.overflow & 1u) c->side_rec_cap = (uint32_t)std::min<uint64_t>((uint64_t)h.side_rec * 5 / 4 + 4096, 0x7fffffffu);
+        if (h.overflow & 2u) c->side_vtx_cap = (uint32_t)std::min<uint64_t>((uint64_t)h.side_vtx * 5 / 4 + 4096, 0x7fffffffu);
+        if (h.overflow & 4u) {
+            uint64_t need = ((uint64_t)h.pairs_hi << 32 | h.pairs_lo) * 5 / 4 + 65536;
+            if (need > 0xfffff000ull) {
+                set_err(c->err, "draw needs %llu (tile, triangle) pairs, more than 2^32", (unsigned long long)need);
+                return FGL_E_UNSUPPORTED;
+            }
+            c->pair_cap = (uint32_t)need;
+        }
+    }
+    set_err(c->err, "draw did not fit after repeated capacity growth");
+    return FGL_E_NOMEM;
+}
+
+int upload(fgl_ctx *c, DevBuf &b, const double *src, size_t count, const double *&dst) {
+    if (!src) {
+        dst = nullptr;
+        return FGL_OK;
+    }
+    HIPCHK(c, b.ensure(count * 8));
+    HIPCHK(c, hipMemcpyAsync(b.p, src, count * 8, hipMemcpyHostToDevice, c->stream));
+    dst = b.as<double>();
+    return FGL_OK;
+}
+
+int check_ctx(const fgl_ctx *c) {
+    if (!c) {
+        g_err = "null context";
+        return FGL_E_INVALID;
+    }
+    return FGL_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+
+extern "C" {
+
+int fgl_init(int32_t device) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n < 1) {
+        set_err(g_err, "no HIP device available (%s)", hipGetErrorString(e));
+        return FGL_E_DEVICE;
+    }
+    if (device < 0 || device >= n) {
+        set_err(g_err, "device %d out of range [0, %d)", device, n);
+        return FGL_E_INVALID;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+        set_err(g_err, "hipGetDeviceProperties failed");
+        return FGL_E_DEVICE;
+    }
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        set_err(g_err, "device %d is %s; this library is built for gfx950", device, prop.gcnArchName);
+        return FGL_E_DEVICE;
+    }
+    if (hipSetDevice(device) != hipSuccess) {
+        set_err(g_err, "hipSetDevice(%d) failed", device);
+        return FGL_E_DEVICE;
+    }
+    g_device = device;
+    return FGL_OK;
+}
+
+const char *fgl_last_error(const fgl_ctx *ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+int fgl_ctx_create(int32_t width, int32_t height, fgl_ctx **out) {
+    if (!out || width <= 0 || height <= 0 || (int64_t)width * height > ((int64_t)1 << 31)) {
+        set_err(g_err, "fgl_ctx_create: invalid size %d x %d", width, height);
+        return FGL_E_INVALID;
+    }
+    *out = nullptr;
+    int r = select_device(g_err);
+    if (r) return r;
+    fgl_ctx *c = new (std::nothrow) fgl_ctx;
+    if (!c) {
+        g_err = "out of host memory";
+        return FGL_E_NOMEM;
+    }
+    c->W = width;
+    c->H = height;
+    (void)hipGetDevice(&c->device);
+    size_t npix = (size_t)width * height;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess || c->color.ensure(npix * 4) ||
+        c->depth.ensure(npix * 8)) {
+        set_err(g_err, "device allocation of a %d x %d framebuffer failed", width, height);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        delete c;
+        return FGL_E_NOMEM;
+    }
+    // image.NewNRGBA starts zeroed; NewContext clears depth to MaxFloat64.
+    if (hipMemsetAsync(c->color.p, 0, npix * 4, c->stream) != hipSuccess) {
+        g_err = "hipMemsetAsync failed";
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return FGL_E_DEVICE;
+    }
+    c->pend_depth = true;
+    c->clear_depth = DBL_MAX;
+    *out = c;
+    return FGL_OK;
+}
+
+void fgl_ctx_destroy(fgl_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    hipStream_t s = ctx->stream;
+    delete ctx;
+    (void)hipStreamDestroy(s);
+}
+
+int fgl_ctx_size(const fgl_ctx *ctx, int32_t *width, int32_t *height) {
+    if (check_ctx(ctx)) return FGL_E_INVALID;
+    if (width) *width = ctx->W;
+    if (height) *height = ctx->H;
+    return FGL_OK;
+}
+
+int fgl_clear_color(fgl_ctx *ctx, const uint8_t nrgba[4]) {
+    if (check_ctx(ctx)) return FGL_E_INVALID;
+    if (!nrgba) {
+        ctx->err = "fgl_clear_color: null color";
+        return FGL_E_INVALID;
+    }
+    ctx->clear_color = (uint32_t)nrgba[0] | ((uint32_t)nrgba[1] << 8) | ((uint32_t)nrgba[2] << 16) |
+                       ((uint32_t)nrgba[3] << 24);
+    ctx->pend_color = true;
+    return FGL_OK;
+}
+
+int fgl_clear_depth(fgl_ctx *ctx, double value) {
+    if (check_ctx(ctx)) return FGL_E_INVALID;
+    ctx->clear_depth = value;
+    ctx->pend_depth = true;
+    return FGL_OK;
+}
+
+int fgl_texture_create(fgl_ctx *ctx, int32_t width, int32_t height, int32_t kind, const void *texels,
+                       int64_t *handle) {
+    if (check_ctx(ctx)) return FGL_E_INVALID;
+    if (!texels || !handle || width <= 0 || height <= 0 ||
+        (kind != FGL_TEX_NRGBA8 && kind != FGL_TEX_RGBA8 && kind != FGL_TEX_RGBA16)) {
+        ctx->err = "fgl_texture_create: invalid argument";
+        return FGL_E_INVALID;
+    }
+    size_t bytes = (size_t)width * height * (kind == FGL_TEX_RGBA16 ? 8 : 4);
+    Tex t;
+    t.w = width;
+    t.h = height;
+    t.kind = kind;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipMalloc(&t.d, bytes));
+    hipError_t e = hipMemcpy(t.d, texels, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(t.d);
+        set_err(ctx->err, "texture upload failed: %s", hipGetErrorString(e));
+        return FGL_E_DEVICE;
+    }
+    int64_t h = ctx->next_tex++;
+    ctx->tex[h] = t;
+    *handle = h;
+    return FGL_OK;
+}
+
+int fgl_texture_destroy(fgl_ctx *ctx, int64_t handle) {
+    if (check_ctx(ctx)) return FGL_E_INVALID;
+    auto it = ctx->tex.find(handle);
+    if (it == ctx->tex.end()) {
+        ctx->err = "fgl_texture_destroy: unknown handle";
+        return FGL_E_INVALID;
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(it->second.d);
+    ctx->tex.erase(it);
+    return FGL_OK;
+}
+
+static int draw_host(fgl_ctx *ctx, const fgl_state *state, const fgl_shader *shader, const fgl_prims *p, int kind,
+                     uint64_t info[2]) {
+    if (check_ctx(ctx)) return FGL_E_INVALID;
+    if (!state || !shader || !p) {
+        ctx->err = "draw: null state, shader or primitives";
+        return FGL_E_INVALID;
+    }
+    if (p->count < 0 || p->count > kMaxPrims) {
+        set_err(ctx->err, "draw: primitive count %lld outside [0, %lld]", (long long)p->count, (long long)kMaxPrims);
+        return FGL_E_INVALID;
+    }
+    if (p->count > 0 && !p->position) {
+        ctx->err = "draw: position array is required";
+        return FGL_E_INVALID;
+    }
+    if (hipSetDevice(ctx->device) != hipSuccess) {
+        ctx->err = "hipSetDevice failed";
+        return FGL_E_DEVICE;
+    }
+    const size_t nv = (size_t)p->count * (kind == PRIM_TRI ? 3 : 2);
+    DevPrims in;
+    in.n = p->count;
+    int r;
+    if ((r = upload(ctx, ctx->in_pos, p->position, nv * 3, in.pos))) return r;
+    if ((r = upload(ctx, ctx->in_nrm, p->normal, nv * 3, in.nrm))) return r;
+    if ((r = upload(ctx, ctx->in_uv, p->texture, nv * 2, in.uv))) return r;
+    if ((r = upload(ctx, ctx->in_col, p->color, nv * 4, in.col))) return r;
+    return run_draw(ctx, state, shader, kind, in, info);
+}
+
+int fgl_draw_triangles(fgl_ctx *ctx, const fgl_state *state, const fgl_shader *shader, const fgl_prims *tris,
+                       uint64_t info[2]) {
+    return draw_host(ctx, state, shader, tris, PRIM_TRI, info);
+}
+
+int fgl_draw_lines(fgl_ctx *ctx, const fgl_state *state, const fgl_shader *shader, const fgl_prims *lines,
+                   uint64_t info[2]) {
+    return draw_host(ctx, state, shader, lines, PRIM_LINE, info);
+}
+
+static int mesh_upload(fgl_ctx *c, DevBuf &b, const double *src, size_t count, bool &has) {
+    has = false;
+    if (!src || count == 0) return FGL_OK;
+    HIPCHK(c, b.ensure(count * 8));
+    HIPCHK(c, hipMemcpy(b.p, src, count * 8, hipMemcpyHostToDevice));
+    has = true;
+    return FGL_OK;
+}
+
+int fgl_mesh_create(fgl_ctx *ctx, const fgl_prims *tris, const fgl_prims *lines, fgl_mesh **out) {
+    if (check_ctx(ctx)) return FGL_E_INVALID;
+    if (!out) {
+        ctx->err = "fgl_mesh_create: null output";
+        return FGL_E_INVALID;
+    }
+    *out = nullptr;
+    int64_t nt = tris ? tris->count : 0, nl = lines ? lines->count : 0;
+    if (nt < 0 || nl < 0 || nt > kMaxPrims || nl > kMaxPrims || (nt > 0 && !tris->position) ||
+        (nl > 0 && !lines->position)) {
+        ctx->err = "fgl_mesh_create: invalid primitives";
+        return FGL_E_INVALID;
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    fgl_mesh *m = new (std::nothrow) fgl_mesh;
+    if (!m) {
+        ctx->err = "out of host memory";
+        return FGL_E_NOMEM;
+    }
+    m->ctx = ctx;
+    m->ntri = nt;
+    m->nline = nl;
+    int r = FGL_OK;
+    if (nt > 0) {
+        size_t nv = (size_t)nt * 3;
+        if (!r) r = mesh_upload(ctx, m->tpos, tris->position, nv * 3, m->t_has[0]);
+        if (!r) r = mesh_upload(ctx, m->tnrm, tris->normal, nv * 3, m->t_has[1]);
+        if (!r) r = mesh_upload(ctx, m->tuv, tris->texture, nv * 2, m->t_has[2]);
+        if (!r) r = mesh_upload(ctx, m->tcol, tris->color, nv * 4, m->t_has[3]);
+    }
+    if (nl > 0) {
+        size_t nv = (size_t)nl * 2;
+        if (!r) r = mesh_upload(ctx, m->lpos, lines->position, nv * 3, m->l_has[0]);
+        if (!r) r = mesh_upload(ctx, m->lnrm, lines->normal, nv * 3, m->l_has[1]);
+        if (!r) r = mesh_upload(ctx, m->luv, lines->texture, nv * 2, m->l_has[2]);
+        if (!r) r = mesh_upload(ctx, m->lcol, lines->color, nv * 4, m->l_has[3]);
+    }
+    if (r) {
+        delete m;
+        return r;
+    }
+    *out = m;
+    return FGL_OK;
+}
+
+void fgl_mesh_destroy(fgl_mesh *mesh) {
+    if (!mesh) return;
+    (void)hipSetDevice(mesh->ctx->device);
+    (void)hipStreamSynchronize(mesh->ctx->stream);
+    delete mesh;
+}
+
+int fgl_draw_mesh(fgl_ctx *ctx, const fgl_state *state, const fgl_shader *shader, const fgl_mesh *mesh,
+                  uint64_t info[2]) {
+    if (check_ctx(ctx)) return FGL_E_INVALID;
+    if (!state || !shader || !mesh) {
+        ctx->err = "fgl_draw_mesh: null argument";
+        return FGL_E_INVALID;
+    }
+    if (mesh->ctx->device != ctx->device) {
+        ctx->err = "fgl_draw_mesh: mesh lives on another device";
+        return FGL_E_INVALID;
+    }
+    uint64_t i1[2] = {0, 0}, i2[2] = {0, 0};
+    DevPrims t;
+    t.n = mesh->ntri;
+    t.pos = mesh->t_has[0] ? mesh->tpos.as<double>() : nullptr;
+    t.nrm = mesh->t_has[1] ? mesh->tnrm.as<double>() : nullptr;
+    t.uv = mesh->t_has[2] ? mesh->tuv.as<double>() : nullptr;
+    t.col = mesh->t_has[3] ? mesh->tcol.as<double>() : nullptr;
+    int r = run_draw(ctx, state, shader, PRIM_TRI, t, i1);
+    if (r) return r;
+    DevPrims l;
+    l.n = mesh->nline;
+    l.pos = mesh->l_has[0] ? mesh->lpos.as<double>() : nullptr;
+    l.nrm = mesh->l_has[1] ? mesh->lnrm.as<double>() : nullptr;
+    l.uv = mesh->l_has[2] ? mesh->luv.as<double>() : nullptr;
+    l.col = mesh->l_has[3] ? mesh->lcol.as<double>() : nullptr;
+    r = run_draw(ctx, state, shader, PRIM_LINE, l, i2);
+    if (r) return r;
+    if (info) {
+        info[0] = i1[0] + i2[0];
+        info[1] = i1[1] + i2[1];
+    }
+    return FGL_OK;
+}
+
+int fgl_read_color(fgl_ctx *ctx, uint8_t *nrgba) {
+    if (check_ctx(ctx)) return FGL_E_INVALID;
+    if (!nrgba) {
+        ctx->err = "fgl_read_color: null buffer";
+        return FGL_E_INVALID;
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    int r = materialize(ctx);
+    if (r) return r;
+    HIPCHK(ctx, hipMemcpyAsync(nrgba, ctx->color.p, (size_t)ctx->W * ctx->H * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->prof) prof_collect(ctx);
+    return FGL_OK;
+}
+
+int fgl_read_depth(fgl_ctx *ctx, double *depth) {
+    if (check_ctx(ctx)) return FGL_E_INVALID;
+    if (!depth) {
+        ctx->err = "fgl_read_depth: null buffer";
+        return FGL_E_INVALID;
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    int r = materialize(ctx);
+    if (r) return r;
+    HIPCHK(ctx, hipMemcpyAsync(depth, ctx->depth.p, (size_t)ctx->W * ctx->H * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->prof) prof_collect(ctx);
+    return FGL_OK;
+}
+
+int fgl_write_color(fgl_ctx *ctx, const uint8_t *nrgba) {
+    if (check_ctx(ctx)) return FGL_E_INVALID;
+    if (!nrgba) {
+        ctx->err = "fgl_write_color: null buffer";
+        return FGL_E_INVALID;
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->color.p, nrgba, (size_t)ctx->W * ctx->H * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->pend_color = false;
+    return FGL_OK;
+}
+
+int fgl_write_depth(fgl_ctx *ctx, const double *depth) {
+    if (check_ctx(ctx)) return FGL_E_INVALID;
+    if (!depth) {
+        ctx->err = "fgl_write_depth: null buffer";
+        return FGL_E_INVALID;
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->depth.p, depth, (size_t)ctx->W * ctx->H * 8, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->pend_depth = false;
+    return FGL_OK;
+}
+
+int fgl_depth_image(fgl_ctx *ctx, uint16_t *gray16) {
+    if (check_ctx(ctx)) return FGL_E_INVALID;
+    if (!gray16) {
+        ctx->err = "fgl_depth_image: null buffer";
+        return FGL_E_INVALID;
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    int r = materialize(ctx);
+    if (r) return r;
+    const int64_t n = (int64_t)ctx->W * ctx->H;
+    const int32_t nb = 512;
+    HIPCHK(ctx, ctx->dpart.ensure((size_t)nb * 16));
+    HIPCHK(ctx, ctx->dimg.ensure((size_t)n * 2));
+    r = launch(ctx, "depth_minmax", [&] { return launch_depth_minmax(ctx->depth.as<double>(), n, ctx->dpart.as<double>(), nb, ctx->stream); });
+    if (r) return r;
+    r = launch(ctx, "depth_image", [&] { return launch_depth_image(ctx->depth.as<double>(), n, ctx->dpart.as<double>(), nb, ctx->dimg.as<uint16_t>(), ctx->stream); });
+    if (r) return r;
+    HIPCHK(ctx, hipMemcpyAsync(gray16, ctx->dimg.p, (size_t)n * 2, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->prof) prof_collect(ctx);
+    return FGL_OK;
+}
+
+// Tent weights: the linear kernel widened by the scale factor, 8-bit fixed
+// point, as the Bilinear resize the reference pipeline applies after
+// rendering (README.md:109).
+static void tent_weights(int32_t src, int32_t dst, std::vector<int16_t> &w, std::vector<int32_t> &off, int32_t &fl) {
+    const double blur = 1.0;
+    double scale = (double)src / (double)dst;
+    fl = 2 * (int32_t)std::max(std::ceil(blur * scale), 1.0);
+    double factor = std::min(1.0 / (blur * scale), 1.0);
+    w.assign((size_t)dst * fl, 0);
+    off.assign((size_t)dst, 0);
+    for (int32_t y = 0; y < dst; y++) {
+        double interp = scale * ((double)y + 0.5) - 0.5;
+        int32_t start = (int32_t)interp - fl / 2 + 1;
+        off[y] = start;
+        interp -= (double)start;
+        for (int32_t i = 0; i < fl; i++) {
+            double in = (interp - (double)i) * factor;
+            double a = std::fabs(in);
+            double k = a <= 1 ? 1 - a : 0;
+            w[(size_t)y * fl + i] = (int16_t)(k * 256);
+        }
+    }
+}
+
+int fgl_resolve(fgl_ctx *src, fgl_ctx *dst, int32_t filter) {
+    if (check_ctx(src) || check_ctx(dst)) return FGL_E_INVALID;
+    if (src->device != dst->device) {
+        src->err = "fgl_resolve: contexts on different devices";
+        return FGL_E_INVALID;
+    }
+    if (dst->W <= 0 || src->W % dst->W || src->H % dst->H || src->W / dst->W != src->H / dst->H) {
+        set_err(src->err, "fgl_resolve: %dx%d is not an integer multiple of %dx%d", src->W, src->H, dst->W, dst->H);
+        return FGL_E_INVALID;
+    }
+    if (filter != FGL_RESOLVE_TENT && filter != FGL_RESOLVE_BOX) {
+        src->err = "fgl_resolve: unknown filter";
+        return FGL_E_INVALID;
+    }
+    HIPCHK(src, hipSetDevice(src->device));
+    int r = materialize(src);
+    if (r) return r;
+    HIPCHK(dst, hipStreamSynchronize(dst->stream));
+    int32_t flx = 0, fly = 0;
+    if (filter == FGL_RESOLVE_TENT) {
+        std::vector<int16_t> wx, wy;
+        std::vector<int32_t> ox, oy;
+        tent_weights(src->W, dst->W, wx, ox, flx);
+        tent_weights(src->H, dst->H, wy, oy, fly);
+        HIPCHK(src, src->rs_wx.ensure(wx.size() * 2));
+        HIPCHK(src, src->rs_ox.ensure(ox.size() * 4));
+        HIPCHK(src, src->rs_wy.ensure(wy.size() * 2));
+        HIPCHK(src, src->rs_oy.ensure(oy.size() * 4));
+        HIPCHK(src, hipMemcpyAsync(src->rs_wx.p, wx.data(), wx.size() * 2, hipMemcpyHostToDevice, src->stream));
+        HIPCHK(src, hipMemcpyAsync(src->rs_ox.p, ox.data(), ox.size() * 4, hipMemcpyHostToDevice, src->stream));
+        HIPCHK(src, hipMemcpyAsync(src->rs_wy.p, wy.data(), wy.size() * 2, hipMemcpyHostToDevice, src->stream));
+        HIPCHK(src, hipMemcpyAsync(src->rs_oy.p, oy.data(), oy.size() * 4, hipMemcpyHostToDevice, src->stream));
+        HIPCHK(src, src->rs_tmp.ensure((size_t)dst->W * src->H * 4));
+    }
+    r = launch(src, "resolve", [&] {
+        return launch_resolve(src->color.as<uint32_t>(), src->W, src->H, src->rs_tmp.as<uint32_t>(),
+                              dst->color.as<uint32_t>(), dst->W, dst->H, filter, src->rs_wx.as<int16_t>(),
+                              src->rs_ox.as<int32_t>(), flx, src->rs_wy.as<int16_t>(), src->rs_oy.as<int32_t>(), fly,
+                              src->stream);
+    });
+    if (r) return r;
+    HIPCHK(src, hipStreamSynchronize(src->stream));
+    if (src->prof) prof_collect(src);
+    dst->pend_color = false;
+    return FGL_OK;
+}
+
+int fgl_profile_enable(fgl_ctx *ctx, int32_t enable) {
+    if (check_ctx(ctx)) return FGL_E_INVALID;
+    ctx->prof = enable != 0;
+    return FGL_OK;
+}
+
+int fgl_profile_read(fgl_ctx *ctx, fgl_kernel_time *out, int32_t cap, int32_t *n) {
+    if (check_ctx(ctx)) return FGL_E_INVALID;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    prof_collect(ctx);
+    int32_t k = 0;
+    for (auto &kv : ctx->kstats) {
+        if (out && k < cap) {
+            std::memset(out[k].name, 0, sizeof out[k].name);
+            std::strncpy(out[k].name, kv.first.c_str(), sizeof out[k].name - 1);
+            out[k].launches = kv.second.launches;
+            out[k].total_ms = kv.second.ms;
+        }
+        k++;
+    }
+    if (n) *n = k;
+    return FGL_OK;
+}
+
+int fgl_profile_reset(fgl_ctx *ctx) {
+    if (check_ctx(ctx)) return FGL_E_INVALID;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    prof_collect(ctx);
+    ctx->kstats.clear();
+    return FGL_OK;
+}
+
+int fgl_ctx_sync(fgl_ctx *ctx) {
+    if (check_ctx(ctx)) return FGL_E_INVALID;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->prof) prof_collect(ctx);
+    return FGL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,89 @@
+// fgl_internal.h -- kernel argument blocks and launch entry points shared by
+// the HIP translation units and the host runtime (fgl_host.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fgl_device.h"
+
+namespace fgl {
+
+enum PrimKind : int32_t { PRIM_TRI = 0, PRIM_LINE = 1 };
+
+// Input soup on the device (SoA, see fgl_prims in include/fgl.h).
+struct DevPrims {
+    int64_t n;
+    const double *pos, *nrm, *uv, *col;
+};
+
+// Everything a draw's kernels share.
+struct DrawArgs {
+    int32_t W, H;          // framebuffer size
+    int32_t ntx, nty;      // tiles per row / column
+    int32_t ntiles;        // ntx * nty
+    int32_t kind;          // PrimKind
+    DevPrims in;
+    DevState st;
+    DevShader sh;
+    double S[16];          // Screen(W, H) (matrix.go:119-128)
+
+    // vertex stage outputs
+    PrimRec *rec_main;     // in.n records (triangles drawn without clipping)
+    PrimRec *side_rec;     // raster triangles from clipping, wireframe and lines
+    uint32_t side_rec_cap;
+    double *side_vtx;      // clipped vertices, SIDE_VTX_DOUBLES each
+    uint32_t side_vtx_cap;
+    uint32_t *ntl;         // per input primitive: (tile, triangle) pairs it emits
+    uint32_t *nrast;       // per input primitive: raster triangles (bit 31: side records)
+    uint32_t *sstart;      // per input primitive: first side record
+    uint64_t *blocksum;    // per vertex-stage block: sum of ntl
+    uint64_t *blockoff;    // exclusive prefix of blocksum
+    int32_t nvsblocks;
+
+    // binning
+    uint32_t *keys[2], *vals[2];  // ping-pong (tile id, triangle ref) pairs
+    uint32_t pair_cap;
+    uint32_t *hist;        // radix histograms, 256 * nrsblocks
+    int32_t nrsblocks;
+    int32_t key_bits;
+    uint32_t *tstart, *tend;  // per tile range in the sorted pairs
+    int32_t sorted_buf;    // which of keys/vals holds the sorted result
+
+    // framebuffer and deferred clears
+    uint32_t *color;       // NRGBA8 as little-endian u32
+    double *depth;
+    int32_t clear_color_pending, clear_depth_pending;
+    uint32_t clear_color;
+    double clear_depth;
+
+    DrawCounters *ctr;
+};
+
+constexpr int VS_THREADS = 256;
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_BLOCK = RS_THREADS * RS_ITEMS;
+
+// Kernel launchers (stream-ordered).  Each returns the hipError_t of the launch.
+
+hipError_t launch_zero(const DrawArgs &a, hipStream_t s);
+hipError_t launch_vs(const DrawArgs &a, hipStream_t s);
+hipError_t launch_scan_top(const DrawArgs &a, hipStream_t s);
+hipError_t launch_emit(const DrawArgs &a, hipStream_t s);
+hipError_t launch_radix_hist(const DrawArgs &a, int pass, hipStream_t s);
+hipError_t launch_radix_scan(const DrawArgs &a, hipStream_t s);
+hipError_t launch_radix_scatter(const DrawArgs &a, int pass, hipStream_t s);
+hipError_t launch_ranges(const DrawArgs &a, hipStream_t s);
+hipError_t launch_tile(const DrawArgs &a, hipStream_t s);
+
+hipError_t launch_fill_color(uint32_t *color, int64_t n, uint32_t v, hipStream_t s);
+hipError_t launch_fill_depth(double *depth, int64_t n, double v, hipStream_t s);
+hipError_t launch_resolve(const uint32_t *src, int32_t sw, int32_t sh, uint32_t *tmp, uint32_t *dst, int32_t dw,
+                          int32_t dh, int32_t filter, const int16_t *wx, const int32_t *ox, int32_t flx,
+                          const int16_t *wy, const int32_t *oy, int32_t fly, hipStream_t s);
+hipError_t launch_depth_minmax(const double *depth, int64_t n, double *partial, int32_t nblocks, hipStream_t s);
+hipError_t launch_depth_image(const double *depth, int64_t n, const double *partial, int32_t nblocks,
+                              uint16_t *out, hipStream_t s);
+
+}  // namespace fgl

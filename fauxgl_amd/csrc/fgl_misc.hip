@@ -1,0 +1,196 @@
+// fgl_misc.hip -- framebuffer fills (deferred clears made concrete), the
+// SSAA resolve, and DepthImage.
+#include "fgl_internal.h"
+
+namespace fgl {
+
+// ClearColorBufferWith / ClearDepthBufferWith (context.go:119-141)
+__global__ void k_fill_color(uint32_t *color, int64_t n, uint32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        color[i] = v;
+}
+__global__ void k_fill_depth(double *depth, int64_t n, double v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        depth[i] = v;
+}
+
+static unsigned grid_for(int64_t n) {
+    int64_t b = (n + 255) / 256;
+    if (b > 8192) b = 8192;
+    if (b < 1) b = 1;
+    return (unsigned)b;
+}
+
+hipError_t launch_fill_color(uint32_t *color, int64_t n, uint32_t v, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_color, dim3(grid_for(n)), dim3(256), 0, s, color, n, v);
+    return hipGetLastError();
+}
+hipError_t launch_fill_depth(double *depth, int64_t n, double v, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_depth, dim3(grid_for(n)), dim3(256), 0, s, depth, n, v);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// SSAA resolve.  Tent: separable, one pass per axis with 8-bit fixed-point
+// weights int16(k(t) * 256) of the linear kernel widened by the scale
+// factor, clamped edge taps and truncating division by the weight sum; the
+// horizontal pass premultiplies NRGBA by alpha.  Box: mean of each f x f
+// block, rounded to nearest.
+
+__global__ void k_resolve_h(const uint32_t *src, int32_t sw, int32_t sh, uint32_t *tmp, int32_t dw,
+                            const int16_t *wx, const int32_t *ox, int32_t flx) {
+    // tmp is dw x sh, premultiplied RGBA8
+    int64_t n = (int64_t)dw * sh;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int32_t x = (int32_t)(i % dw), y = (int32_t)(i / dw);
+        const uint32_t *row = src + (int64_t)y * sw;
+        int32_t acc[4] = {0, 0, 0, 0};
+        int32_t sum = 0;
+        int32_t start = ox[x];
+        for (int32_t k = 0; k < flx; k++) {
+            int32_t c = wx[x * flx + k];
+            if (c == 0) continue;
+            int32_t xi = start + k;
+            if (xi < 0) xi = 0;
+            if (xi >= sw - 1) xi = sw - 1;
+            uint32_t p = row[xi];
+            int32_t a = (int32_t)(p >> 24);
+            acc[0] += c * ((int32_t)(p & 0xff) * a / 0xff);
+            acc[1] += c * ((int32_t)((p >> 8) & 0xff) * a / 0xff);
+            acc[2] += c * ((int32_t)((p >> 16) & 0xff) * a / 0xff);
+            acc[3] += c * a;
+            sum += c;
+        }
+        uint32_t o = 0;
+        for (int ch = 0; ch < 4; ch++) {
+            int32_t v = sum ? acc[ch] / sum : 0;
+            v = v < 0 ? 0 : (v > 255 ? 255 : v);
+            o |= (uint32_t)v << (8 * ch);
+        }
+        tmp[i] = o;
+    }
+}
+
+__global__ void k_resolve_v(const uint32_t *tmp, int32_t dw, int32_t sh, uint32_t *dst, int32_t dh,
+                            const int16_t *wy, const int32_t *oy, int32_t fly) {
+    int64_t n = (int64_t)dw * dh;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int32_t x = (int32_t)(i % dw), y = (int32_t)(i / dw);
+        int32_t acc[4] = {0, 0, 0, 0};
+        int32_t sum = 0;
+        int32_t start = oy[y];
+        for (int32_t k = 0; k < fly; k++) {
+            int32_t c = wy[y * fly + k];
+            if (c == 0) continue;
+            int32_t yi = start + k;
+            if (yi < 0) yi = 0;
+            if (yi >= sh - 1) yi = sh - 1;
+            uint32_t p = tmp[(int64_t)yi * dw + x];
+            for (int ch = 0; ch < 4; ch++) acc[ch] += c * (int32_t)((p >> (8 * ch)) & 0xff);
+            sum += c;
+        }
+        uint32_t o = 0;
+        for (int ch = 0; ch < 4; ch++) {
+            int32_t v = sum ? acc[ch] / sum : 0;
+            v = v < 0 ? 0 : (v > 255 ? 255 : v);
+            o |= (uint32_t)v << (8 * ch);
+        }
+        dst[i] = o;
+    }
+}
+
+__global__ void k_resolve_box(const uint32_t *src, int32_t sw, uint32_t *dst, int32_t dw, int32_t dh, int32_t f) {
+    int64_t n = (int64_t)dw * dh;
+    const int32_t cnt = f * f;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int32_t x = (int32_t)(i % dw), y = (int32_t)(i / dw);
+        uint32_t acc[4] = {0, 0, 0, 0};
+        for (int32_t dy = 0; dy < f; dy++) {
+            const uint32_t *row = src + (int64_t)(y * f + dy) * sw + (int64_t)x * f;
+            for (int32_t dx = 0; dx < f; dx++) {
+                uint32_t p = row[dx];
+                for (int ch = 0; ch < 4; ch++) acc[ch] += (p >> (8 * ch)) & 0xff;
+            }
+        }
+        uint32_t o = 0;
+        for (int ch = 0; ch < 4; ch++) o |= ((acc[ch] + (uint32_t)cnt / 2) / (uint32_t)cnt) << (8 * ch);
+        dst[i] = o;
+    }
+}
+
+hipError_t launch_resolve(const uint32_t *src, int32_t sw, int32_t sh, uint32_t *tmp, uint32_t *dst, int32_t dw,
+                          int32_t dh, int32_t filter, const int16_t *wx, const int32_t *ox, int32_t flx,
+                          const int16_t *wy, const int32_t *oy, int32_t fly, hipStream_t s) {
+    if (filter == 2) {
+        hipLaunchKernelGGL(k_resolve_box, dim3(grid_for((int64_t)dw * dh)), dim3(256), 0, s, src, sw, dst, dw, dh,
+                           sw / dw);
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(k_resolve_h, dim3(grid_for((int64_t)dw * sh)), dim3(256), 0, s, src, sw, sh, tmp, dw, wx, ox,
+                       flx);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_resolve_v, dim3(grid_for((int64_t)dw * dh)), dim3(256), 0, s, tmp, dw, sh, dst, dh, wy, oy,
+                       fly);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// DepthImage (context.go:87-117): lo/hi over depths != MaxFloat64, then
+// uint16(t * 0xffff) with t = (d - lo) / (hi - lo) (t = 1 for cleared).
+
+constexpr double kMaxFloat64 = 1.79769313486231570814527423731704356798070e+308;
+
+__global__ __launch_bounds__(256) void k_depth_minmax(const double *depth, int64_t n, double *partial) {
+    __shared__ double slo[256], shi[256];
+    double lo = kMaxFloat64, hi = -kMaxFloat64;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double d = depth[i];
+        if (d == kMaxFloat64) continue;
+        if (d < lo) lo = d;
+        if (d > hi) hi = d;
+    }
+    slo[threadIdx.x] = lo;
+    shi[threadIdx.x] = hi;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            if (slo[threadIdx.x + off] < slo[threadIdx.x]) slo[threadIdx.x] = slo[threadIdx.x + off];
+            if (shi[threadIdx.x + off] > shi[threadIdx.x]) shi[threadIdx.x] = shi[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = slo[0];
+        partial[2 * blockIdx.x + 1] = shi[0];
+    }
+}
+
+__global__ void k_depth_image(const double *depth, int64_t n, const double *partial, int32_t nblocks, uint16_t *out) {
+    double lo = kMaxFloat64, hi = -kMaxFloat64;
+    for (int32_t b = 0; b < nblocks; b++) {
+        double l = partial[2 * b], h = partial[2 * b + 1];
+        if (l < lo) lo = l;
+        if (h > hi) hi = h;
+    }
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double d = depth[i];
+        double t = (d - lo) / (hi - lo);
+        if (d == kMaxFloat64) t = 1;
+        double v = t * 65535.0;
+        uint32_t u = (v > -2147483649.0 && v < 2147483648.0) ? (uint32_t)(int32_t)v : 0u;
+        out[i] = (uint16_t)(u & 0xffffu);
+    }
+}
+
+hipError_t launch_depth_minmax(const double *depth, int64_t n, double *partial, int32_t nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_depth_minmax, dim3(nblocks), dim3(256), 0, s, depth, n, partial);
+    return hipGetLastError();
+}
+hipError_t launch_depth_image(const double *depth, int64_t n, const double *partial, int32_t nblocks, uint16_t *out,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_depth_image, dim3(grid_for(n)), dim3(256), 0, s, depth, n, partial, nblocks, out);
+    return hipGetLastError();
+}
+
+}  // namespace fgl

@@ -1,0 +1,421 @@
+// fgl_vs.hip -- vertex stage: vertex shader, view-volume clipping, culling,
+// screen mapping and raster-triangle records, one thread per input
+// triangle or line.  Restates DrawTriangle / DrawLine (context.go:351-389),
+// ClipTriangle / ClipLine (clipping.go:54-98), drawClippedTriangle /
+// drawClippedLine (context.go:303-349) and line / wireframe
+// (context.go:283-301).  Every expression keeps the reference's operation
+// order; the file is compiled with -ffp-contract=off.
+#include "fgl_internal.h"
+#include "fgl_tiles.h"
+
+namespace fgl {
+
+// A vertex as the pipeline carries it (vertex.go:3-12; Texture.Z dropped,
+// no built-in shader reads it).
+struct Vtx {
+    V3 pos, nrm;
+    double u, v;
+    C4 col;
+    V4 out;
+};
+
+FGL_D Vtx load_vertex(const DevPrims &in, int64_t vid) {
+    Vtx o;
+    o.pos = in.pos ? v3(in.pos[3 * vid], in.pos[3 * vid + 1], in.pos[3 * vid + 2]) : v3(0, 0, 0);
+    o.nrm = in.nrm ? v3(in.nrm[3 * vid], in.nrm[3 * vid + 1], in.nrm[3 * vid + 2]) : v3(0, 0, 0);
+    if (in.uv) {
+        o.u = in.uv[2 * vid];
+        o.v = in.uv[2 * vid + 1];
+    } else {
+        o.u = 0;
+        o.v = 0;
+    }
+    o.col = in.col ? c4(in.col[4 * vid], in.col[4 * vid + 1], in.col[4 * vid + 2], in.col[4 * vid + 3])
+                   : c4(0, 0, 0, 0);
+    o.out = v4(0, 0, 0, 0);
+    return o;
+}
+
+FGL_D V3 load_pos(const DevPrims &in, int64_t vid) {
+    return in.pos ? v3(in.pos[3 * vid], in.pos[3 * vid + 1], in.pos[3 * vid + 2]) : v3(0, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// Raster triangle records
+
+// Writes (when out != nullptr) the record of one rasterize() call and
+// reports how many tiles it is binned to.  Returns false when no pixel can
+// result (the box is pathological, or no index y*W+x of it lands in the
+// framebuffer); such a call has no effect in the reference either.
+FGL_D bool make_raster(const DrawArgs &a, V3 s0, V3 s1, V3 s2, double W0, double W1, double W2, uint64_t attr,
+                       PrimRec *out, uint32_t &ntiles) {
+    // integer bounding box (context.go:155-160)
+    double mnx = __builtin_floor(go_min(s0.x, go_min(s1.x, s2.x)));
+    double mny = __builtin_floor(go_min(s0.y, go_min(s1.y, s2.y)));
+    double mxx = __builtin_ceil(go_max(s0.x, go_max(s1.x, s2.x)));
+    double mxy = __builtin_ceil(go_max(s0.y, go_max(s1.y, s2.y)));
+    int64_t x0 = go_f2i(mnx), x1 = go_f2i(mxx), y0 = go_f2i(mny), y1 = go_f2i(mxy);
+    if (x0 < -BBOX_LIM || x0 > BBOX_LIM || x1 < -BBOX_LIM || x1 > BBOX_LIM || y0 < -BBOX_LIM || y0 > BBOX_LIM ||
+        y1 < -BBOX_LIM || y1 > BBOX_LIM)
+        return false;
+    TileRect r;
+    if (!prim_tile_rect((int32_t)x0, (int32_t)y0, (int32_t)x1, (int32_t)y1, a.W, a.H, r)) return false;
+    ntiles = (uint32_t)(r.tx1 - r.tx0 + 1) * (uint32_t)(r.ty1 - r.ty0 + 1);
+    if (out) {
+        out->sx[0] = s0.x; out->sx[1] = s1.x; out->sx[2] = s2.x;
+        out->sy[0] = s0.y; out->sy[1] = s1.y; out->sy[2] = s2.y;
+        out->sz[0] = s0.z; out->sz[1] = s1.z; out->sz[2] = s2.z;
+        out->r[0] = 1.0 / W0;
+        out->r[1] = 1.0 / W1;
+        out->r[2] = 1.0 / W2;
+        out->attr = attr;
+        out->x0 = (int32_t)x0; out->y0 = (int32_t)y0;
+        out->x1 = (int32_t)x1; out->y1 = (int32_t)y1;
+        out->pad = 0;
+    }
+    return true;
+}
+
+// Collects the raster triangles of one input primitive, counting on the
+// first pass and writing on the second.
+struct Emitter {
+    PrimRec *base;  // nullptr on the counting pass
+    uint32_t n;
+    uint32_t ntiles;
+    FGL_D void add(const DrawArgs &a, V3 s0, V3 s1, V3 s2, double W0, double W1, double W2, uint64_t attr) {
+        uint32_t t = 0;
+        if (make_raster(a, s0, s1, s2, W0, W1, W2, attr, base ? base + n : nullptr, t)) {
+            n++;
+            ntiles += t;
+        }
+    }
+};
+
+// Vector.Perpendicular (vector.go:179-187)
+FGL_D V3 perpendicular(V3 a) {
+    if (a.x == 0 && a.y == 0) {
+        if (a.z == 0) return v3(0, 0, 0);
+        return v3(0, 1, 0);
+    }
+    return vnormalize(v3(-a.y, a.x, 0));
+}
+
+// line() (context.go:283-294) between vertices with selectors sa, sb of
+// attribute source (kind, index).
+FGL_D void emit_line(const DrawArgs &a, Emitter &em, uint32_t kind, uint64_t index, uint32_t sa, uint32_t sb,
+                     V3 s0, V3 s1, double Wa, double Wb) {
+    double half = a.st.line_width / 2;
+    V3 n = vmuls(perpendicular(vsub(s1, s0)), a.st.line_width / 2);
+    s0 = vadd(s0, vmuls(vnormalize(vsub(s0, s1)), half));
+    s1 = vadd(s1, vmuls(vnormalize(vsub(s1, s0)), half));
+    V3 s00 = vadd(s0, n);
+    V3 s01 = vsub(s0, n);
+    V3 s10 = vadd(s1, n);
+    V3 s11 = vsub(s1, n);
+    em.add(a, s11, s01, s00, Wb, Wa, Wa, make_attr(kind, sb, sa, sa, index));
+    em.add(a, s10, s11, s00, Wb, Wb, Wa, make_attr(kind, sb, sb, sa, index));
+}
+
+// drawClippedTriangle (context.go:316-349) for three vertex outputs whose
+// attributes are vertices sel[0..2] of source (kind, index).
+FGL_D void emit_clipped_triangle(const DrawArgs &a, Emitter &em, uint32_t kind, uint64_t index, const uint32_t sel[3],
+                                 V4 o0, V4 o1, V4 o2) {
+    // normalized device coordinates (Output.DivScalar(W).Vector())
+    V3 n0 = v3(o0.x / o0.w, o0.y / o0.w, o0.z / o0.w);
+    V3 n1 = v3(o1.x / o1.w, o1.y / o1.w, o1.z / o1.w);
+    V3 n2 = v3(o2.x / o2.w, o2.y / o2.w, o2.z / o2.w);
+    // back face culling
+    double ar = (n1.x - n0.x) * (n2.y - n0.y) - (n2.x - n0.x) * (n1.y - n0.y);
+    uint32_t q0 = sel[0], q1 = sel[1], q2 = sel[2];
+    double w0 = o0.w, w1 = o1.w, w2 = o2.w;
+    if (ar < 0) {
+        uint32_t tq = q0; q0 = q2; q2 = tq;
+        double tw = w0; w0 = w2; w2 = tw;
+        V3 tn = n0; n0 = n2; n2 = tn;
+    }
+    if (a.st.cull == 2 /* CullFront */) ar = -ar;
+    if (a.st.front_face == 1 /* FaceCW */) ar = -ar;
+    if (a.st.cull != 1 /* CullNone */ && ar <= 0) return;
+    // screen coordinates
+    V3 s0 = mul_position(a.S, n0);
+    V3 s1 = mul_position(a.S, n1);
+    V3 s2 = mul_position(a.S, n2);
+    if (a.st.wireframe) {  // wireframe (context.go:296-301)
+        emit_line(a, em, kind, index, q0, q1, s0, s1, w0, w1);
+        emit_line(a, em, kind, index, q1, q2, s1, s2, w1, w2);
+        emit_line(a, em, kind, index, q2, q0, s2, s0, w2, w0);
+    } else {
+        em.add(a, s0, s1, s2, w0, w1, w2, make_attr(kind, q0, q1, q2, index));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Clipping (clipping.go)
+
+__constant__ double kClipP[6][4] = {{1, 0, 0, 1}, {-1, 0, 0, 1}, {0, 1, 0, 1},
+                                    {0, -1, 0, 1}, {0, 0, 1, 1}, {0, 0, -1, 1}};
+__constant__ double kClipN[6][4] = {{-1, 0, 0, 1}, {1, 0, 0, 1}, {0, -1, 0, 1},
+                                    {0, 1, 0, 1}, {0, 0, -1, 1}, {0, 0, 1, 1}};
+
+FGL_D V4 clipP(int p) { return v4(kClipP[p][0], kClipP[p][1], kClipP[p][2], kClipP[p][3]); }
+FGL_D V4 clipN(int p) { return v4(kClipN[p][0], kClipN[p][1], kClipN[p][2], kClipN[p][3]); }
+
+FGL_D bool point_in_front(int p, V4 v) { return wdot(wsub(v, clipP(p)), clipN(p)) > 0; }  // 16-18
+FGL_D V4 intersect_segment(int p, V4 v0, V4 v1) {                                        // 20-26
+    V4 u = wsub(v1, v0);
+    V4 w = wsub(v0, clipP(p));
+    V4 N = clipN(p);
+    double d = wdot(N, u);
+    double n = -wdot(N, w);
+    return wadd(v0, wmuls(u, n / d));
+}
+
+// sutherlandHodgman (clipping.go:28-52); a triangle yields at most 9 points.
+__device__ __noinline__ int sutherland_hodgman(const V4 *pts, V4 *out) {
+    V4 buf[2][10];
+    int n = 3;
+    for (int i = 0; i < 3; i++) buf[0][i] = pts[i];
+    int cur = 0;
+    for (int p = 0; p < 6; p++) {
+        if (n == 0) return 0;
+        const V4 *input = buf[cur];
+        V4 *output = buf[cur ^ 1];
+        int m = 0;
+        V4 s = input[n - 1];
+        for (int k = 0; k < n; k++) {
+            V4 e = input[k];
+            bool ef = point_in_front(p, e);
+            bool sf = point_in_front(p, s);
+            if (ef) {
+                if (!sf && m < 10) output[m++] = intersect_segment(p, s, e);
+                if (m < 10) output[m++] = e;
+            } else if (sf) {
+                if (m < 10) output[m++] = intersect_segment(p, s, e);
+            }
+            s = e;
+        }
+        n = m;
+        cur ^= 1;
+    }
+    for (int i = 0; i < n; i++) out[i] = buf[cur][i];
+    return n;
+}
+
+// Barycentric (vertex.go:81-95)
+FGL_D V4 barycentric(V3 p1, V3 p2, V3 p3, V3 p) {
+    V3 v0 = vsub(p2, p1);
+    V3 v1 = vsub(p3, p1);
+    V3 v2 = vsub(p, p1);
+    double d00 = vdot(v0, v0);
+    double d01 = vdot(v0, v1);
+    double d11 = vdot(v1, v1);
+    double d20 = vdot(v2, v0);
+    double d21 = vdot(v2, v1);
+    double d = d00 * d11 - d01 * d01;
+    double v = (d11 * d20 - d01 * d21) / d;
+    double w = (d00 * d21 - d01 * d20) / d;
+    double u = 1 - v - w;
+    return v4(u, v, w, 1);
+}
+
+// InterpolateVertexes (vertex.go:18-79) with b.W == 1, as ClipTriangle calls it.
+FGL_D double ip(double a, double b, double c, V4 k) {
+    double n = 0;
+    n = n + a * k.x;
+    n = n + b * k.y;
+    n = n + c * k.z;
+    return n * k.w;
+}
+FGL_D Vtx interpolate_clip(const Vtx *V, V4 b) {
+    Vtx o;
+    o.pos = v3(ip(V[0].pos.x, V[1].pos.x, V[2].pos.x, b), ip(V[0].pos.y, V[1].pos.y, V[2].pos.y, b),
+               ip(V[0].pos.z, V[1].pos.z, V[2].pos.z, b));
+    o.nrm = vnormalize(v3(ip(V[0].nrm.x, V[1].nrm.x, V[2].nrm.x, b), ip(V[0].nrm.y, V[1].nrm.y, V[2].nrm.y, b),
+                          ip(V[0].nrm.z, V[1].nrm.z, V[2].nrm.z, b)));
+    o.u = ip(V[0].u, V[1].u, V[2].u, b);
+    o.v = ip(V[0].v, V[1].v, V[2].v, b);
+    o.col = c4(ip(V[0].col.r, V[1].col.r, V[2].col.r, b), ip(V[0].col.g, V[1].col.g, V[2].col.g, b),
+               ip(V[0].col.b, V[1].col.b, V[2].col.b, b), ip(V[0].col.a, V[1].col.a, V[2].col.a, b));
+    o.out = v4(ip(V[0].out.x, V[1].out.x, V[2].out.x, b), ip(V[0].out.y, V[1].out.y, V[2].out.y, b),
+               ip(V[0].out.z, V[1].out.z, V[2].out.z, b), ip(V[0].out.w, V[1].out.w, V[2].out.w, b));
+    return o;
+}
+
+// Triangle.FixNormals (triangle.go:33-37, 46-58)
+FGL_D void fix_normals(Vtx *V) {
+    V3 e1 = vsub(V[1].pos, V[0].pos);
+    V3 e2 = vsub(V[2].pos, V[0].pos);
+    V3 n = vnormalize(vcross(e1, e2));
+    for (int k = 0; k < 3; k++)
+        if (V[k].nrm.x == 0 && V[k].nrm.y == 0 && V[k].nrm.z == 0) V[k].nrm = n;
+}
+
+// ClipTriangle + per-fan drawClippedTriangle.  With em.base == nullptr this
+// only counts; otherwise it also writes the fan vertices at side_vtx[vbase..].
+__device__ __noinline__ void clip_fans(const DrawArgs &a, const Vtx *V, Emitter &em, uint32_t vbase, bool write,
+                                       uint32_t *nfan_out) {
+    V4 pts[3] = {V[0].out, V[1].out, V[2].out};
+    V4 poly[10];
+    int np = sutherland_hodgman(pts, poly);
+    V3 p1 = v3(V[0].out.x, V[0].out.y, V[0].out.z);
+    V3 p2 = v3(V[1].out.x, V[1].out.y, V[1].out.z);
+    V3 p3 = v3(V[2].out.x, V[2].out.y, V[2].out.z);
+    uint32_t nfan = np >= 3 ? (uint32_t)(np - 2) : 0u;
+    if (nfan_out) *nfan_out = nfan;
+    for (int i = 2; i < np; i++) {
+        V4 b1 = barycentric(p1, p2, p3, v3(poly[0].x, poly[0].y, poly[0].z));
+        V4 b2 = barycentric(p1, p2, p3, v3(poly[i - 1].x, poly[i - 1].y, poly[i - 1].z));
+        V4 b3 = barycentric(p1, p2, p3, v3(poly[i].x, poly[i].y, poly[i].z));
+        Vtx F[3];
+        F[0] = interpolate_clip(V, b1);
+        F[1] = interpolate_clip(V, b2);
+        F[2] = interpolate_clip(V, b3);
+        fix_normals(F);  // NewTriangle
+        uint64_t vb = (uint64_t)vbase + 3u * (uint32_t)(i - 2);
+        if (write) {
+            for (int k = 0; k < 3; k++) {
+                double *d = a.side_vtx + (vb + k) * SIDE_VTX_DOUBLES;
+                d[0] = F[k].pos.x; d[1] = F[k].pos.y; d[2] = F[k].pos.z;
+                d[3] = F[k].nrm.x; d[4] = F[k].nrm.y; d[5] = F[k].nrm.z;
+                d[6] = F[k].u; d[7] = F[k].v;
+                d[8] = F[k].col.r; d[9] = F[k].col.g; d[10] = F[k].col.b; d[11] = F[k].col.a;
+            }
+        }
+        const uint32_t sel[3] = {0, 1, 2};
+        emit_clipped_triangle(a, em, ATTR_SIDE, vb, sel, F[0].out, F[1].out, F[2].out);
+    }
+}
+
+// ClipLine (clipping.go:76-98): clips the outputs only; false for nil.
+FGL_D bool clip_line(V4 &w1, V4 &w2) {
+    for (int p = 0; p < 6; p++) {
+        bool f1 = point_in_front(p, w1);
+        bool f2 = point_in_front(p, w2);
+        if (f1 && f2) continue;
+        else if (f1) w2 = intersect_segment(p, w1, w2);
+        else if (f2) w1 = intersect_segment(p, w2, w1);
+        else return false;
+    }
+    return true;
+}
+
+FGL_D bool alloc_side(const DrawArgs &a, uint32_t nrec, uint32_t nvtx, uint32_t &rbase, uint32_t &vbase) {
+    rbase = nrec ? atomicAdd(&a.ctr->side_rec, nrec) : 0u;
+    vbase = nvtx ? atomicAdd(&a.ctr->side_vtx, nvtx) : 0u;
+    bool ok = true;
+    if (nrec && (uint64_t)rbase + nrec > a.side_rec_cap) {
+        atomicOr(&a.ctr->overflow, 1u);
+        ok = false;
+    }
+    if (nvtx && (uint64_t)vbase + nvtx > a.side_vtx_cap) {
+        atomicOr(&a.ctr->overflow, 2u);
+        ok = false;
+    }
+    return ok;
+}
+
+// One input triangle (DrawTriangle, context.go:370-389).
+FGL_D void vs_triangle(const DrawArgs &a, int64_t i, uint32_t &nr, uint32_t &nt, uint32_t &ss) {
+    V3 P0 = load_pos(a.in, 3 * i), P1 = load_pos(a.in, 3 * i + 1), P2 = load_pos(a.in, 3 * i + 2);
+    V4 o0 = mul_position_w(a.sh.M, P0);
+    V4 o1 = mul_position_w(a.sh.M, P1);
+    V4 o2 = mul_position_w(a.sh.M, P2);
+    const uint32_t ident[3] = {0, 1, 2};
+    if (!(woutside(o0) || woutside(o1) || woutside(o2))) {
+        if (!a.st.wireframe) {
+            Emitter em{a.rec_main + i, 0, 0};
+            emit_clipped_triangle(a, em, ATTR_TRI, (uint64_t)i, ident, o0, o1, o2);
+            nr = em.n;
+            nt = em.ntiles;
+            return;
+        }
+        Emitter cnt{nullptr, 0, 0};
+        emit_clipped_triangle(a, cnt, ATTR_TRI, (uint64_t)i, ident, o0, o1, o2);
+        if (cnt.n == 0) return;
+        uint32_t rb, vb;
+        if (!alloc_side(a, cnt.n, 0, rb, vb)) return;
+        Emitter em{a.side_rec + rb, 0, 0};
+        emit_clipped_triangle(a, em, ATTR_TRI, (uint64_t)i, ident, o0, o1, o2);
+        nr = em.n | REF_SIDE;
+        nt = em.ntiles;
+        ss = rb;
+        return;
+    }
+    // clip to viewing volume (ClipTriangle(NewTriangle(v1, v2, v3)))
+    Vtx V[3];
+    V[0] = load_vertex(a.in, 3 * i);
+    V[1] = load_vertex(a.in, 3 * i + 1);
+    V[2] = load_vertex(a.in, 3 * i + 2);
+    V[0].out = o0;
+    V[1].out = o1;
+    V[2].out = o2;
+    fix_normals(V);
+    Emitter cnt{nullptr, 0, 0};
+    uint32_t nfan = 0;
+    clip_fans(a, V, cnt, 0, false, &nfan);
+    if (cnt.n == 0) return;
+    uint32_t rb, vb;
+    if (!alloc_side(a, cnt.n, 3 * nfan, rb, vb)) return;
+    Emitter em{a.side_rec + rb, 0, 0};
+    clip_fans(a, V, em, vb, true, nullptr);
+    nr = em.n | REF_SIDE;
+    nt = em.ntiles;
+    ss = rb;
+}
+
+// One input line (DrawLine, context.go:351-368).
+FGL_D void vs_line(const DrawArgs &a, int64_t i, uint32_t &nr, uint32_t &nt, uint32_t &ss) {
+    V3 P0 = load_pos(a.in, 2 * i), P1 = load_pos(a.in, 2 * i + 1);
+    V4 w1 = mul_position_w(a.sh.M, P0);
+    V4 w2 = mul_position_w(a.sh.M, P1);
+    if (woutside(w1) || woutside(w2)) {
+        if (!clip_line(w1, w2)) return;
+    }
+    // drawClippedLine (context.go:303-314)
+    V3 n0 = v3(w1.x / w1.w, w1.y / w1.w, w1.z / w1.w);
+    V3 n1 = v3(w2.x / w2.w, w2.y / w2.w, w2.z / w2.w);
+    V3 s0 = mul_position(a.S, n0);
+    V3 s1 = mul_position(a.S, n1);
+    Emitter cnt{nullptr, 0, 0};
+    emit_line(a, cnt, ATTR_LINE, (uint64_t)i, 0, 1, s0, s1, w1.w, w2.w);
+    if (cnt.n == 0) return;
+    uint32_t rb, vb;
+    if (!alloc_side(a, cnt.n, 0, rb, vb)) return;
+    Emitter em{a.side_rec + rb, 0, 0};
+    emit_line(a, em, ATTR_LINE, (uint64_t)i, 0, 1, s0, s1, w1.w, w2.w);
+    nr = em.n | REF_SIDE;
+    nt = em.ntiles;
+    ss = rb;
+}
+
+template <int KIND>
+__global__ __launch_bounds__(VS_THREADS) void k_vs(DrawArgs a) {
+    __shared__ uint64_t wsum[VS_THREADS / 64];
+    int64_t i = (int64_t)blockIdx.x * VS_THREADS + threadIdx.x;
+    uint32_t nr = 0, nt = 0, ss = 0;
+    if (i < a.in.n) {
+        if (KIND == PRIM_TRI) vs_triangle(a, i, nr, nt, ss);
+        else vs_line(a, i, nr, nt, ss);
+        a.nrast[i] = nr;
+        a.ntl[i] = nt;
+        a.sstart[i] = ss;
+    }
+    // block sum of pairs
+    uint64_t v = nt;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t s = 0;
+        for (int w = 0; w < VS_THREADS / 64; w++) s += wsum[w];
+        a.blocksum[blockIdx.x] = s;
+    }
+}
+
+hipError_t launch_vs(const DrawArgs &a, hipStream_t s) {
+    if (a.kind == PRIM_TRI) hipLaunchKernelGGL(k_vs<PRIM_TRI>, dim3(a.nvsblocks), dim3(VS_THREADS), 0, s, a);
+    else hipLaunchKernelGGL(k_vs<PRIM_LINE>, dim3(a.nvsblocks), dim3(VS_THREADS), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace fgl

@@ -1,0 +1,205 @@
+/*
+ * fgl.h -- C ABI of the MI355X (gfx950) draw path behind fauxgl's Go API.
+ *
+ * A thin cgo shim in package fauxgl binds these entry points (the binding
+ * is shown in INTEGRATION.md).  Every entry point names the reference
+ * function it replaces (file:line in fogleman/fauxgl).
+ *
+ * Conventions
+ *   - Plain C types only.  Host pointers unless the name says "device".
+ *   - Every function returns FGL_OK (0) or a negative fgl_status; the
+ *     message is available from fgl_last_error(ctx) (or
+ *     fgl_last_error(NULL) for failures that have no context yet).
+ *     Nothing aborts the process.
+ *   - Calls on one context are serialised on that context's HIP stream and
+ *     are synchronous unless the name ends in _async.  Different contexts
+ *     may be used from different threads.
+ *   - The framebuffer lives in device memory: color is NRGBA8
+ *     (non-premultiplied, byte order R,G,B,A, row-major, stride 4*W, like
+ *     image.NRGBA.Pix) and depth is float64 (W*H, index y*W+x), exactly the
+ *     layouts of Context.ColorBuffer / Context.DepthBuffer (context.go:43-44).
+ *
+ * Semantics follow the single-threaded reference: primitives are drawn in
+ * input order (a clipped triangle's fan in fan order, DrawMesh's triangles
+ * before its lines), so coverage and depth are bit-exact against fauxgl run
+ * with one worker, and RasterizeInfo matches.
+ */
+#ifndef FGL_H
+#define FGL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum fgl_status {
+    FGL_OK = 0,
+    FGL_E_INVALID = -1,     /* bad argument (null pointer, size, enum value) */
+    FGL_E_NOMEM = -2,       /* device or host allocation failed */
+    FGL_E_DEVICE = -3,      /* HIP runtime error or no usable gfx950 device */
+    FGL_E_UNSUPPORTED = -4  /* request outside what the device path implements */
+} fgl_status;
+
+/* context.go:11-26 */
+enum { FGL_FACE_CW = 1, FGL_FACE_CCW = 2 };
+enum { FGL_CULL_NONE = 1, FGL_CULL_FRONT = 2, FGL_CULL_BACK = 3 };
+
+/* Built-in shaders (shader.go:11-59). */
+enum { FGL_SHADER_SOLID = 1, FGL_SHADER_TEXTURE = 2, FGL_SHADER_PHONG = 3 };
+
+/* Texture texel formats.  NRGBA8: an image.NRGBA (At().RGBA() premultiplies
+ * on the fly, color.go:25-29 via image/color); RGBA8: an image.RGBA (already
+ * premultiplied); RGBA16: the four 16-bit values At(x,y).RGBA() returns for
+ * any Go image type, stored as uint16 R,G,B,A. */
+enum { FGL_TEX_NRGBA8 = 1, FGL_TEX_RGBA8 = 2, FGL_TEX_RGBA16 = 3 };
+
+/* SSAA resolve filters (the step after the path: the reference pipeline
+ * calls nfnt/resize's Bilinear, README.md:109). */
+enum { FGL_RESOLVE_TENT = 1, FGL_RESOLVE_BOX = 2 };
+
+/* Pipeline state = the public Context fields read at draw time
+ * (context.go:47-55). */
+typedef struct fgl_state {
+    uint8_t read_depth, write_depth, write_color, alpha_blend, wireframe;
+    uint8_t pad[3];
+    int32_t front_face; /* FGL_FACE_* */
+    int32_t cull;       /* FGL_CULL_* */
+    double line_width;
+    double depth_bias;
+} fgl_state;
+
+/* A built-in shader (shader.go:11-59).  Fields a shader kind does not have
+ * are ignored. */
+typedef struct fgl_shader {
+    int32_t kind; /* FGL_SHADER_* */
+    int32_t pad;
+    double matrix[16];       /* Matrix X00..X33, row-major (matrix.go:5-10) */
+    double color[4];         /* SolidColorShader.Color */
+    double light_dir[3];     /* PhongShader.LightDirection */
+    double camera_pos[3];    /* PhongShader.CameraPosition */
+    double object_color[4];  /* PhongShader.ObjectColor (all zero = Discard = use vertex color) */
+    double ambient[4];       /* PhongShader.AmbientColor */
+    double diffuse[4];       /* PhongShader.DiffuseColor */
+    double specular[4];      /* PhongShader.SpecularColor */
+    double specular_power;   /* PhongShader.SpecularPower */
+    int64_t texture;         /* TextureShader.Texture / PhongShader.Texture handle; -1 = nil */
+} fgl_shader;
+
+/* A triangle or line soup in SoA form: nv = 3 vertices per triangle, 2 per
+ * line, in the order Triangle.V1,V2,V3 / Line.V1,V2 (triangle.go:3-5,
+ * line.go:3-5).  position: count*nv*3 doubles (Vertex.Position); normal:
+ * count*nv*3 (Vertex.Normal); texture: count*nv*2 (Vertex.Texture.X,Y; the
+ * built-in shaders never read Texture.Z); color: count*nv*4 (Vertex.Color).
+ * A NULL attribute array stands for all-zero values, like a zero Vertex
+ * field. */
+typedef struct fgl_prims {
+    int64_t count;
+    const double *position;
+    const double *normal;
+    const double *texture;
+    const double *color;
+} fgl_prims;
+
+typedef struct fgl_ctx fgl_ctx;
+typedef struct fgl_mesh fgl_mesh;
+
+/* ---- library ---- */
+
+/* Selects device `device` (0-based) for contexts created afterwards on this
+ * thread and checks it is a gfx950 part.  Optional: fgl_ctx_create selects
+ * device 0 when this was never called.  No reference counterpart. */
+int fgl_init(int32_t device);
+
+/* Message for the last failure on ctx, or the last failure without a
+ * context when ctx is NULL.  Never returns NULL. */
+const char *fgl_last_error(const fgl_ctx *ctx);
+
+/* ---- context: NewContext, context.go:60-81 ---- */
+
+/* Allocates a W x H color + depth framebuffer on the device and clears the
+ * depth buffer to math.MaxFloat64 (context.go:79); the color buffer starts
+ * all zero like image.NewNRGBA. */
+int fgl_ctx_create(int32_t width, int32_t height, fgl_ctx **out);
+void fgl_ctx_destroy(fgl_ctx *ctx);
+int fgl_ctx_size(const fgl_ctx *ctx, int32_t *width, int32_t *height);
+
+/* ClearColorBufferWith (context.go:119-131): nrgba is Color.NRGBA() of the
+ * clear color, converted by the caller (color.go:56-63).  Deferred: the
+ * next draw writes the cleared tiles without reading them first. */
+int fgl_clear_color(fgl_ctx *ctx, const uint8_t nrgba[4]);
+/* ClearDepthBufferWith (context.go:137-141); ClearDepthBuffer passes
+ * math.MaxFloat64.  Deferred like fgl_clear_color. */
+int fgl_clear_depth(fgl_ctx *ctx, double value);
+
+/* ---- textures: ImageTexture, texture.go:21-25 ---- */
+
+/* Uploads width*height texels (row-major; 4 uint8 per texel for
+ * FGL_TEX_NRGBA8/RGBA8, 4 uint16 per texel for FGL_TEX_RGBA16) and returns
+ * a handle usable in fgl_shader.texture on this context. */
+int fgl_texture_create(fgl_ctx *ctx, int32_t width, int32_t height, int32_t kind, const void *texels,
+                       int64_t *handle);
+int fgl_texture_destroy(fgl_ctx *ctx, int64_t handle);
+
+/* ---- drawing ---- */
+
+/* DrawTriangles (context.go:413-433) with one of the built-in shaders:
+ * vertex shader, view-volume clipping, culling, rasterization, depth test,
+ * shading and blending, in input order.  info[0] = TotalPixels,
+ * info[1] = UpdatedPixels (context.go:28-31).  The host arrays are copied
+ * to the device during the call. */
+int fgl_draw_triangles(fgl_ctx *ctx, const fgl_state *state, const fgl_shader *shader, const fgl_prims *tris,
+                       uint64_t info[2]);
+/* DrawLines (context.go:391-411): each line becomes a LineWidth-wide quad
+ * (context.go:283-294). */
+int fgl_draw_lines(fgl_ctx *ctx, const fgl_state *state, const fgl_shader *shader, const fgl_prims *lines,
+                   uint64_t info[2]);
+
+/* Device-resident meshes: Mesh{Triangles, Lines} (mesh.go:9-13) uploaded
+ * once, for callers that draw the same mesh many times. */
+int fgl_mesh_create(fgl_ctx *ctx, const fgl_prims *tris, const fgl_prims *lines, fgl_mesh **out);
+void fgl_mesh_destroy(fgl_mesh *mesh);
+/* DrawMesh (context.go:435-439): triangles, then lines. */
+int fgl_draw_mesh(fgl_ctx *ctx, const fgl_state *state, const fgl_shader *shader, const fgl_mesh *mesh,
+                  uint64_t info[2]);
+
+/* ---- readback and host writes (the public ColorBuffer/DepthBuffer fields) ---- */
+
+/* Image() (context.go:83-85): copies the 4*W*H NRGBA bytes out. */
+int fgl_read_color(fgl_ctx *ctx, uint8_t *nrgba);
+/* DepthBuffer: copies W*H float64 out. */
+int fgl_read_depth(fgl_ctx *ctx, double *depth);
+/* For callers that modified ColorBuffer.Pix / DepthBuffer on the host. */
+int fgl_write_color(fgl_ctx *ctx, const uint8_t *nrgba);
+int fgl_write_depth(fgl_ctx *ctx, const double *depth);
+/* DepthImage() (context.go:87-117): W*H Gray16 values. */
+int fgl_depth_image(fgl_ctx *ctx, uint16_t *gray16);
+
+/* SSAA resolve: dst (w x h) color = src (w*f x h*f) color downsampled by the
+ * integer factor f with `filter`.  FGL_RESOLVE_TENT is a separable 8-bit
+ * fixed-point tent filter of radius f (the Bilinear filter widened by the
+ * scale factor); FGL_RESOLVE_BOX averages each f x f block.  Both contexts
+ * must be on the same device; dst's depth buffer is untouched. */
+int fgl_resolve(fgl_ctx *src, fgl_ctx *dst, int32_t filter);
+
+/* ---- measurement (no reference counterpart) ---- */
+
+/* When enabled, every kernel launch on ctx is bracketed by HIP events on the
+ * context's stream and its duration accumulated per kernel. */
+int fgl_profile_enable(fgl_ctx *ctx, int32_t enable);
+/* Copies up to `cap` records: name (NUL-terminated, 48 bytes), launch count
+ * and total milliseconds.  Returns the number of kernels via *n. */
+typedef struct fgl_kernel_time {
+    char name[48];
+    uint64_t launches;
+    double total_ms;
+} fgl_kernel_time;
+int fgl_profile_read(fgl_ctx *ctx, fgl_kernel_time *out, int32_t cap, int32_t *n);
+int fgl_profile_reset(fgl_ctx *ctx);
+/* Waits for all work queued on ctx. */
+int fgl_ctx_sync(fgl_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1132 @@
+/*
+ * fauxgl_oracle.c -- CPU restatement of fauxgl's draw path (test
+ * infrastructure only: the checker and the timed CPU baseline; the product
+ * never links it).  See fauxgl_oracle.h for the semantics and what pins each
+ * part.  Every function cites the reference file:line it restates.
+ *
+ * Build: gcc -O2 -ffp-contract=off -fPIC -shared (see oracle/Makefile).
+ * Do not build with -ffast-math or -march flags that enable FMA contraction.
+ */
+#include "fauxgl_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* Raster triangles whose integer bounding box leaves [-LIM, LIM] are
+ * skipped.  In the reference such boxes come only from NaN/Inf screen
+ * coordinates (int() of NaN/Inf is INT64_MIN on amd64), where its loops
+ * either never end or wrap int64; no finite, clipped input reaches it. */
+#define ORC_BBOX_LIM ((int64_t)1 << 26)
+
+/* ------------------------------------------------------------------ */
+/* L0 math types (vector.go:8-10, 204-206; color.go:17-19; matrix.go:5-10) */
+
+typedef struct { double X, Y, Z; } Vec;
+typedef struct { double X, Y, Z, W; } VecW;
+typedef struct { double R, G, B, A; } Col;
+typedef struct { double m[16]; } Mat; /* X00 X01 X02 X03 X10 ... X33 */
+
+/* vertex.go:3-12 */
+typedef struct {
+    Vec Position, Normal, Texture;
+    Col Color;
+    VecW Output;
+} Vertex;
+typedef struct { Vertex V1, V2, V3; } Triangle; /* triangle.go:3-5 */
+typedef struct { Vertex V1, V2; } Line;         /* line.go:3-5 */
+
+/* ------------------------------------------------------------------ */
+/* Go semantics helpers */
+
+/* math.Min: -Inf wins over NaN, NaN propagates, Min(-0,+0) = -0. */
+static double go_min(double x, double y) {
+    if ((isinf(x) && x < 0) || (isinf(y) && y < 0)) return -INFINITY;
+    if (isnan(x) || isnan(y)) return NAN;
+    if (x == 0 && x == y) return signbit(x) ? x : y;
+    return x < y ? x : y;
+}
+
+/* math.Max: +Inf wins over NaN, NaN propagates, Max(+0,-0) = +0. */
+static double go_max(double x, double y) {
+    if ((isinf(x) && x > 0) || (isinf(y) && y > 0)) return INFINITY;
+    if (isnan(x) || isnan(y)) return NAN;
+    if (x == 0 && x == y) return signbit(x) ? y : x;
+    return x > y ? x : y;
+}
+
+/* int(x) on amd64 (CVTTSD2SQ): truncation; NaN / out of range -> INT64_MIN. */
+static int64_t go_f2i(double x) {
+    if (!(x >= -9223372036854775808.0 && x < 9223372036854775808.0)) return INT64_MIN;
+    return (int64_t)x;
+}
+
+/* uint8(x) / uint16(x) on amd64 convert through int32 (CVTTSD2SL), then
+ * truncate; NaN / out of range -> INT32_MIN, whose low bits are 0. */
+static int32_t go_f2i32(double x) {
+    if (!(x > -2147483649.0 && x < 2147483648.0)) return INT32_MIN;
+    return (int32_t)x;
+}
+static uint8_t go_u8(double x) { return (uint8_t)(uint32_t)go_f2i32(x); }
+static uint16_t go_u16(double x) { return (uint16_t)(uint32_t)go_f2i32(x); }
+
+/* util.go:74-82 (NaN passes through). */
+static double clampd(double x, double lo, double hi) {
+    if (x < lo) return lo;
+    if (x > hi) return hi;
+    return x;
+}
+
+static int go_is_odd_int(double x) {
+    if (fabs(x) >= 9007199254740992.0) return 0;
+    double xi;
+    double xf = modf(x, &xi);
+    return xf == 0 && (((int64_t)xi) & 1) == 1;
+}
+
+/* Go math.Pow: special cases, then x**yi by repeated squaring of the
+ * Frexp mantissa with the binary exponent tracked separately, times
+ * x**yf (exp/log; libm here -- parity unpinned for non-integer y). */
+static double go_pow(double x, double y) {
+    if (y == 0 || x == 1) return 1;
+    if (y == 1) return x;
+    if (isnan(x) || isnan(y)) return NAN;
+    if (x == 0) {
+        if (y < 0) {
+            if (signbit(x) && go_is_odd_int(y)) return -INFINITY;
+            return INFINITY;
+        }
+        if (signbit(x) && go_is_odd_int(y)) return x;
+        return 0;
+    }
+    if (isinf(y)) {
+        if (x == -1) return 1;
+        if ((fabs(x) < 1) == (y > 0)) return 0;
+        return INFINITY;
+    }
+    if (isinf(x)) {
+        if (x < 0) return go_pow(1 / x, -y);
+        if (y < 0) return 0;
+        return INFINITY;
+    }
+    if (y == 0.5) return sqrt(x);
+    if (y == -0.5) return 1 / sqrt(x);
+
+    double yi;
+    double yf = modf(fabs(y), &yi);
+    if (yf != 0 && x < 0) return NAN;
+    if (yi >= 9223372036854775808.0) {
+        if (x == -1) return 1;
+        if ((fabs(x) < 1) == (y > 0)) return 0;
+        return INFINITY;
+    }
+    double a1 = 1.0;
+    int64_t ae = 0;
+    if (yf != 0) {
+        if (yf > 0.5) {
+            yf--;
+            yi++;
+        }
+        a1 = exp(yf * log(x));
+    }
+    int xe_i;
+    double x1 = frexp(x, &xe_i);
+    int64_t xe = xe_i;
+    for (int64_t i = (int64_t)yi; i != 0; i >>= 1) {
+        if (xe < -(1 << 12) || (1 << 12) < xe) {
+            ae += xe;
+            break;
+        }
+        if (i & 1) {
+            a1 *= x1;
+            ae += xe;
+        }
+        x1 *= x1;
+        xe <<= 1;
+        if (x1 < .5) {
+            x1 += x1;
+            xe--;
+        }
+    }
+    if (y < 0) {
+        a1 = 1 / a1;
+        ae = -ae;
+    }
+    if (ae > 100000) ae = 100000;
+    if (ae < -100000) ae = -100000;
+    return ldexp(a1, (int)ae);
+}
+
+/* ------------------------------------------------------------------ */
+/* vector.go */
+
+static Vec vadd(Vec a, Vec b) { return (Vec){a.X + b.X, a.Y + b.Y, a.Z + b.Z}; }  /* 96-98 */
+static Vec vsub(Vec a, Vec b) { return (Vec){a.X - b.X, a.Y - b.Y, a.Z - b.Z}; }  /* 100-102 */
+static Vec vmuls(Vec a, double b) { return (Vec){a.X * b, a.Y * b, a.Z * b}; }     /* 128-130 */
+static Vec vdivs(Vec a, double b) { return (Vec){a.X / b, a.Y / b, a.Z / b}; }     /* 132-134 */
+static double vdot(Vec a, Vec b) { return a.X * b.X + a.Y * b.Y + a.Z * b.Z; }    /* 72-74 */
+static Vec vneg(Vec a) { return (Vec){-a.X, -a.Y, -a.Z}; }                        /* 88-90 */
+static Vec vcross(Vec a, Vec b) {                                                  /* 76-81 */
+    double x = a.Y * b.Z - a.Z * b.Y;
+    double y = a.Z * b.X - a.X * b.Z;
+    double z = a.X * b.Y - a.Y * b.X;
+    return (Vec){x, y, z};
+}
+static Vec vnormalize(Vec a) { /* 83-86 */
+    double r = 1 / sqrt(a.X * a.X + a.Y * a.Y + a.Z * a.Z);
+    return (Vec){a.X * r, a.Y * r, a.Z * r};
+}
+static Vec vmin(Vec a, Vec b) { return (Vec){go_min(a.X, b.X), go_min(a.Y, b.Y), go_min(a.Z, b.Z)}; } /* 140-142 */
+static Vec vmax(Vec a, Vec b) { return (Vec){go_max(a.X, b.X), go_max(a.Y, b.Y), go_max(a.Z, b.Z)}; } /* 144-146 */
+static Vec vfloor(Vec a) { return (Vec){floor(a.X), floor(a.Y), floor(a.Z)}; }    /* 148-150 */
+static Vec vceil(Vec a) { return (Vec){ceil(a.X), ceil(a.Y), ceil(a.Z)}; }        /* 152-154 */
+static Vec vreflect(Vec i, Vec n) { return vsub(i, vmuls(n, 2 * vdot(n, i))); }   /* 175-177 */
+static Vec vperpendicular(Vec a) {                                                 /* 179-187 */
+    if (a.X == 0 && a.Y == 0) {
+        if (a.Z == 0) return (Vec){0, 0, 0};
+        return (Vec){0, 1, 0};
+    }
+    return vnormalize((Vec){-a.Y, a.X, 0});
+}
+static int vequal(Vec a, Vec b) { return a.X == b.X && a.Y == b.Y && a.Z == b.Z; }
+
+static Vec wvec(VecW a) { return (Vec){a.X, a.Y, a.Z}; }                           /* 208-210 */
+static int woutside(VecW a) {                                                      /* 212-215 */
+    double x = a.X, y = a.Y, z = a.Z, w = a.W;
+    return x < -w || x > w || y < -w || y > w || z < -w || z > w;
+}
+static double wdot(VecW a, VecW b) { return a.X * b.X + a.Y * b.Y + a.Z * b.Z + a.W * b.W; } /* 217-219 */
+static VecW wadd(VecW a, VecW b) { return (VecW){a.X + b.X, a.Y + b.Y, a.Z + b.Z, a.W + b.W}; }
+static VecW wsub(VecW a, VecW b) { return (VecW){a.X - b.X, a.Y - b.Y, a.Z - b.Z, a.W - b.W}; }
+static VecW wmuls(VecW a, double b) { return (VecW){a.X * b, a.Y * b, a.Z * b, a.W * b}; }
+static VecW wdivs(VecW a, double b) { return (VecW){a.X / b, a.Y / b, a.Z / b, a.W / b}; }
+
+/* ------------------------------------------------------------------ */
+/* color.go */
+
+static Col cadd(Col a, Col b) { return (Col){a.R + b.R, a.G + b.G, a.B + b.B, a.A + b.A}; }  /* 80-82 */
+static Col cmul(Col a, Col b) { return (Col){a.R * b.R, a.G * b.G, a.B * b.B, a.A * b.A}; }  /* 88-90 */
+static Col cmuls(Col a, double b) { return (Col){a.R * b, a.G * b, a.B * b, a.A * b}; }      /* 104-106 */
+static Col cmin(Col a, Col b) {                                                               /* 116-118 */
+    return (Col){go_min(a.R, b.R), go_min(a.G, b.G), go_min(a.B, b.B), go_min(a.A, b.A)};
+}
+static int cdiscard(Col c) { return c.R == 0 && c.G == 0 && c.B == 0 && c.A == 0; } /* context.go:241 */
+
+typedef struct { uint8_t R, G, B, A; } NRGBA8;
+
+/* color.go:56-63 */
+static NRGBA8 cnrgba(Col c) {
+    const double d = 0xff;
+    double r = clampd(c.R, 0, 1), g = clampd(c.G, 0, 1), b = clampd(c.B, 0, 1), a = clampd(c.A, 0, 1);
+    NRGBA8 o = {go_u8(r * d), go_u8(g * d), go_u8(b * d), go_u8(a * d)};
+    return o;
+}
+
+/* image/color NRGBA.RGBA(): 16-bit alpha-premultiplied components. */
+static void nrgba_rgba(NRGBA8 c, uint32_t *r, uint32_t *g, uint32_t *b, uint32_t *a) {
+    uint32_t x;
+    x = c.R; x |= x << 8; x *= c.A; x /= 0xff; *r = x;
+    x = c.G; x |= x << 8; x *= c.A; x /= 0xff; *g = x;
+    x = c.B; x |= x << 8; x *= c.A; x /= 0xff; *b = x;
+    x = c.A; x |= x << 8; *a = x;
+}
+
+/* ------------------------------------------------------------------ */
+/* matrix.go */
+
+#define M(mat, r, c) ((mat).m[(r) * 4 + (c)])
+
+static Mat mat_mul(Mat a, Mat b) { /* 188-207 */
+    Mat o;
+    for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++)
+            M(o, r, c) = M(a, r, 0) * M(b, 0, c) + M(a, r, 1) * M(b, 1, c) + M(a, r, 2) * M(b, 2, c) +
+                         M(a, r, 3) * M(b, 3, c);
+    return o;
+}
+static Vec mat_mul_position(Mat a, Vec b) { /* 209-214 */
+    double x = M(a, 0, 0) * b.X + M(a, 0, 1) * b.Y + M(a, 0, 2) * b.Z + M(a, 0, 3);
+    double y = M(a, 1, 0) * b.X + M(a, 1, 1) * b.Y + M(a, 1, 2) * b.Z + M(a, 1, 3);
+    double z = M(a, 2, 0) * b.X + M(a, 2, 1) * b.Y + M(a, 2, 2) * b.Z + M(a, 2, 3);
+    return (Vec){x, y, z};
+}
+static VecW mat_mul_position_w(Mat a, Vec b) { /* 216-222 */
+    double x = M(a, 0, 0) * b.X + M(a, 0, 1) * b.Y + M(a, 0, 2) * b.Z + M(a, 0, 3);
+    double y = M(a, 1, 0) * b.X + M(a, 1, 1) * b.Y + M(a, 1, 2) * b.Z + M(a, 1, 3);
+    double z = M(a, 2, 0) * b.X + M(a, 2, 1) * b.Y + M(a, 2, 2) * b.Z + M(a, 2, 3);
+    double w = M(a, 3, 0) * b.X + M(a, 3, 1) * b.Y + M(a, 3, 2) * b.Z + M(a, 3, 3);
+    return (VecW){x, y, z, w};
+}
+static Mat mat_screen(int w, int h) { /* 119-128 */
+    double w2 = (double)w / 2;
+    double h2 = (double)h / 2;
+    Mat o = {{w2, 0, 0, w2, 0, -h2, 0, h2, 0, 0, 0.5, 0.5, 0, 0, 0, 1}};
+    return o;
+}
+static Mat mat_frustum(double l, double r, double b, double t, double n, double f) { /* 69-79 */
+    double t1 = 2 * n, t2 = r - l, t3 = t - b, t4 = f - n;
+    Mat o = {{t1 / t2, 0, (r + l) / t2, 0, 0, t1 / t3, (t + b) / t3, 0, 0, 0, (-f - n) / t4, (-t1 * f) / t4, 0, 0, -1, 0}};
+    return o;
+}
+static Mat mat_perspective(double fovy, double aspect, double near, double far) { /* 89-93 */
+    double ymax = near * tan(fovy * M_PI / 360);
+    double xmax = ymax * aspect;
+    return mat_frustum(-xmax, xmax, -ymax, ymax, near, far);
+}
+static Mat mat_look_at(Vec eye, Vec center, Vec up) { /* 95-105 */
+    Vec z = vnormalize(vsub(eye, center));
+    Vec x = vnormalize(vcross(up, z));
+    Vec y = vcross(z, x);
+    Mat o = {{x.X, x.Y, x.Z, -vdot(x, eye), y.X, y.Y, y.Z, -vdot(y, eye), z.X, z.Y, z.Z, -vdot(z, eye), 0, 0, 0, 1}};
+    return o;
+}
+
+/* ------------------------------------------------------------------ */
+/* vertex.go / triangle.go */
+
+static Vec interp_vec(Vec v1, Vec v2, Vec v3, VecW b) { /* vertex.go:59-65 */
+    Vec n = {0, 0, 0};
+    n = vadd(n, vmuls(v1, b.X));
+    n = vadd(n, vmuls(v2, b.Y));
+    n = vadd(n, vmuls(v3, b.Z));
+    return vmuls(n, b.W);
+}
+static Col interp_col(Col v1, Col v2, Col v3, VecW b) { /* vertex.go:51-57 */
+    Col n = {0, 0, 0, 0};
+    n = cadd(n, cmuls(v1, b.X));
+    n = cadd(n, cmuls(v2, b.Y));
+    n = cadd(n, cmuls(v3, b.Z));
+    return cmuls(n, b.W);
+}
+static VecW interp_vecw(VecW v1, VecW v2, VecW v3, VecW b) { /* vertex.go:67-73 */
+    VecW n = {0, 0, 0, 0};
+    n = wadd(n, wmuls(v1, b.X));
+    n = wadd(n, wmuls(v2, b.Y));
+    n = wadd(n, wmuls(v3, b.Z));
+    return wmuls(n, b.W);
+}
+/* vertex.go:18-47 (all fields, including those no built-in shader reads) */
+static Vertex interpolate_vertexes(const Vertex *v1, const Vertex *v2, const Vertex *v3, VecW b) {
+    Vertex v;
+    v.Position = interp_vec(v1->Position, v2->Position, v3->Position, b);
+    v.Normal = vnormalize(interp_vec(v1->Normal, v2->Normal, v3->Normal, b));
+    v.Texture = interp_vec(v1->Texture, v2->Texture, v3->Texture, b);
+    v.Color = interp_col(v1->Color, v2->Color, v3->Color, b);
+    v.Output = interp_vecw(v1->Output, v2->Output, v3->Output, b);
+    return v;
+}
+static VecW barycentric(Vec p1, Vec p2, Vec p3, Vec p) { /* vertex.go:81-95 */
+    Vec v0 = vsub(p2, p1);
+    Vec v1 = vsub(p3, p1);
+    Vec v2 = vsub(p, p1);
+    double d00 = vdot(v0, v0);
+    double d01 = vdot(v0, v1);
+    double d11 = vdot(v1, v1);
+    double d20 = vdot(v2, v0);
+    double d21 = vdot(v2, v1);
+    double d = d00 * d11 - d01 * d01;
+    double v = (d11 * d20 - d01 * d21) / d;
+    double w = (d00 * d21 - d01 * d20) / d;
+    double u = 1 - v - w;
+    return (VecW){u, v, w, 1};
+}
+static void fix_normals(Triangle *t) { /* triangle.go:33-37, 46-58 */
+    Vec e1 = vsub(t->V2.Position, t->V1.Position);
+    Vec e2 = vsub(t->V3.Position, t->V1.Position);
+    Vec n = vnormalize(vcross(e1, e2));
+    Vec zero = {0, 0, 0};
+    if (vequal(t->V1.Normal, zero)) t->V1.Normal = n;
+    if (vequal(t->V2.Normal, zero)) t->V2.Normal = n;
+    if (vequal(t->V3.Normal, zero)) t->V3.Normal = n;
+}
+
+/* ------------------------------------------------------------------ */
+/* clipping.go */
+
+static const VecW clip_P[6] = {{1, 0, 0, 1}, {-1, 0, 0, 1}, {0, 1, 0, 1}, {0, -1, 0, 1}, {0, 0, 1, 1}, {0, 0, -1, 1}};
+static const VecW clip_N[6] = {{-1, 0, 0, 1}, {1, 0, 0, 1}, {0, -1, 0, 1}, {0, 1, 0, 1}, {0, 0, -1, 1}, {0, 0, 1, 1}};
+
+static int point_in_front(int p, VecW v) { return wdot(wsub(v, clip_P[p]), clip_N[p]) > 0; } /* 16-18 */
+static VecW intersect_segment(int p, VecW v0, VecW v1) {                                   /* 20-26 */
+    VecW u = wsub(v1, v0);
+    VecW w = wsub(v0, clip_P[p]);
+    double d = wdot(clip_N[p], u);
+    double n = -wdot(clip_N[p], w);
+    return wadd(v0, wmuls(u, n / d));
+}
+/* 28-52; at most 9 output points for a triangle. */
+static int sutherland_hodgman(const VecW *points, int npoints, VecW *out) {
+    VecW buf[2][16];
+    int n = npoints;
+    for (int i = 0; i < n; i++) buf[0][i] = points[i];
+    int cur = 0;
+    for (int p = 0; p < 6; p++) {
+        VecW *input = buf[cur];
+        VecW *output = buf[cur ^ 1];
+        int m = 0;
+        if (n == 0) return 0;
+        VecW s = input[n - 1];
+        for (int k = 0; k < n; k++) {
+            VecW e = input[k];
+            if (point_in_front(p, e)) {
+                if (!point_in_front(p, s)) output[m++] = intersect_segment(p, s, e);
+                output[m++] = e;
+            } else if (point_in_front(p, s)) {
+                output[m++] = intersect_segment(p, s, e);
+            }
+            s = e;
+        }
+        n = m;
+        cur ^= 1;
+    }
+    for (int i = 0; i < n; i++) out[i] = buf[cur][i];
+    return n;
+}
+/* 54-74: returns the fan triangles (NewTriangle -> FixNormals each). */
+static int clip_triangle(const Triangle *t, Triangle *out) {
+    VecW w1 = t->V1.Output, w2 = t->V2.Output, w3 = t->V3.Output;
+    Vec p1 = wvec(w1), p2 = wvec(w2), p3 = wvec(w3);
+    VecW points[3] = {w1, w2, w3};
+    VecW np[16];
+    int n = sutherland_hodgman(points, 3, np);
+    int k = 0;
+    for (int i = 2; i < n; i++) {
+        VecW b1 = barycentric(p1, p2, p3, wvec(np[0]));
+        VecW b2 = barycentric(p1, p2, p3, wvec(np[i - 1]));
+        VecW b3 = barycentric(p1, p2, p3, wvec(np[i]));
+        Triangle r;
+        r.V1 = interpolate_vertexes(&t->V1, &t->V2, &t->V3, b1);
+        r.V2 = interpolate_vertexes(&t->V1, &t->V2, &t->V3, b2);
+        r.V3 = interpolate_vertexes(&t->V1, &t->V2, &t->V3, b3);
+        fix_normals(&r);
+        out[k++] = r;
+    }
+    return k;
+}
+/* 76-98: attributes are not re-interpolated; returns 0 for nil. */
+static int clip_line(const Line *l, Line *out) {
+    VecW w1 = l->V1.Output, w2 = l->V2.Output;
+    for (int p = 0; p < 6; p++) {
+        int f1 = point_in_front(p, w1);
+        int f2 = point_in_front(p, w2);
+        if (f1 && f2) continue;
+        else if (f1) w2 = intersect_segment(p, w1, w2);
+        else if (f2) w1 = intersect_segment(p, w2, w1);
+        else return 0;
+    }
+    out->V1 = l->V1;
+    out->V2 = l->V2;
+    out->V1.Output = w1;
+    out->V2.Output = w2;
+    return 1;
+}
+
+/* ------------------------------------------------------------------ */
+/* texture.go */
+
+static Col tex_at(const orc_texture *t, int64_t x, int64_t y) {
+    /* image.Image.At(x,y).RGBA() then MakeColor (color.go:25-29). */
+    uint32_t r, g, b, a;
+    if (x < 0 || y < 0 || x >= t->width || y >= t->height) {
+        r = g = b = a = 0;
+    } else {
+        int64_t i = (y * t->width + x) * 4;
+        if (t->kind == ORC_TEX_RGBA16) {
+            const uint16_t *p = (const uint16_t *)t->pix;
+            r = p[i]; g = p[i + 1]; b = p[i + 2]; a = p[i + 3];
+        } else {
+            const uint8_t *p = (const uint8_t *)t->pix;
+            NRGBA8 c = {p[i], p[i + 1], p[i + 2], p[i + 3]};
+            if (t->kind == ORC_TEX_NRGBA8) {
+                nrgba_rgba(c, &r, &g, &b, &a);
+            } else { /* image.RGBA: already premultiplied */
+                r = c.R; r |= r << 8;
+                g = c.G; g |= g << 8;
+                b = c.B; b |= b << 8;
+                a = c.A; a |= a << 8;
+            }
+        }
+    }
+    const double d = 0xffff;
+    return (Col){(double)r / d, (double)g / d, (double)b / d, (double)a / d};
+}
+static Col bilinear_sample(const orc_texture *t, double u, double v) { /* texture.go:41-63 */
+    v = 1 - v;
+    u -= floor(u);
+    v -= floor(v);
+    double x = u * (double)(t->width - 1);
+    double y = v * (double)(t->height - 1);
+    int64_t x0 = go_f2i(x);
+    int64_t y0 = go_f2i(y);
+    int64_t x1 = x0 + 1;
+    int64_t y1 = y0 + 1;
+    x -= (double)x0;
+    y -= (double)y0;
+    Col c00 = tex_at(t, x0, y0);
+    Col c01 = tex_at(t, x0, y1);
+    Col c10 = tex_at(t, x1, y0);
+    Col c11 = tex_at(t, x1, y1);
+    Col c = {0, 0, 0, 0};
+    c = cadd(c, cmuls(c00, (1 - x) * (1 - y)));
+    c = cadd(c, cmuls(c10, x * (1 - y)));
+    c = cadd(c, cmuls(c01, (1 - x) * y));
+    c = cadd(c, cmuls(c11, x * y));
+    return c;
+}
+
+/* ------------------------------------------------------------------ */
+/* shader.go */
+
+typedef struct {
+    int kind;
+    Mat Matrix;
+    Col Color;
+    Vec LightDirection, CameraPosition;
+    Col ObjectColor, AmbientColor, DiffuseColor, SpecularColor;
+    double SpecularPower;
+    orc_texture tex;
+    int has_tex;
+} Shader;
+
+static Vertex shader_vertex(const Shader *s, Vertex v) { /* shader.go:20-23, 39-42, 70-73 */
+    v.Output = mat_mul_position_w(s->Matrix, v.Position);
+    return v;
+}
+static Col phong_fragment(const Shader *s, const Vertex *v) { /* shader.go:75-96 */
+    Col light = s->AmbientColor;
+    Col color = v->Color;
+    if (!cdiscard(s->ObjectColor)) color = s->ObjectColor;
+    if (s->has_tex) color = bilinear_sample(&s->tex, v->Texture.X, v->Texture.Y);
+    double diffuse = go_max(vdot(v->Normal, s->LightDirection), 0);
+    light = cadd(light, cmuls(s->DiffuseColor, diffuse));
+    if (diffuse > 0 && s->SpecularPower > 0) {
+        Vec camera = vnormalize(vsub(s->CameraPosition, v->Position));
+        Vec reflected = vreflect(vneg(s->LightDirection), v->Normal);
+        double specular = go_max(vdot(camera, reflected), 0);
+        if (specular > 0) {
+            specular = go_pow(specular, s->SpecularPower);
+            light = cadd(light, cmuls(s->SpecularColor, specular));
+        }
+    }
+    Col o = cmin(cmul(color, light), (Col){1, 1, 1, 1});
+    o.A = color.A;
+    return o;
+}
+/* Fragment dispatch through a function pointer, like the reference's
+ * interface call (context.go:240), so the full interpolated Vertex is
+ * materialised per fragment as in the reference. */
+typedef Col (*frag_fn)(const Shader *, const Vertex *);
+static Col solid_fragment(const Shader *s, const Vertex *v) { (void)v; return s->Color; }           /* 25-27 */
+static Col texture_fragment(const Shader *s, const Vertex *v) {                                     /* 44-46 */
+    return bilinear_sample(&s->tex, v->Texture.X, v->Texture.Y);
+}
+
+/* ------------------------------------------------------------------ */
+/* context.go */
+
+struct orc_ctx {
+    int32_t Width, Height;
+    uint8_t *Pix;   /* ColorBuffer.Pix, stride 4*Width */
+    double *Depth;  /* DepthBuffer */
+    Shader shader;
+    frag_fn fragment;
+    int ReadDepth, WriteDepth, WriteColor, AlphaBlend, Wireframe;
+    int FrontFace, Cull;
+    double LineWidth, DepthBias;
+    Mat screenMatrix;
+    pthread_mutex_t locks[256];
+    int threaded;
+    uint64_t counters[ORC_CNT_N];
+};
+
+typedef struct {
+    uint64_t total, updated;
+} Info;
+
+typedef struct {
+    orc_ctx *dc;
+    uint64_t cnt[ORC_CNT_N];
+} Worker;
+
+static double load_depth(const orc_ctx *dc, int64_t i) {
+    /* The reference reads the depth buffer without a lock (context.go:232);
+     * relaxed atomic access keeps the threaded mirror free of UB. */
+    uint64_t bits = __atomic_load_n((const uint64_t *)&dc->Depth[i], __ATOMIC_RELAXED);
+    double d;
+    memcpy(&d, &bits, 8);
+    return d;
+}
+static void store_depth(orc_ctx *dc, int64_t i, double d) {
+    uint64_t bits;
+    memcpy(&bits, &d, 8);
+    __atomic_store_n((uint64_t *)&dc->Depth[i], bits, __ATOMIC_RELAXED);
+}
+
+/* context.go:147-149 */
+static double edge(Vec a, Vec b, Vec c) { return (b.X - c.X) * (a.Y - c.Y) - (b.Y - c.Y) * (a.X - c.X); }
+
+/* context.go:151-281 */
+static Info rasterize(Worker *wk, const Vertex *v0, const Vertex *v1, const Vertex *v2, Vec s0, Vec s1, Vec s2) {
+    orc_ctx *dc = wk->dc;
+    Info info = {0, 0};
+    wk->cnt[ORC_CNT_RASTER]++;
+
+    /* integer bounding box */
+    Vec mn = vfloor(vmin(s0, vmin(s1, s2)));
+    Vec mx = vceil(vmax(s0, vmax(s1, s2)));
+    int64_t x0 = go_f2i(mn.X);
+    int64_t x1 = go_f2i(mx.X);
+    int64_t y0 = go_f2i(mn.Y);
+    int64_t y1 = go_f2i(mx.Y);
+    if (x0 < -ORC_BBOX_LIM || x0 > ORC_BBOX_LIM || x1 < -ORC_BBOX_LIM || x1 > ORC_BBOX_LIM ||
+        y0 < -ORC_BBOX_LIM || y0 > ORC_BBOX_LIM || y1 < -ORC_BBOX_LIM || y1 > ORC_BBOX_LIM) {
+        wk->cnt[ORC_CNT_SKIPPED]++;
+        return info;
+    }
+
+    /* forward differencing variables */
+    Vec p = {(double)x0 + 0.5, (double)y0 + 0.5, 0};
+    double w00 = edge(s1, s2, p);
+    double w01 = edge(s2, s0, p);
+    double w02 = edge(s0, s1, p);
+    double a01 = s1.Y - s0.Y;
+    double b01 = s0.X - s1.X;
+    double a12 = s2.Y - s1.Y;
+    double b12 = s1.X - s2.X;
+    double a20 = s0.Y - s2.Y;
+    double b20 = s2.X - s0.X;
+
+    /* reciprocals */
+    double ra = 1 / edge(s0, s1, s2);
+    double r0 = 1 / v0->Output.W;
+    double r1 = 1 / v1->Output.W;
+    double r2 = 1 / v2->Output.W;
+    double ra12 = 1 / a12;
+    double ra20 = 1 / a20;
+    double ra01 = 1 / a01;
+
+    const int64_t W = dc->Width;
+    const int64_t npix = (int64_t)dc->Width * dc->Height;
+    for (int64_t y = y0; y <= y1; y++) {
+        wk->cnt[ORC_CNT_ROWS]++;
+        double d = 0;
+        double d0 = -w00 * ra12;
+        double d1 = -w01 * ra20;
+        double d2 = -w02 * ra01;
+        if (w00 < 0 && d0 > d) d = d0;
+        if (w01 < 0 && d1 > d) d = d1;
+        if (w02 < 0 && d2 > d) d = d2;
+        d = (double)go_f2i(d);
+        if (d < 0) d = 0; /* occurs in pathological cases */
+        double w0 = w00 + a12 * d;
+        double w1 = w01 + a20 * d;
+        double w2 = w02 + a01 * d;
+        int wasInside = 0;
+        int64_t id = go_f2i(d);
+        /* x0 + int(d) > x1 (also guards the int64 wrap of the sum) */
+        int64_t xs = (id > x1 - x0) ? x1 + 1 : x0 + id;
+        for (int64_t x = xs; x <= x1; x++) {
+            wk->cnt[ORC_CNT_SCANNED]++;
+            double b0 = w0 * ra;
+            double b1 = w1 * ra;
+            double b2 = w2 * ra;
+            w0 += a12;
+            w1 += a20;
+            w2 += a01;
+            /* check if inside triangle */
+            if (b0 < 0 || b1 < 0 || b2 < 0) {
+                if (wasInside) break;
+                continue;
+            }
+            wasInside = 1;
+            /* check depth buffer for early abort */
+            int64_t i = y * W + x;
+            if (i < 0 || i >= npix) continue;
+            info.total++;
+            wk->cnt[ORC_CNT_INSIDE]++;
+            double z = b0 * s0.Z + b1 * s1.Z + b2 * s2.Z;
+            double bz = z + dc->DepthBias;
+            if (dc->ReadDepth && bz > load_depth(dc, i)) continue;
+            /* perspective-correct interpolation of vertex data */
+            VecW b = {b0 * r0, b1 * r1, b2 * r2, 0};
+            b.W = 1 / (b.X + b.Y + b.Z);
+            Vertex v = interpolate_vertexes(v0, v1, v2, b);
+            /* invoke fragment shader */
+            wk->cnt[ORC_CNT_SHADED]++;
+            Col color = dc->fragment(&dc->shader, &v);
+            if (cdiscard(color)) continue;
+            /* update buffers atomically */
+            pthread_mutex_t *lock = &dc->locks[(x + y) & 255];
+            if (dc->threaded) pthread_mutex_lock(lock);
+            if (bz <= load_depth(dc, i) || !dc->ReadDepth) {
+                info.updated++;
+                if (dc->WriteDepth) store_depth(dc, i, z);
+                if (dc->WriteColor) {
+                    if (dc->AlphaBlend && color.A < 1) {
+                        uint32_t sr, sg, sb, sa;
+                        nrgba_rgba(cnrgba(color), &sr, &sg, &sb, &sa);
+                        uint32_t a = (0xffff - sa) * 0x101;
+                        int64_t j = (y * W + x) * 4; /* PixOffset(x, y) */
+                        uint8_t *dr = &dc->Pix[j + 0];
+                        uint8_t *dg = &dc->Pix[j + 1];
+                        uint8_t *db = &dc->Pix[j + 2];
+                        uint8_t *da = &dc->Pix[j + 3];
+                        *dr = (uint8_t)(((uint32_t)*dr * a / 0xffff + sr) >> 8);
+                        *dg = (uint8_t)(((uint32_t)*dg * a / 0xffff + sg) >> 8);
+                        *db = (uint8_t)(((uint32_t)*db * a / 0xffff + sb) >> 8);
+                        *da = (uint8_t)(((uint32_t)*da * a / 0xffff + sa) >> 8);
+                    } else if (x >= 0 && y >= 0 && x < W && y < dc->Height) {
+                        /* SetNRGBA bounds-checks x and y */
+                        NRGBA8 c = cnrgba(color);
+                        int64_t j = (y * W + x) * 4;
+                        dc->Pix[j + 0] = c.R;
+                        dc->Pix[j + 1] = c.G;
+                        dc->Pix[j + 2] = c.B;
+                        dc->Pix[j + 3] = c.A;
+                    }
+                }
+            }
+            if (dc->threaded) pthread_mutex_unlock(lock);
+        }
+        w00 += b12;
+        w01 += b20;
+        w02 += b01;
+    }
+    return info;
+}
+
+static Info info_add(Info a, Info b) { return (Info){a.total + b.total, a.updated + b.updated}; }
+
+/* context.go:283-294 */
+static Info line(Worker *wk, const Vertex *v0, const Vertex *v1, Vec s0, Vec s1) {
+    orc_ctx *dc = wk->dc;
+    Vec n = vmuls(vperpendicular(vsub(s1, s0)), dc->LineWidth / 2);
+    s0 = vadd(s0, vmuls(vnormalize(vsub(s0, s1)), dc->LineWidth / 2));
+    s1 = vadd(s1, vmuls(vnormalize(vsub(s1, s0)), dc->LineWidth / 2));
+    Vec s00 = vadd(s0, n);
+    Vec s01 = vsub(s0, n);
+    Vec s10 = vadd(s1, n);
+    Vec s11 = vsub(s1, n);
+    Info info1 = rasterize(wk, v1, v0, v0, s11, s01, s00);
+    Info info2 = rasterize(wk, v1, v1, v0, s10, s11, s00);
+    return info_add(info1, info2);
+}
+
+/* context.go:296-301 */
+static Info wireframe(Worker *wk, const Vertex *v0, const Vertex *v1, const Vertex *v2, Vec s0, Vec s1, Vec s2) {
+    Info info1 = line(wk, v0, v1, s0, s1);
+    Info info2 = line(wk, v1, v2, s1, s2);
+    Info info3 = line(wk, v2, v0, s2, s0);
+    return info_add(info_add(info1, info2), info3);
+}
+
+/* context.go:303-314 */
+static Info draw_clipped_line(Worker *wk, const Vertex *v0, const Vertex *v1) {
+    orc_ctx *dc = wk->dc;
+    Vec ndc0 = wvec(wdivs(v0->Output, v0->Output.W));
+    Vec ndc1 = wvec(wdivs(v1->Output, v1->Output.W));
+    Vec s0 = mat_mul_position(dc->screenMatrix, ndc0);
+    Vec s1 = mat_mul_position(dc->screenMatrix, ndc1);
+    return line(wk, v0, v1, s0, s1);
+}
+
+/* context.go:316-349 */
+static Info draw_clipped_triangle(Worker *wk, const Vertex *v0in, const Vertex *v1in, const Vertex *v2in) {
+    orc_ctx *dc = wk->dc;
+    const Vertex *v0 = v0in, *v1 = v1in, *v2 = v2in;
+    Vec ndc0 = wvec(wdivs(v0->Output, v0->Output.W));
+    Vec ndc1 = wvec(wdivs(v1->Output, v1->Output.W));
+    Vec ndc2 = wvec(wdivs(v2->Output, v2->Output.W));
+    /* back face culling */
+    double a = (ndc1.X - ndc0.X) * (ndc2.Y - ndc0.Y) - (ndc2.X - ndc0.X) * (ndc1.Y - ndc0.Y);
+    if (a < 0) {
+        const Vertex *tv = v0; v0 = v2; v2 = tv;
+        Vec tn = ndc0; ndc0 = ndc2; ndc2 = tn;
+    }
+    if (dc->Cull == ORC_CULL_FRONT) a = -a;
+    if (dc->FrontFace == ORC_FACE_CW) a = -a;
+    if (dc->Cull != ORC_CULL_NONE && a <= 0) return (Info){0, 0};
+    /* screen coordinates */
+    Vec s0 = mat_mul_position(dc->screenMatrix, ndc0);
+    Vec s1 = mat_mul_position(dc->screenMatrix, ndc1);
+    Vec s2 = mat_mul_position(dc->screenMatrix, ndc2);
+    if (dc->Wireframe) return wireframe(wk, v0, v1, v2, s0, s1, s2);
+    return rasterize(wk, v0, v1, v2, s0, s1, s2);
+}
+
+/* context.go:351-368 */
+static Info draw_line(Worker *wk, const Line *t) {
+    orc_ctx *dc = wk->dc;
+    wk->cnt[ORC_CNT_INPUT]++;
+    Vertex v1 = shader_vertex(&dc->shader, t->V1);
+    Vertex v2 = shader_vertex(&dc->shader, t->V2);
+    if (woutside(v1.Output) || woutside(v2.Output)) {
+        wk->cnt[ORC_CNT_CLIPPED]++;
+        Line in = {v1, v2}, out;
+        if (clip_line(&in, &out)) return draw_clipped_line(wk, &out.V1, &out.V2);
+        return (Info){0, 0};
+    }
+    return draw_clipped_line(wk, &v1, &v2);
+}
+
+/* context.go:370-389 */
+static Info draw_triangle(Worker *wk, const Triangle *t) {
+    orc_ctx *dc = wk->dc;
+    wk->cnt[ORC_CNT_INPUT]++;
+    Vertex v1 = shader_vertex(&dc->shader, t->V1);
+    Vertex v2 = shader_vertex(&dc->shader, t->V2);
+    Vertex v3 = shader_vertex(&dc->shader, t->V3);
+    if (woutside(v1.Output) || woutside(v2.Output) || woutside(v3.Output)) {
+        wk->cnt[ORC_CNT_CLIPPED]++;
+        Triangle nt = {v1, v2, v3};
+        fix_normals(&nt); /* NewTriangle */
+        Triangle fan[8];
+        int n = clip_triangle(&nt, fan);
+        Info result = {0, 0};
+        for (int k = 0; k < n; k++) result = info_add(result, draw_clipped_triangle(wk, &fan[k].V1, &fan[k].V2, &fan[k].V3));
+        return result;
+    }
+    return draw_clipped_triangle(wk, &v1, &v2, &v3);
+}
+
+/* ------------------------------------------------------------------ */
+/* meshes and the DrawTriangles / DrawLines fan-out (context.go:391-439) */
+
+struct orc_mesh {
+    int64_t ntri, nline;
+    Triangle **tris; /* []*Triangle */
+    Line **lines;    /* []*Line */
+};
+
+static void load_vertex(Vertex *v, int64_t k, const double *pos, const double *nrm, const double *uv,
+                        const double *col) {
+    memset(v, 0, sizeof *v);
+    if (pos) v->Position = (Vec){pos[3 * k], pos[3 * k + 1], pos[3 * k + 2]};
+    if (nrm) v->Normal = (Vec){nrm[3 * k], nrm[3 * k + 1], nrm[3 * k + 2]};
+    if (uv) v->Texture = (Vec){uv[2 * k], uv[2 * k + 1], 0};
+    if (col) v->Color = (Col){col[4 * k], col[4 * k + 1], col[4 * k + 2], col[4 * k + 3]};
+}
+
+orc_mesh *orc_mesh_new(int64_t ntri, const double *tpos, const double *tnrm, const double *tuv,
+                       const double *tcol, int64_t nline, const double *lpos, const double *lnrm,
+                       const double *luv, const double *lcol) {
+    orc_mesh *m = (orc_mesh *)calloc(1, sizeof *m);
+    m->ntri = ntri;
+    m->nline = nline;
+    m->tris = (Triangle **)calloc(ntri > 0 ? ntri : 1, sizeof(Triangle *));
+    m->lines = (Line **)calloc(nline > 0 ? nline : 1, sizeof(Line *));
+    for (int64_t i = 0; i < ntri; i++) {
+        Triangle *t = (Triangle *)malloc(sizeof *t);
+        load_vertex(&t->V1, 3 * i + 0, tpos, tnrm, tuv, tcol);
+        load_vertex(&t->V2, 3 * i + 1, tpos, tnrm, tuv, tcol);
+        load_vertex(&t->V3, 3 * i + 2, tpos, tnrm, tuv, tcol);
+        m->tris[i] = t;
+    }
+    for (int64_t i = 0; i < nline; i++) {
+        Line *l = (Line *)malloc(sizeof *l);
+        load_vertex(&l->V1, 2 * i + 0, lpos, lnrm, luv, lcol);
+        load_vertex(&l->V2, 2 * i + 1, lpos, lnrm, luv, lcol);
+        m->lines[i] = l;
+    }
+    return m;
+}
+
+void orc_mesh_free(orc_mesh *m) {
+    if (!m) return;
+    for (int64_t i = 0; i < m->ntri; i++) free(m->tris[i]);
+    for (int64_t i = 0; i < m->nline; i++) free(m->lines[i]);
+    free(m->tris);
+    free(m->lines);
+    free(m);
+}
+
+typedef struct {
+    Worker wk;
+    const orc_mesh *m;
+    int wi, wn, lines;
+    Info result;
+} Job;
+
+static void *job_run(void *arg) {
+    Job *j = (Job *)arg;
+    Info result = {0, 0};
+    if (j->lines) {
+        for (int64_t i = 0; i < j->m->nline; i++)
+            if (i % j->wn == j->wi) result = info_add(result, draw_line(&j->wk, j->m->lines[i]));
+    } else {
+        for (int64_t i = 0; i < j->m->ntri; i++)
+            if (i % j->wn == j->wi) result = info_add(result, draw_triangle(&j->wk, j->m->tris[i]));
+    }
+    j->result = result;
+    return NULL;
+}
+
+static Info draw_prims(orc_ctx *c, const orc_mesh *m, int nthreads, int lines) {
+    int wn = nthreads < 1 ? 1 : nthreads;
+    Job *jobs = (Job *)calloc(wn, sizeof(Job));
+    pthread_t *th = (pthread_t *)calloc(wn, sizeof(pthread_t));
+    c->threaded = wn > 1;
+    for (int wi = 0; wi < wn; wi++) {
+        jobs[wi].wk.dc = c;
+        jobs[wi].m = m;
+        jobs[wi].wi = wi;
+        jobs[wi].wn = wn;
+        jobs[wi].lines = lines;
+    }
+    if (wn == 1) {
+        job_run(&jobs[0]);
+    } else {
+        for (int wi = 0; wi < wn; wi++) pthread_create(&th[wi], NULL, job_run, &jobs[wi]);
+        for (int wi = 0; wi < wn; wi++) pthread_join(th[wi], NULL);
+    }
+    Info result = {0, 0};
+    for (int wi = 0; wi < wn; wi++) {
+        result = info_add(result, jobs[wi].result);
+        for (int k = 0; k < ORC_CNT_N; k++) c->counters[k] += jobs[wi].wk.cnt[k];
+    }
+    free(jobs);
+    free(th);
+    return result;
+}
+
+void orc_draw_triangles(orc_ctx *c, const orc_mesh *m, int32_t nthreads, uint64_t info[2]) {
+    Info r = draw_prims(c, m, nthreads, 0);
+    info[0] = r.total;
+    info[1] = r.updated;
+}
+void orc_draw_lines(orc_ctx *c, const orc_mesh *m, int32_t nthreads, uint64_t info[2]) {
+    Info r = draw_prims(c, m, nthreads, 1);
+    info[0] = r.total;
+    info[1] = r.updated;
+}
+/* context.go:435-439 */
+void orc_draw_mesh(orc_ctx *c, const orc_mesh *m, int32_t nthreads, uint64_t info[2]) {
+    Info a = draw_prims(c, m, nthreads, 0);
+    Info b = draw_prims(c, m, nthreads, 1);
+    info[0] = a.total + b.total;
+    info[1] = a.updated + b.updated;
+}
+
+/* ------------------------------------------------------------------ */
+/* context construction, clears, readback */
+
+orc_ctx *orc_ctx_new(int32_t width, int32_t height) { /* context.go:60-81 */
+    orc_ctx *c = (orc_ctx *)calloc(1, sizeof *c);
+    c->Width = width;
+    c->Height = height;
+    size_t n = (size_t)width * (size_t)height;
+    c->Pix = (uint8_t *)calloc(n * 4 + 1, 1);
+    c->Depth = (double *)malloc((n + 1) * sizeof(double));
+    c->shader.kind = ORC_SHADER_SOLID;
+    for (int k = 0; k < 16; k++) c->shader.Matrix.m[k] = (k % 5 == 0) ? 1 : 0;
+    c->shader.Color = (Col){1, 0, 1, 1};
+    c->fragment = solid_fragment;
+    c->ReadDepth = c->WriteDepth = c->WriteColor = c->AlphaBlend = 1;
+    c->Wireframe = 0;
+    c->FrontFace = ORC_FACE_CCW;
+    c->Cull = ORC_CULL_BACK;
+    c->LineWidth = 2;
+    c->DepthBias = 0;
+    c->screenMatrix = mat_screen(width, height);
+    for (int k = 0; k < 256; k++) pthread_mutex_init(&c->locks[k], NULL);
+    orc_clear_depth(c, DBL_MAX);
+    return c;
+}
+
+void orc_ctx_free(orc_ctx *c) {
+    if (!c) return;
+    for (int k = 0; k < 256; k++) pthread_mutex_destroy(&c->locks[k]);
+    free(c->Pix);
+    free(c->Depth);
+    free(c);
+}
+
+void orc_set_state(orc_ctx *c, const orc_state *s) {
+    c->ReadDepth = s->read_depth != 0;
+    c->WriteDepth = s->write_depth != 0;
+    c->WriteColor = s->write_color != 0;
+    c->AlphaBlend = s->alpha_blend != 0;
+    c->Wireframe = s->wireframe != 0;
+    c->FrontFace = s->front_face;
+    c->Cull = s->cull;
+    c->LineWidth = s->line_width;
+    c->DepthBias = s->depth_bias;
+}
+
+void orc_set_shader(orc_ctx *c, const orc_shader *s) {
+    Shader *d = &c->shader;
+    memset(d, 0, sizeof *d);
+    d->kind = s->kind;
+    memcpy(d->Matrix.m, s->matrix, sizeof d->Matrix.m);
+    d->Color = (Col){s->color[0], s->color[1], s->color[2], s->color[3]};
+    d->LightDirection = (Vec){s->light_dir[0], s->light_dir[1], s->light_dir[2]};
+    d->CameraPosition = (Vec){s->camera_pos[0], s->camera_pos[1], s->camera_pos[2]};
+    d->ObjectColor = (Col){s->object_color[0], s->object_color[1], s->object_color[2], s->object_color[3]};
+    d->AmbientColor = (Col){s->ambient[0], s->ambient[1], s->ambient[2], s->ambient[3]};
+    d->DiffuseColor = (Col){s->diffuse[0], s->diffuse[1], s->diffuse[2], s->diffuse[3]};
+    d->SpecularColor = (Col){s->specular[0], s->specular[1], s->specular[2], s->specular[3]};
+    d->SpecularPower = s->specular_power;
+    if (s->texture) {
+        d->tex = *s->texture;
+        d->has_tex = 1;
+    }
+    if (s->kind == ORC_SHADER_TEXTURE) c->fragment = texture_fragment;
+    else if (s->kind == ORC_SHADER_PHONG) c->fragment = phong_fragment;
+    else c->fragment = solid_fragment;
+}
+
+void orc_clear_color(orc_ctx *c, const double rgba[4]) { /* context.go:119-131 */
+    NRGBA8 v = cnrgba((Col){rgba[0], rgba[1], rgba[2], rgba[3]});
+    size_t n = (size_t)c->Width * (size_t)c->Height;
+    for (size_t i = 0; i < n; i++) {
+        c->Pix[4 * i + 0] = v.R;
+        c->Pix[4 * i + 1] = v.G;
+        c->Pix[4 * i + 2] = v.B;
+        c->Pix[4 * i + 3] = v.A;
+    }
+}
+
+void orc_clear_depth(orc_ctx *c, double v) { /* context.go:137-141 */
+    size_t n = (size_t)c->Width * (size_t)c->Height;
+    for (size_t i = 0; i < n; i++) c->Depth[i] = v;
+}
+
+uint8_t *orc_color(orc_ctx *c) { return c->Pix; }
+double *orc_depth(orc_ctx *c) { return c->Depth; }
+
+void orc_depth_image(const orc_ctx *c, uint16_t *out) { /* context.go:87-117 */
+    double lo = DBL_MAX;
+    double hi = -DBL_MAX;
+    size_t n = (size_t)c->Width * (size_t)c->Height;
+    for (size_t i = 0; i < n; i++) {
+        double d = c->Depth[i];
+        if (d == DBL_MAX) continue;
+        if (d < lo) lo = d;
+        if (d > hi) hi = d;
+    }
+    for (size_t i = 0; i < n; i++) {
+        double d = c->Depth[i];
+        double t = (d - lo) / (hi - lo);
+        if (d == DBL_MAX) t = 1;
+        out[i] = go_u16(t * 0xffff);
+    }
+}
+
+void orc_counters(const orc_ctx *c, uint64_t out[ORC_CNT_N]) { memcpy(out, c->counters, sizeof c->counters); }
+void orc_reset_counters(orc_ctx *c) { memset(c->counters, 0, sizeof c->counters); }
+
+/* ------------------------------------------------------------------ */
+/* exported pieces for the pinning tests */
+
+double orc_go_min(double a, double b) { return go_min(a, b); }
+double orc_go_max(double a, double b) { return go_max(a, b); }
+int64_t orc_go_f2i(double x) { return go_f2i(x); }
+double orc_go_pow(double x, double y) { return go_pow(x, y); }
+uint8_t orc_go_u8(double x) { return go_u8(x); }
+
+void orc_color_nrgba(const double c[4], uint8_t out[4]) {
+    NRGBA8 v = cnrgba((Col){c[0], c[1], c[2], c[3]});
+    out[0] = v.R; out[1] = v.G; out[2] = v.B; out[3] = v.A;
+}
+void orc_nrgba_rgba(const uint8_t c[4], uint32_t out[4]) {
+    NRGBA8 v = {c[0], c[1], c[2], c[3]};
+    nrgba_rgba(v, &out[0], &out[1], &out[2], &out[3]);
+}
+void orc_blend(uint8_t dst[4], const double color[4]) {
+    uint32_t sr, sg, sb, sa;
+    nrgba_rgba(cnrgba((Col){color[0], color[1], color[2], color[3]}), &sr, &sg, &sb, &sa);
+    uint32_t a = (0xffff - sa) * 0x101;
+    dst[0] = (uint8_t)(((uint32_t)dst[0] * a / 0xffff + sr) >> 8);
+    dst[1] = (uint8_t)(((uint32_t)dst[1] * a / 0xffff + sg) >> 8);
+    dst[2] = (uint8_t)(((uint32_t)dst[2] * a / 0xffff + sb) >> 8);
+    dst[3] = (uint8_t)(((uint32_t)dst[3] * a / 0xffff + sa) >> 8);
+}
+void orc_barycentric(const double p1[3], const double p2[3], const double p3[3], const double p[3], double out[4]) {
+    VecW b = barycentric((Vec){p1[0], p1[1], p1[2]}, (Vec){p2[0], p2[1], p2[2]}, (Vec){p3[0], p3[1], p3[2]},
+                         (Vec){p[0], p[1], p[2]});
+    out[0] = b.X; out[1] = b.Y; out[2] = b.Z; out[3] = b.W;
+}
+int32_t orc_clip_polygon(const double in[3][4], double out[9][4]) {
+    VecW pts[3], res[16];
+    for (int i = 0; i < 3; i++) pts[i] = (VecW){in[i][0], in[i][1], in[i][2], in[i][3]};
+    int n = sutherland_hodgman(pts, 3, res);
+    for (int i = 0; i < n && i < 9; i++) {
+        out[i][0] = res[i].X; out[i][1] = res[i].Y; out[i][2] = res[i].Z; out[i][3] = res[i].W;
+    }
+    return n;
+}
+int32_t orc_clip_line(const double w1[4], const double w2[4], double out[2][4]) {
+    Line l, o;
+    memset(&l, 0, sizeof l);
+    l.V1.Output = (VecW){w1[0], w1[1], w1[2], w1[3]};
+    l.V2.Output = (VecW){w2[0], w2[1], w2[2], w2[3]};
+    if (!clip_line(&l, &o)) return 0;
+    out[0][0] = o.V1.Output.X; out[0][1] = o.V1.Output.Y; out[0][2] = o.V1.Output.Z; out[0][3] = o.V1.Output.W;
+    out[1][0] = o.V2.Output.X; out[1][1] = o.V2.Output.Y; out[1][2] = o.V2.Output.Z; out[1][3] = o.V2.Output.W;
+    return 1;
+}
+void orc_matrix_mul(const double a[16], const double b[16], double out[16]) {
+    Mat x, y;
+    memcpy(x.m, a, sizeof x.m);
+    memcpy(y.m, b, sizeof y.m);
+    Mat o = mat_mul(x, y);
+    memcpy(out, o.m, sizeof o.m);
+}
+void orc_mul_position_w(const double m[16], const double p[3], double out[4]) {
+    Mat x;
+    memcpy(x.m, m, sizeof x.m);
+    VecW o = mat_mul_position_w(x, (Vec){p[0], p[1], p[2]});
+    out[0] = o.X; out[1] = o.Y; out[2] = o.Z; out[3] = o.W;
+}
+void orc_look_at(const double eye[3], const double center[3], const double up[3], double out[16]) {
+    Mat o = mat_look_at((Vec){eye[0], eye[1], eye[2]}, (Vec){center[0], center[1], center[2]},
+                        (Vec){up[0], up[1], up[2]});
+    memcpy(out, o.m, sizeof o.m);
+}
+void orc_frustum(double l, double r, double b, double t, double n, double f, double out[16]) {
+    Mat o = mat_frustum(l, r, b, t, n, f);
+    memcpy(out, o.m, sizeof o.m);
+}
+void orc_perspective(double fovy, double aspect, double near, double far, double out[16]) {
+    Mat o = mat_perspective(fovy, aspect, near, far);
+    memcpy(out, o.m, sizeof o.m);
+}
+void orc_screen(int32_t w, int32_t h, double out[16]) {
+    Mat o = mat_screen(w, h);
+    memcpy(out, o.m, sizeof o.m);
+}
+void orc_bilinear_sample(const orc_texture *t, double u, double v, double out[4]) {
+    Col c = bilinear_sample(t, u, v);
+    out[0] = c.R; out[1] = c.G; out[2] = c.B; out[3] = c.A;
+}
+void orc_phong_fragment(const orc_shader *s, const double pos[3], const double nrm[3], const double uv[2],
+                        const double col[4], double out[4]) {
+    orc_ctx tmp;
+    memset(&tmp, 0, sizeof tmp);
+    orc_set_shader(&tmp, s);
+    Vertex v;
+    memset(&v, 0, sizeof v);
+    v.Position = (Vec){pos[0], pos[1], pos[2]};
+    v.Normal = (Vec){nrm[0], nrm[1], nrm[2]};
+    v.Texture = (Vec){uv[0], uv[1], 0};
+    v.Color = (Col){col[0], col[1], col[2], col[3]};
+    Col c = phong_fragment(&tmp.shader, &v);
+    out[0] = c.R; out[1] = c.G; out[2] = c.B; out[3] = c.A;
+}
+void orc_rasterize_screen(orc_ctx *c, const double s0[3], const double s1[3], const double s2[3],
+                          const double w[3], uint64_t info[2]) {
+    Worker wk;
+    memset(&wk, 0, sizeof wk);
+    wk.dc = c;
+    c->threaded = 0;
+    Vertex v[3];
+    memset(v, 0, sizeof v);
+    for (int k = 0; k < 3; k++) v[k].Output.W = w[k];
+    Info r = rasterize(&wk, &v[0], &v[1], &v[2], (Vec){s0[0], s0[1], s0[2]}, (Vec){s1[0], s1[1], s1[2]},
+                       (Vec){s2[0], s2[1], s2[2]});
+    for (int k = 0; k < ORC_CNT_N; k++) c->counters[k] += wk.cnt[k];
+    info[0] = r.total;
+    info[1] = r.updated;
+}
