@@ -7,7 +7,9 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libfauxgl_amd.so")
+# FAUXGL_AMD_LIB selects another build of the same library (e.g. one built
+# with different tile constants); the default is the in-tree build.
+LIB_PATH = os.environ.get("FAUXGL_AMD_LIB") or os.path.join(HERE, "libfauxgl_amd.so")
 
 FGL_OK = 0
 FGL_E_INVALID = -1

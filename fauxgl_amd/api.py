@@ -287,12 +287,13 @@ class ImageTexture:
         self._handles = {}
 
     def handle(self, ctx):
-        key = ctx._ptr()
+        ptr = ctx._ptr()
+        key = ptr.value
         h = self._handles.get(key)
         if h is None:
             out = ctypes.c_int64(0)
-            N.check(N.load().fgl_texture_create(key, self.Width, self.Height, self._kinds[self.kind],
-                                                self.Image.ctypes.data_as(ctypes.c_void_p), ctypes.byref(out)), key)
+            N.check(N.load().fgl_texture_create(ptr, self.Width, self.Height, self._kinds[self.kind],
+                                                self.Image.ctypes.data_as(ctypes.c_void_p), ctypes.byref(out)), ptr)
             h = out.value
             self._handles[key] = h
             ctx._textures.append(self)

@@ -14,8 +14,14 @@ namespace fgl {
 // ---------------------------------------------------------------------------
 // Tiling constants.  A tile is the unit a workgroup owns in LDS.
 
-constexpr int TILE_W = 32;
-constexpr int TILE_H = 32;
+#ifndef FGL_TILE_W
+#define FGL_TILE_W 32
+#endif
+#ifndef FGL_TILE_H
+#define FGL_TILE_H 32
+#endif
+constexpr int TILE_W = FGL_TILE_W;
+constexpr int TILE_H = FGL_TILE_H;
 constexpr int TILE_PIX = TILE_W * TILE_H;
 constexpr int TILE_THREADS = 256;
 // Raster triangles whose integer bounding box leaves [-BBOX_LIM, BBOX_LIM]
@@ -88,6 +94,7 @@ struct DrawCounters {
     uint32_t overflow;           // bit 0 side records, 1 side vertices, 2 pairs, 3 fragment buffer
     uint32_t pairs_lo, pairs_hi; // unclamped pair count
     uint32_t pad;
+    unsigned long long phase[8];  // tile-kernel phase cycles (instrumented builds only)
 };
 
 // ---------------------------------------------------------------------------

@@ -90,6 +90,7 @@ struct fgl_ctx {
     DevBuf rs_tmp, rs_wx, rs_ox, rs_wy, rs_oy;
     DevBuf dpart, dimg;
     uint32_t side_rec_cap = 0, side_vtx_cap = 0, pair_cap = 0;
+    DrawCounters last{};
 
     bool prof = false;
     std::map<std::string, KStat> kstats;
@@ -385,6 +386,7 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
         HIPCHK(c, hipMemcpyAsync(&h, a.ctr, sizeof h, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         if (c->prof) prof_collect(c);
+        c->last = h;
         if (h.overflow == 0) {
             if (info) {
                 info[0] = h.total;
@@ -898,6 +900,15 @@ int fgl_profile_reset(fgl_ctx *ctx) {
     ctx->kstats.clear();
     return FGL_OK;
 }
+
+#ifdef FGL_TILE_PHASE_TIMERS
+// Profiling builds only: tile-kernel phase cycles of the last draw.
+int fgl_debug_phase_cycles(fgl_ctx *ctx, uint64_t out[8]) {
+    if (check_ctx(ctx)) return FGL_E_INVALID;
+    for (int k = 0; k < 8; k++) out[k] = ctx->last.phase[k];
+    return FGL_OK;
+}
+#endif
 
 int fgl_ctx_sync(fgl_ctx *ctx) {
     if (check_ctx(ctx)) return FGL_E_INVALID;

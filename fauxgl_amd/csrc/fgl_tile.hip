@@ -24,8 +24,35 @@
 
 namespace fgl {
 
-constexpr int FRAG_CAP = 1024;
-constexpr int MAX_ITEMS = 512;
+#ifndef FGL_FRAG_CAP
+#define FGL_FRAG_CAP 1024
+#endif
+#ifndef FGL_MAX_ITEMS
+#define FGL_MAX_ITEMS 512
+#endif
+constexpr int FRAG_CAP = FGL_FRAG_CAP;
+constexpr int MAX_ITEMS = FGL_MAX_ITEMS;
+
+// Phase timers for profiling builds (-DFGL_TILE_PHASE_TIMERS): thread 0 of
+// each workgroup accumulates shader-clock cycles per phase into the draw
+// counters.  They add no work to the default build.
+#ifdef FGL_TILE_PHASE_TIMERS
+#define PT_DECL uint64_t pt_last = 0, pt_acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#define PT_START if (tid == 0) pt_last = clock64();
+#define PT_MARK(k)                           \
+    if (tid == 0) {                          \
+        uint64_t now_ = clock64();           \
+        pt_acc[k] += now_ - pt_last;         \
+        pt_last = now_;                      \
+    }
+#define PT_FLUSH \
+    if (tid == 0) for (int k_ = 0; k_ < 8; k_++) atomicAdd(&a.ctr->phase[k_], (unsigned long long)pt_acc[k_]);
+#else
+#define PT_DECL
+#define PT_START
+#define PT_MARK(k)
+#define PT_FLUSH
+#endif
 constexpr int BATCH = TILE_THREADS;
 constexpr uint32_t OWNER_NONE = 0xffffffffu;
 
@@ -333,6 +360,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(DrawArgs a) {
     const int cx0 = tx * TILE_W, cy0 = ty * TILE_H;
     const int cw = min(TILE_W, a.W - cx0), chh = min(TILE_H, a.H - cy0);
     const int cx1 = cx0 + cw - 1, cy1 = cy0 + chh - 1;
+    PT_DECL
+    PT_START
     const uint32_t ls = a.tstart[t], le = a.tend[t];
     const bool has = le > ls;
     if (!has) {
@@ -375,6 +404,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(DrawArgs a) {
 
     uint32_t lp = ls, io = 0;
     __syncthreads();
+    PT_MARK(0)
     while (lp < le) {
         // ---- batch formation: items of triangles lp, lp+1, ... that fit ----
         uint32_t idx = lp + (uint32_t)tid;
@@ -440,6 +470,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(DrawArgs a) {
         __syncthreads();
         const uint32_t nitems = s.nitems;
         const uint32_t io0 = io;
+        PT_MARK(1)
 
         // ---- A: coverage walk per (triangle, row) item ----
         for (uint32_t m = (uint32_t)tid; m < nitems; m += TILE_THREADS) {
@@ -543,6 +574,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(DrawArgs a) {
             }
         }
         __syncthreads();
+        PT_MARK(2)
 
         // ---- B: shading ----
         uint32_t nf = s.nfrag;
@@ -576,6 +608,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(DrawArgs a) {
             s.fpix[f] = fp;
         }
         __syncthreads();
+        PT_MARK(3)
 
         // ---- C: ordered write, lowest key per pixel first ----
         const bool rd = a.st.read_depth != 0, wd = a.st.write_depth != 0, wc = a.st.write_color != 0;
@@ -613,6 +646,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(DrawArgs a) {
         lp = s.next_lp;
         io = s.next_io;
         __syncthreads();
+        PT_MARK(4)
     }
 
     // write the tile back
@@ -627,6 +661,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(DrawArgs a) {
     atomicAdd(&s.total, (unsigned long long)my_total);
     atomicAdd(&s.updated, (unsigned long long)my_updated);
     __syncthreads();
+    PT_MARK(5)
+    PT_FLUSH
     if (tid == 0) {
         atomicAdd(&a.ctr->total, s.total);
         atomicAdd(&a.ctr->updated, s.updated);
