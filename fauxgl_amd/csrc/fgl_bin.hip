@@ -148,30 +148,44 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(DrawArgs a, int pass)
     a.hist[(uint64_t)threadIdx.x * a.nrsblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-__global__ __launch_bounds__(SCAN_THREADS) void k_radix_scan(DrawArgs a) {
-    __shared__ uint32_t part[SCAN_THREADS];
-    if (a.ctr->overflow) return;
-    const int64_t n = (int64_t)256 * a.nrsblocks;
-    const int tid = threadIdx.x;
-    const int64_t per = (n + SCAN_THREADS - 1) / SCAN_THREADS;
-    const int64_t lo = tid * per;
-    const int64_t hi = lo + per < n ? lo + per : n;
-    uint32_t s = 0;
-    for (int64_t k = lo; k < hi; k++) s += a.hist[k];
-    part[tid] = s;
+// Block-wide exclusive scan of one value per thread (256 threads); returns
+// the exclusive prefix and writes the block total to *total.
+__device__ uint32_t block_exclusive_scan_256(uint32_t v, uint32_t *wsum, uint32_t *total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
     __syncthreads();
-    for (int off = 1; off < SCAN_THREADS; off <<= 1) {
-        uint32_t v = tid >= off ? part[tid - off] : 0;
-        __syncthreads();
-        part[tid] += v;
-        __syncthreads();
+    uint32_t base = 0, tot = 0;
+    for (int w = 0; w < 4; w++) {
+        if (w < wave) base += wsum[w];
+        tot += wsum[w];
     }
-    uint32_t run = part[tid] - s;
-    for (int64_t k = lo; k < hi; k++) {
-        uint32_t v = a.hist[k];
-        a.hist[k] = run;
-        run += v;
+    __syncthreads();
+    *total = tot;
+    return base + x - v;
+}
+
+// One workgroup per digit: exclusive scan of that digit's per-block counts
+// (hist is digit-major) and the digit total.
+__global__ __launch_bounds__(RS_THREADS) void k_radix_scan(DrawArgs a) {
+    __shared__ uint32_t wsum[4];
+    if (a.ctr->overflow) return;
+    const int d = blockIdx.x;
+    uint32_t *row = a.hist + (uint64_t)d * a.nrsblocks;
+    uint32_t carry = 0;
+    for (int base = 0; base < a.nrsblocks; base += RS_THREADS) {
+        int k = base + threadIdx.x;
+        uint32_t v = k < a.nrsblocks ? row[k] : 0u;
+        uint32_t tot;
+        uint32_t ex = block_exclusive_scan_256(v, wsum, &tot);
+        if (k < a.nrsblocks) row[k] = carry + ex;
+        carry += tot;
     }
+    if (threadIdx.x == 0) a.dtot[d] = carry;
 }
 
 __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(DrawArgs a, int pass) {
@@ -186,7 +200,13 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(DrawArgs a, int pa
     const int shift = 8 * pass;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint64_t ltmask = (1ull << lane) - 1ull;
-    run[tid] = a.hist[(uint64_t)tid * a.nrsblocks + blockIdx.x];
+    {
+        // digit d starts after all pairs with a smaller digit
+        __shared__ uint32_t wsum[4];
+        uint32_t tot;
+        uint32_t dbase = block_exclusive_scan_256(a.dtot[tid], wsum, &tot);
+        run[tid] = dbase + a.hist[(uint64_t)tid * a.nrsblocks + blockIdx.x];
+    }
     const uint64_t base = (uint64_t)blockIdx.x * RS_BLOCK;
     for (int it = 0; it < RS_ITEMS; it++) {
         for (int w = 0; w < RS_THREADS / 64; w++) wcnt[w][tid] = 0;
@@ -230,7 +250,7 @@ hipError_t launch_radix_hist(const DrawArgs &a, int pass, hipStream_t s) {
     return hipGetLastError();
 }
 hipError_t launch_radix_scan(const DrawArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(SCAN_THREADS), 0, s, a);
+    hipLaunchKernelGGL(k_radix_scan, dim3(256), dim3(RS_THREADS), 0, s, a);
     return hipGetLastError();
 }
 hipError_t launch_radix_scatter(const DrawArgs &a, int pass, hipStream_t s) {

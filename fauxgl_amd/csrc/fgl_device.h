@@ -89,7 +89,7 @@ struct DrawCounters {
     unsigned long long updated;  // UpdatedPixels
     uint32_t side_rec;           // side raster records allocated
     uint32_t side_vtx;           // side vertices allocated
-    uint32_t nprim;              // raster triangles emitted
+    uint32_t ndeferred;          // input triangles deferred to the clipping kernel
     uint32_t npairs;             // (tile, triangle) pairs emitted
     uint32_t overflow;           // bit 0 side records, 1 side vertices, 2 pairs, 3 fragment buffer
     uint32_t pairs_lo, pairs_hi; // unclamped pair count

@@ -85,7 +85,7 @@ struct fgl_ctx {
     int64_t next_tex = 1;
 
     DevBuf ctr, ntl, nrast, sstart, blocksum, blockoff, rec_main, side_rec, side_vtx;
-    DevBuf keys0, vals0, keys1, vals1, hist, tstart, tend;
+    DevBuf keys0, vals0, keys1, vals1, hist, dtot, tstart, tend, deferred;
     DevBuf in_pos, in_nrm, in_uv, in_col;
     DevBuf rs_tmp, rs_wx, rs_ox, rs_wy, rs_oy;
     DevBuf dpart, dimg;
@@ -333,6 +333,8 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
     HIPCHK(c, c->blocksum.ensure((size_t)a.nvsblocks * 8));
     HIPCHK(c, c->blockoff.ensure((size_t)a.nvsblocks * 8));
     if (kind == PRIM_TRI) HIPCHK(c, c->rec_main.ensure((size_t)n * sizeof(PrimRec)));
+    if (kind == PRIM_TRI) HIPCHK(c, c->deferred.ensure((size_t)n * 4));
+    HIPCHK(c, c->dtot.ensure(256 * 4));
     HIPCHK(c, c->tstart.ensure((size_t)a.ntiles * 4));
     HIPCHK(c, c->tend.ensure((size_t)a.ntiles * 4));
 
@@ -362,6 +364,8 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
         a.vals[1] = c->vals1.as<uint32_t>();
         a.pair_cap = c->pair_cap;
         a.hist = c->hist.as<uint32_t>();
+        a.dtot = c->dtot.as<uint32_t>();
+        a.deferred = c->deferred.as<uint32_t>();
         a.tstart = c->tstart.as<uint32_t>();
         a.tend = c->tend.as<uint32_t>();
         a.clear_color_pending = c->pend_color;
@@ -372,6 +376,8 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
 
         if ((r = launch(c, "zero", [&] { return launch_zero(a, c->stream); }))) return r;
         if ((r = launch(c, "vertex", [&] { return launch_vs(a, c->stream); }))) return r;
+        if (kind == PRIM_TRI && !a.st.wireframe)
+            if ((r = launch(c, "vertex_clip", [&] { return launch_vs_deferred(a, c->stream); }))) return r;
         if ((r = launch(c, "scan", [&] { return launch_scan_top(a, c->stream); }))) return r;
         if ((r = launch(c, "emit", [&] { return launch_emit(a, c->stream); }))) return r;
         for (int p = 0; p < npass; p++) {

@@ -40,11 +40,13 @@ struct DrawArgs {
     uint64_t *blocksum;    // per vertex-stage block: sum of ntl
     uint64_t *blockoff;    // exclusive prefix of blocksum
     int32_t nvsblocks;
+    uint32_t *deferred;    // input triangles that need clipping (in.n slots)
 
     // binning
     uint32_t *keys[2], *vals[2];  // ping-pong (tile id, triangle ref) pairs
     uint32_t pair_cap;
     uint32_t *hist;        // radix histograms, 256 * nrsblocks
+    uint32_t *dtot;        // per-digit totals of the current radix pass
     int32_t nrsblocks;
     int32_t key_bits;
     uint32_t *tstart, *tend;  // per tile range in the sorted pairs
@@ -69,6 +71,7 @@ constexpr int RS_BLOCK = RS_THREADS * RS_ITEMS;
 
 hipError_t launch_zero(const DrawArgs &a, hipStream_t s);
 hipError_t launch_vs(const DrawArgs &a, hipStream_t s);
+hipError_t launch_vs_deferred(const DrawArgs &a, hipStream_t s);
 hipError_t launch_scan_top(const DrawArgs &a, hipStream_t s);
 hipError_t launch_emit(const DrawArgs &a, hipStream_t s);
 hipError_t launch_radix_hist(const DrawArgs &a, int pass, hipStream_t s);

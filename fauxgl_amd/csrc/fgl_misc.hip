@@ -118,12 +118,91 @@ __global__ void k_resolve_box(const uint32_t *src, int32_t sw, uint32_t *dst, in
     }
 }
 
+// Both tent passes in one kernel: a workgroup owns RT_OW x RT_OH output
+// pixels, runs the horizontal pass over exactly the input rows its vertical
+// taps reach (keeping those rows in LDS), then the vertical pass.  Each row
+// is computed with the same integer steps as k_resolve_h / k_resolve_v, so
+// the result is identical to the two-pass form.
+constexpr int RT_OW = 64, RT_OH = 4, RT_MAXROWS = 48;
+
+__global__ __launch_bounds__(256) void k_resolve_tent(const uint32_t *src, int32_t sw, int32_t sh, uint32_t *dst,
+                                                      int32_t dw, int32_t dh, const int16_t *wx, const int32_t *ox,
+                                                      int32_t flx, const int16_t *wy, const int32_t *oy, int32_t fly) {
+    __shared__ uint32_t hrow[RT_MAXROWS][RT_OW];
+    const int x0 = blockIdx.x * RT_OW, y0 = blockIdx.y * RT_OH;
+    const int nx = min(RT_OW, dw - x0), ny = min(RT_OH, dh - y0);
+    const int r0 = oy[y0];
+    const int nrows = oy[y0 + ny - 1] + fly - r0;
+    for (int q = threadIdx.x; q < nrows * RT_OW; q += blockDim.x) {
+        int j = q / RT_OW, x = q % RT_OW;
+        if (x >= nx) continue;
+        int ry = r0 + j;
+        if (ry < 0) ry = 0;
+        if (ry >= sh - 1) ry = sh - 1;
+        const uint32_t *row = src + (int64_t)ry * sw;
+        int32_t acc[4] = {0, 0, 0, 0};
+        int32_t sum = 0;
+        const int gx = x0 + x;
+        const int32_t start = ox[gx];
+        for (int32_t k = 0; k < flx; k++) {
+            int32_t c = wx[gx * flx + k];
+            if (c == 0) continue;
+            int32_t xi = start + k;
+            if (xi < 0) xi = 0;
+            if (xi >= sw - 1) xi = sw - 1;
+            uint32_t p = row[xi];
+            int32_t a = (int32_t)(p >> 24);
+            acc[0] += c * ((int32_t)(p & 0xff) * a / 0xff);
+            acc[1] += c * ((int32_t)((p >> 8) & 0xff) * a / 0xff);
+            acc[2] += c * ((int32_t)((p >> 16) & 0xff) * a / 0xff);
+            acc[3] += c * a;
+            sum += c;
+        }
+        uint32_t o = 0;
+        for (int ch = 0; ch < 4; ch++) {
+            int32_t v = sum ? acc[ch] / sum : 0;
+            v = v < 0 ? 0 : (v > 255 ? 255 : v);
+            o |= (uint32_t)v << (8 * ch);
+        }
+        hrow[j][x] = o;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < RT_OH * RT_OW; q += blockDim.x) {
+        int yy = q / RT_OW, x = q % RT_OW;
+        if (x >= nx || yy >= ny) continue;
+        const int gy = y0 + yy;
+        int32_t acc[4] = {0, 0, 0, 0};
+        int32_t sum = 0;
+        const int32_t start = oy[gy];
+        for (int32_t k = 0; k < fly; k++) {
+            int32_t c = wy[gy * fly + k];
+            if (c == 0) continue;
+            uint32_t p = hrow[start + k - r0][x];
+            for (int ch = 0; ch < 4; ch++) acc[ch] += c * (int32_t)((p >> (8 * ch)) & 0xff);
+            sum += c;
+        }
+        uint32_t o = 0;
+        for (int ch = 0; ch < 4; ch++) {
+            int32_t v = sum ? acc[ch] / sum : 0;
+            v = v < 0 ? 0 : (v > 255 ? 255 : v);
+            o |= (uint32_t)v << (8 * ch);
+        }
+        dst[(int64_t)gy * dw + x0 + x] = o;
+    }
+}
+
 hipError_t launch_resolve(const uint32_t *src, int32_t sw, int32_t sh, uint32_t *tmp, uint32_t *dst, int32_t dw,
                           int32_t dh, int32_t filter, const int16_t *wx, const int32_t *ox, int32_t flx,
                           const int16_t *wy, const int32_t *oy, int32_t fly, hipStream_t s) {
     if (filter == 2) {
         hipLaunchKernelGGL(k_resolve_box, dim3(grid_for((int64_t)dw * dh)), dim3(256), 0, s, src, sw, dst, dw, dh,
                            sw / dw);
+        return hipGetLastError();
+    }
+    // rows one output tile's vertical taps reach: (RT_OH - 1) * step + fly
+    if ((RT_OH - 1) * (sh / dh) + fly <= RT_MAXROWS) {
+        dim3 grid((dw + RT_OW - 1) / RT_OW, (dh + RT_OH - 1) / RT_OH);
+        hipLaunchKernelGGL(k_resolve_tent, grid, dim3(256), 0, s, src, sw, sh, dst, dw, dh, wx, ox, flx, wy, oy, fly);
         return hipGetLastError();
     }
     hipLaunchKernelGGL(k_resolve_h, dim3(grid_for((int64_t)dw * sh)), dim3(256), 0, s, src, sw, sh, tmp, dw, wx, ox,

@@ -314,7 +314,32 @@ FGL_D bool alloc_side(const DrawArgs &a, uint32_t nrec, uint32_t nvtx, uint32_t 
     return ok;
 }
 
-// One input triangle (DrawTriangle, context.go:370-389).
+// The clipped branch of DrawTriangle (context.go:376-384):
+// ClipTriangle(NewTriangle(v1, v2, v3)) and each fan triangle in order.
+__device__ __noinline__ void vs_triangle_clip(const DrawArgs &a, int64_t i, V4 o0, V4 o1, V4 o2, uint32_t &nr,
+                                              uint32_t &nt, uint32_t &ss) {
+    Vtx V[3];
+    V[0] = load_vertex(a.in, 3 * i);
+    V[1] = load_vertex(a.in, 3 * i + 1);
+    V[2] = load_vertex(a.in, 3 * i + 2);
+    V[0].out = o0;
+    V[1].out = o1;
+    V[2].out = o2;
+    fix_normals(V);
+    Emitter cnt{nullptr, 0, 0};
+    uint32_t nfan = 0;
+    clip_fans(a, V, cnt, 0, false, &nfan);
+    if (cnt.n == 0) return;
+    uint32_t rb, vb;
+    if (!alloc_side(a, cnt.n, 3 * nfan, rb, vb)) return;
+    Emitter em{a.side_rec + rb, 0, 0};
+    clip_fans(a, V, em, vb, true, nullptr);
+    nr = em.n | REF_SIDE;
+    nt = em.ntiles;
+    ss = rb;
+}
+
+// One input triangle (DrawTriangle, context.go:370-389), any state.
 FGL_D void vs_triangle(const DrawArgs &a, int64_t i, uint32_t &nr, uint32_t &nt, uint32_t &ss) {
     V3 P0 = load_pos(a.in, 3 * i), P1 = load_pos(a.in, 3 * i + 1), P2 = load_pos(a.in, 3 * i + 2);
     V4 o0 = mul_position_w(a.sh.M, P0);
@@ -341,26 +366,61 @@ FGL_D void vs_triangle(const DrawArgs &a, int64_t i, uint32_t &nr, uint32_t &nt,
         ss = rb;
         return;
     }
-    // clip to viewing volume (ClipTriangle(NewTriangle(v1, v2, v3)))
-    Vtx V[3];
-    V[0] = load_vertex(a.in, 3 * i);
-    V[1] = load_vertex(a.in, 3 * i + 1);
-    V[2] = load_vertex(a.in, 3 * i + 2);
-    V[0].out = o0;
-    V[1].out = o1;
-    V[2].out = o2;
-    fix_normals(V);
-    Emitter cnt{nullptr, 0, 0};
-    uint32_t nfan = 0;
-    clip_fans(a, V, cnt, 0, false, &nfan);
-    if (cnt.n == 0) return;
-    uint32_t rb, vb;
-    if (!alloc_side(a, cnt.n, 3 * nfan, rb, vb)) return;
-    Emitter em{a.side_rec + rb, 0, 0};
-    clip_fans(a, V, em, vb, true, nullptr);
-    nr = em.n | REF_SIDE;
-    nt = em.ntiles;
-    ss = rb;
+    vs_triangle_clip(a, i, o0, o1, o2, nr, nt, ss);
+}
+
+// Filled triangles without clipping in one lean pass; triangles with a
+// vertex outside the view volume are queued for k_vs_deferred, which fills
+// in their counts afterwards.
+__global__ __launch_bounds__(VS_THREADS) void k_vs_fast(DrawArgs a) {
+    __shared__ uint64_t wsum[VS_THREADS / 64];
+    int64_t i = (int64_t)blockIdx.x * VS_THREADS + threadIdx.x;
+    uint32_t nr = 0, nt = 0;
+    if (i < a.in.n) {
+        V3 P0 = load_pos(a.in, 3 * i), P1 = load_pos(a.in, 3 * i + 1), P2 = load_pos(a.in, 3 * i + 2);
+        V4 o0 = mul_position_w(a.sh.M, P0);
+        V4 o1 = mul_position_w(a.sh.M, P1);
+        V4 o2 = mul_position_w(a.sh.M, P2);
+        if (woutside(o0) || woutside(o1) || woutside(o2)) {
+            uint32_t slot = atomicAdd(&a.ctr->ndeferred, 1u);
+            a.deferred[slot] = (uint32_t)i;
+        } else {
+            const uint32_t ident[3] = {0, 1, 2};
+            Emitter em{a.rec_main + i, 0, 0};
+            emit_clipped_triangle(a, em, ATTR_TRI, (uint64_t)i, ident, o0, o1, o2);
+            nr = em.n;
+            nt = em.ntiles;
+        }
+        a.nrast[i] = nr;
+        a.ntl[i] = nt;
+        a.sstart[i] = 0;
+    }
+    uint64_t v = nt;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t s = 0;
+        for (int w = 0; w < VS_THREADS / 64; w++) s += wsum[w];
+        a.blocksum[blockIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_vs_deferred(DrawArgs a) {
+    const uint32_t n = a.ctr->ndeferred;
+    for (uint32_t q = blockIdx.x * 64 + threadIdx.x; q < n; q += gridDim.x * 64) {
+        int64_t i = a.deferred[q];
+        V3 P0 = load_pos(a.in, 3 * i), P1 = load_pos(a.in, 3 * i + 1), P2 = load_pos(a.in, 3 * i + 2);
+        V4 o0 = mul_position_w(a.sh.M, P0);
+        V4 o1 = mul_position_w(a.sh.M, P1);
+        V4 o2 = mul_position_w(a.sh.M, P2);
+        uint32_t nr = 0, nt = 0, ss = 0;
+        vs_triangle_clip(a, i, o0, o1, o2, nr, nt, ss);
+        a.nrast[i] = nr;
+        a.ntl[i] = nt;
+        a.sstart[i] = ss;
+        if (nt) atomicAdd((unsigned long long *)&a.blocksum[i / VS_THREADS], (unsigned long long)nt);
+    }
 }
 
 // One input line (DrawLine, context.go:351-368).
@@ -413,8 +473,20 @@ __global__ __launch_bounds__(VS_THREADS) void k_vs(DrawArgs a) {
 }
 
 hipError_t launch_vs(const DrawArgs &a, hipStream_t s) {
-    if (a.kind == PRIM_TRI) hipLaunchKernelGGL(k_vs<PRIM_TRI>, dim3(a.nvsblocks), dim3(VS_THREADS), 0, s, a);
-    else hipLaunchKernelGGL(k_vs<PRIM_LINE>, dim3(a.nvsblocks), dim3(VS_THREADS), 0, s, a);
+    if (a.kind == PRIM_TRI && !a.st.wireframe)
+        hipLaunchKernelGGL(k_vs_fast, dim3(a.nvsblocks), dim3(VS_THREADS), 0, s, a);
+    else if (a.kind == PRIM_TRI)
+        hipLaunchKernelGGL(k_vs<PRIM_TRI>, dim3(a.nvsblocks), dim3(VS_THREADS), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_vs<PRIM_LINE>, dim3(a.nvsblocks), dim3(VS_THREADS), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_vs_deferred(const DrawArgs &a, hipStream_t s) {
+    int64_t blocks = (a.in.n + 63) / 64;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_vs_deferred, dim3((unsigned)blocks), dim3(64), 0, s, a);
     return hipGetLastError();
 }
 

@@ -21,6 +21,13 @@
  * either never end or wrap int64; no finite, clipped input reaches it. */
 #define ORC_BBOX_LIM ((int64_t)1 << 26)
 
+/* Planted mistakes for the mutation tests (tests/test_oracle_mutants.py
+ * builds the oracle with -DORC_MUTANT=<n> and expects the pinning tests to
+ * fail).  The default build defines none of them. */
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+
 /* ------------------------------------------------------------------ */
 /* L0 math types (vector.go:8-10, 204-206; color.go:17-19; matrix.go:5-10) */
 
@@ -147,7 +154,7 @@ static double go_pow(double x, double y) {
         xe <<= 1;
         if (x1 < .5) {
             x1 += x1;
-            xe--;
+            if (ORC_MUTANT != 1) xe--;
         }
     }
     if (y < 0) {
@@ -182,7 +189,10 @@ static Vec vmin(Vec a, Vec b) { return (Vec){go_min(a.X, b.X), go_min(a.Y, b.Y),
 static Vec vmax(Vec a, Vec b) { return (Vec){go_max(a.X, b.X), go_max(a.Y, b.Y), go_max(a.Z, b.Z)}; } /* 144-146 */
 static Vec vfloor(Vec a) { return (Vec){floor(a.X), floor(a.Y), floor(a.Z)}; }    /* 148-150 */
 static Vec vceil(Vec a) { return (Vec){ceil(a.X), ceil(a.Y), ceil(a.Z)}; }        /* 152-154 */
-static Vec vreflect(Vec i, Vec n) { return vsub(i, vmuls(n, 2 * vdot(n, i))); }   /* 175-177 */
+static Vec vreflect(Vec i, Vec n) { /* 175-177 */
+    if (ORC_MUTANT == 2) return vadd(i, vmuls(n, 2 * vdot(n, i)));
+    return vsub(i, vmuls(n, 2 * vdot(n, i)));
+}
 static Vec vperpendicular(Vec a) {                                                 /* 179-187 */
     if (a.X == 0 && a.Y == 0) {
         if (a.Z == 0) return (Vec){0, 0, 0};
@@ -220,6 +230,12 @@ typedef struct { uint8_t R, G, B, A; } NRGBA8;
 static NRGBA8 cnrgba(Col c) {
     const double d = 0xff;
     double r = clampd(c.R, 0, 1), g = clampd(c.G, 0, 1), b = clampd(c.B, 0, 1), a = clampd(c.A, 0, 1);
+    if (ORC_MUTANT == 3) { /* rounds instead of truncating */
+        r = r + 0.5 / d;
+        g = g + 0.5 / d;
+        b = b + 0.5 / d;
+        a = a + 0.5 / d;
+    }
     NRGBA8 o = {go_u8(r * d), go_u8(g * d), go_u8(b * d), go_u8(a * d)};
     return o;
 }
@@ -278,7 +294,7 @@ static Mat mat_perspective(double fovy, double aspect, double near, double far) 
 static Mat mat_look_at(Vec eye, Vec center, Vec up) { /* 95-105 */
     Vec z = vnormalize(vsub(eye, center));
     Vec x = vnormalize(vcross(up, z));
-    Vec y = vcross(z, x);
+    Vec y = ORC_MUTANT == 4 ? vcross(x, z) : vcross(z, x);
     Mat o = {{x.X, x.Y, x.Z, -vdot(x, eye), y.X, y.Y, y.Z, -vdot(y, eye), z.X, z.Y, z.Z, -vdot(z, eye), 0, 0, 0, 1}};
     return o;
 }
@@ -329,7 +345,7 @@ static VecW barycentric(Vec p1, Vec p2, Vec p3, Vec p) { /* vertex.go:81-95 */
     double d = d00 * d11 - d01 * d01;
     double v = (d11 * d20 - d01 * d21) / d;
     double w = (d00 * d21 - d01 * d20) / d;
-    double u = 1 - v - w;
+    double u = ORC_MUTANT == 5 ? 1 - v + w : 1 - v - w;
     return (VecW){u, v, w, 1};
 }
 static void fix_normals(Triangle *t) { /* triangle.go:33-37, 46-58 */
@@ -348,7 +364,10 @@ static void fix_normals(Triangle *t) { /* triangle.go:33-37, 46-58 */
 static const VecW clip_P[6] = {{1, 0, 0, 1}, {-1, 0, 0, 1}, {0, 1, 0, 1}, {0, -1, 0, 1}, {0, 0, 1, 1}, {0, 0, -1, 1}};
 static const VecW clip_N[6] = {{-1, 0, 0, 1}, {1, 0, 0, 1}, {0, -1, 0, 1}, {0, 1, 0, 1}, {0, 0, -1, 1}, {0, 0, 1, 1}};
 
-static int point_in_front(int p, VecW v) { return wdot(wsub(v, clip_P[p]), clip_N[p]) > 0; } /* 16-18 */
+static int point_in_front(int p, VecW v) { /* 16-18 */
+    if (ORC_MUTANT == 11 && p == 4) return wdot(wsub(v, clip_P[p]), clip_P[p]) > 0;
+    return wdot(wsub(v, clip_P[p]), clip_N[p]) > 0;
+}
 static VecW intersect_segment(int p, VecW v0, VecW v1) {                                   /* 20-26 */
     VecW u = wsub(v1, v0);
     VecW w = wsub(v0, clip_P[p]);
@@ -465,7 +484,7 @@ static Col bilinear_sample(const orc_texture *t, double u, double v) { /* textur
     x -= (double)x0;
     y -= (double)y0;
     Col c00 = tex_at(t, x0, y0);
-    Col c01 = tex_at(t, x0, y1);
+    Col c01 = ORC_MUTANT == 6 ? tex_at(t, x1, y0) : tex_at(t, x0, y1);
     Col c10 = tex_at(t, x1, y0);
     Col c11 = tex_at(t, x1, y1);
     Col c = {0, 0, 0, 0};
@@ -565,7 +584,10 @@ static void store_depth(orc_ctx *dc, int64_t i, double d) {
 }
 
 /* context.go:147-149 */
-static double edge(Vec a, Vec b, Vec c) { return (b.X - c.X) * (a.Y - c.Y) - (b.Y - c.Y) * (a.X - c.X); }
+static double edge(Vec a, Vec b, Vec c) {
+    if (ORC_MUTANT == 7) return (b.X - c.X) * (a.Y - c.Y) - (b.Y - c.Y) * (a.X - b.X);
+    return (b.X - c.X) * (a.Y - c.Y) - (b.Y - c.Y) * (a.X - c.X);
+}
 
 /* context.go:151-281 */
 static Info rasterize(Worker *wk, const Vertex *v0, const Vertex *v1, const Vertex *v2, Vec s0, Vec s1, Vec s2) {
@@ -660,14 +682,14 @@ static Info rasterize(Worker *wk, const Vertex *v0, const Vertex *v1, const Vert
             /* update buffers atomically */
             pthread_mutex_t *lock = &dc->locks[(x + y) & 255];
             if (dc->threaded) pthread_mutex_lock(lock);
-            if (bz <= load_depth(dc, i) || !dc->ReadDepth) {
+            if ((ORC_MUTANT == 8 ? bz < load_depth(dc, i) : bz <= load_depth(dc, i)) || !dc->ReadDepth) {
                 info.updated++;
                 if (dc->WriteDepth) store_depth(dc, i, z);
                 if (dc->WriteColor) {
                     if (dc->AlphaBlend && color.A < 1) {
                         uint32_t sr, sg, sb, sa;
                         nrgba_rgba(cnrgba(color), &sr, &sg, &sb, &sa);
-                        uint32_t a = (0xffff - sa) * 0x101;
+                        uint32_t a = ORC_MUTANT == 9 ? (0xffff - sa) : (0xffff - sa) * 0x101;
                         int64_t j = (y * W + x) * 4; /* PixOffset(x, y) */
                         uint8_t *dr = &dc->Pix[j + 0];
                         uint8_t *dg = &dc->Pix[j + 1];
@@ -741,7 +763,7 @@ static Info draw_clipped_triangle(Worker *wk, const Vertex *v0in, const Vertex *
     Vec ndc2 = wvec(wdivs(v2->Output, v2->Output.W));
     /* back face culling */
     double a = (ndc1.X - ndc0.X) * (ndc2.Y - ndc0.Y) - (ndc2.X - ndc0.X) * (ndc1.Y - ndc0.Y);
-    if (a < 0) {
+    if (a < 0 && ORC_MUTANT != 10) {
         const Vertex *tv = v0; v0 = v2; v2 = tv;
         Vec tn = ndc0; ndc0 = ndc2; ndc2 = tn;
     }
@@ -1037,7 +1059,7 @@ void orc_nrgba_rgba(const uint8_t c[4], uint32_t out[4]) {
 void orc_blend(uint8_t dst[4], const double color[4]) {
     uint32_t sr, sg, sb, sa;
     nrgba_rgba(cnrgba((Col){color[0], color[1], color[2], color[3]}), &sr, &sg, &sb, &sa);
-    uint32_t a = (0xffff - sa) * 0x101;
+    uint32_t a = ORC_MUTANT == 9 ? (0xffff - sa) : (0xffff - sa) * 0x101;
     dst[0] = (uint8_t)(((uint32_t)dst[0] * a / 0xffff + sr) >> 8);
     dst[1] = (uint8_t)(((uint32_t)dst[1] * a / 0xffff + sg) >> 8);
     dst[2] = (uint8_t)(((uint32_t)dst[2] * a / 0xffff + sb) >> 8);

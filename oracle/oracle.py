@@ -11,7 +11,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "liboracle.so")
+# ORACLE_LIB points the binding at another build of the oracle (the mutation
+# tests use it to load builds with planted mistakes).
+LIB = os.environ.get("ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 SHADER_SOLID, SHADER_TEXTURE, SHADER_PHONG = 1, 2, 3
 TEX_NRGBA8, TEX_RGBA8, TEX_RGBA16 = 1, 2, 3
