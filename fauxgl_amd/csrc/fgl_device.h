@@ -41,7 +41,7 @@ struct PrimRec {
     double r[3];                 // 1 / v_k.Output.W (context.go:176-178)
     uint64_t attr;               // kind:2 | sel:6 (3 x 2-bit vertex selectors) | index:56
     int32_t x0, y0, x1, y1;      // int(floor(min)), int(ceil(max)) (context.go:155-160)
-    uint64_t pad;
+    double ra;                   // 1 / edge(s0, s1, s2) (context.go:175)
 };
 static_assert(sizeof(PrimRec) == 128, "PrimRec layout");
 
@@ -91,11 +91,40 @@ struct DrawCounters {
     uint32_t side_vtx;           // side vertices allocated
     uint32_t ndeferred;          // input triangles deferred to the clipping kernel
     uint32_t npairs;             // (tile, triangle) pairs emitted
-    uint32_t overflow;           // bit 0 side records, 1 side vertices, 2 pairs, 3 fragment buffer
+    uint32_t overflow;           // bit 0 side records, 1 side vertices, 2 pairs, 4 spans
     uint32_t pairs_lo, pairs_hi; // unclamped pair count
-    uint32_t pad;
+    uint32_t nspans;             // coverage spans (clamped to capacity)
+    uint32_t spans_lo, spans_hi; // unclamped span count
+    uint32_t nfrags;             // fragments (clamped to capacity)
+    uint32_t frags_lo, frags_hi; // unclamped fragment count
+    uint32_t pad2;
     unsigned long long phase[8];  // tile-kernel phase cycles (instrumented builds only)
 };
+
+// Coverage span: the covered pixels of one row of one raster triangle in
+// one tile, with the edge values the reference holds at the first of them.
+struct Span {
+    double w[3];   // w0, w1, w2 at the first covered pixel (context.go:208-213)
+    uint32_t ref;  // raster triangle
+    uint16_t pix;  // tile-local index of the first covered pixel
+    uint16_t n;    // covered pixels | SPAN_INB
+};
+static_assert(sizeof(Span) == 32, "Span layout");
+constexpr uint16_t SPAN_INB = 0x8000;  // row y and x in bounds: SetNRGBA writes (context.go:269)
+constexpr uint16_t SPAN_N = 0x7fff;
+
+// A shaded fragment, in submission order: what the framebuffer update of
+// context.go:230-273 needs at one pixel.
+struct Frag {
+    double z;        // b0*s0.Z + b1*s1.Z + b2*s2.Z (context.go:230)
+    uint32_t color;  // Color.NRGBA() of the shader's color, bytes R,G,B,A
+    uint16_t pix;    // tile-local pixel index
+    uint16_t flags;  // FRAG_*
+};
+static_assert(sizeof(Frag) == 16, "Frag layout");
+constexpr uint16_t FRAG_INB = 1;      // SetNRGBA writes (x, y in bounds)
+constexpr uint16_t FRAG_BLEND = 2;    // color.A < 1 (blended when AlphaBlend)
+constexpr uint16_t FRAG_DISCARD = 4;  // color == Discard (context.go:241)
 
 // ---------------------------------------------------------------------------
 // Go semantics

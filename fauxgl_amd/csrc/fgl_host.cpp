@@ -86,6 +86,9 @@ struct fgl_ctx {
 
     DevBuf ctr, ntl, nrast, sstart, blocksum, blockoff, rec_main, side_rec, side_vtx;
     DevBuf keys0, vals0, keys1, vals1, hist, dtot, tstart, tend, deferred;
+    DevBuf pspans, span_bsum, span_boff, poff, spans;
+    DevBuf frag_bsum, frag_boff, foff, frags;
+    uint32_t span_cap = 0, frag_cap = 0;
     DevBuf in_pos, in_nrm, in_uv, in_col;
     DevBuf rs_tmp, rs_wx, rs_ox, rs_wy, rs_oy;
     DevBuf dpart, dimg;
@@ -325,6 +328,10 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
     if (c->side_rec_cap < want_side) c->side_rec_cap = (uint32_t)std::min<uint64_t>(want_side, 0x7fffffffu);
     if (c->side_vtx_cap < want_vtx) c->side_vtx_cap = (uint32_t)std::min<uint64_t>(want_vtx, 0x7fffffffu);
     if (c->pair_cap < want_pairs) c->pair_cap = (uint32_t)std::min<uint64_t>(want_pairs, 0xfffff000u);
+    uint64_t want_spans = (uint64_t)3 * c->pair_cap;
+    if (c->span_cap < want_spans) c->span_cap = (uint32_t)std::min<uint64_t>(want_spans, 0xfffff000u);
+    uint64_t want_frags = (uint64_t)2 * c->span_cap;
+    if (c->frag_cap < want_frags) c->frag_cap = (uint32_t)std::min<uint64_t>(want_frags, 0xfffff000u);
 
     HIPCHK(c, c->ctr.ensure(sizeof(DrawCounters)));
     HIPCHK(c, c->ntl.ensure((size_t)n * 4));
@@ -347,6 +354,28 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
         HIPCHK(c, c->vals1.ensure((size_t)c->pair_cap * 4));
         a.nrsblocks = (int32_t)(((uint64_t)c->pair_cap + RS_BLOCK - 1) / RS_BLOCK);
         HIPCHK(c, c->hist.ensure((size_t)256 * a.nrsblocks * 4));
+        const size_t nsb = ((size_t)c->pair_cap + SPAN_THREADS - 1) / SPAN_THREADS + 1;
+        HIPCHK(c, c->pspans.ensure((size_t)c->pair_cap * 4));
+        HIPCHK(c, c->span_bsum.ensure(nsb * 8));
+        HIPCHK(c, c->span_boff.ensure(nsb * 8));
+        HIPCHK(c, c->poff.ensure(((size_t)c->pair_cap + 1) * 4));
+        HIPCHK(c, c->spans.ensure((size_t)c->span_cap * sizeof(Span)));
+        a.pspans = c->pspans.as<uint32_t>();
+        a.span_blocksum = c->span_bsum.as<uint64_t>();
+        a.span_blockoff = c->span_boff.as<uint64_t>();
+        a.poff = c->poff.as<uint32_t>();
+        a.spans = c->spans.as<Span>();
+        a.span_cap = c->span_cap;
+        const size_t nfb = ((size_t)c->span_cap + SPAN_THREADS - 1) / SPAN_THREADS + 1;
+        HIPCHK(c, c->frag_bsum.ensure(nfb * 8));
+        HIPCHK(c, c->frag_boff.ensure(nfb * 8));
+        HIPCHK(c, c->foff.ensure(((size_t)c->span_cap + 1) * 4));
+        HIPCHK(c, c->frags.ensure((size_t)c->frag_cap * sizeof(Frag)));
+        a.frag_blocksum = c->frag_bsum.as<uint64_t>();
+        a.frag_blockoff = c->frag_boff.as<uint64_t>();
+        a.foff = c->foff.as<uint32_t>();
+        a.frags = c->frags.as<Frag>();
+        a.frag_cap = c->frag_cap;
 
         a.rec_main = c->rec_main.as<PrimRec>();
         a.side_rec = c->side_rec.as<PrimRec>();
@@ -386,7 +415,13 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
             if ((r = launch(c, "radix_scatter", [&] { return launch_radix_scatter(a, p, c->stream); }))) return r;
         }
         if ((r = launch(c, "ranges", [&] { return launch_ranges(a, c->stream); }))) return r;
-        if ((r = launch(c, "tile", [&] { return launch_tile(a, c->stream); }))) return r;
+        if ((r = launch(c, "span_count", [&] { return launch_span_count(a, c->stream); }))) return r;
+        if ((r = launch(c, "span_scan", [&] { return launch_span_scan(a, c->stream); }))) return r;
+        if ((r = launch(c, "spans", [&] { return launch_spans(a, c->stream); }))) return r;
+        if ((r = launch(c, "frag_count", [&] { return launch_frag_count(a, c->stream); }))) return r;
+        if ((r = launch(c, "frag_scan", [&] { return launch_frag_scan(a, c->stream); }))) return r;
+        if ((r = launch(c, "shade", [&] { return launch_shade(a, c->stream); }))) return r;
+        if ((r = launch(c, "rop", [&] { return launch_tile(a, c->stream); }))) return r;
 
         DrawCounters h;
         HIPCHK(c, hipMemcpyAsync(&h, a.ctr, sizeof h, hipMemcpyDeviceToHost, c->stream));
@@ -402,9 +437,21 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
             c->pend_depth = false;
             return FGL_OK;
         }
-        if (h.overflow & 8u) {
-            set_err(c->err, "internal error: tile fragment buffer overflow");
-            return FGL_E_DEVICE;
+        if (h.overflow & 16u) {
+            uint64_t need = ((uint64_t)h.spans_hi << 32 | h.spans_lo) * 5 / 4 + 65536;
+            if (need > 0xfffff000ull) {
+                set_err(c->err, "draw needs %llu coverage spans, more than 2^32", (unsigned long long)need);
+                return FGL_E_UNSUPPORTED;
+            }
+            c->span_cap = (uint32_t)need;
+        }
+        if (h.overflow & 32u) {
+            uint64_t need = ((uint64_t)h.frags_hi << 32 | h.frags_lo) * 5 / 4 + 65536;
+            if (need > 0xfffff000ull) {
+                set_err(c->err, "draw needs %llu fragments, more than 2^32", (unsigned long long)need);
+                return FGL_E_UNSUPPORTED;
+            }
+            c->frag_cap = (uint32_t)need;
         }
         if (h.overflow & 1u) c->side_rec_cap = (uint32_t)std::min<uint64_t>((uint64_t)h.side_rec * 5 / 4 + 4096, 0x7fffffffu);
         if (h.overflow & 2u) c->side_vtx_cap = (uint32_t)std::min<uint64_t>((uint64_t)h.side_vtx * 5 / 4 + 4096, 0x7fffffffu);

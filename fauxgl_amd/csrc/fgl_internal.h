@@ -52,6 +52,21 @@ struct DrawArgs {
     uint32_t *tstart, *tend;  // per tile range in the sorted pairs
     int32_t sorted_buf;    // which of keys/vals holds the sorted result
 
+    // coverage spans
+    uint32_t *pspans;           // spans per sorted pair
+    uint64_t *span_blocksum;    // per SPAN_THREADS pairs
+    uint64_t *span_blockoff;
+    uint32_t *poff;             // first span of each sorted pair (pair_cap + 1 entries)
+    Span *spans;
+    uint32_t span_cap;
+
+    // shaded fragments
+    uint64_t *frag_blocksum;    // per SPAN_THREADS spans
+    uint64_t *frag_blockoff;
+    uint32_t *foff;             // first fragment of each span (span_cap + 1 entries)
+    Frag *frags;
+    uint32_t frag_cap;
+
     // framebuffer and deferred clears
     uint32_t *color;       // NRGBA8 as little-endian u32
     double *depth;
@@ -63,6 +78,7 @@ struct DrawArgs {
 };
 
 constexpr int VS_THREADS = 256;
+constexpr int SPAN_THREADS = 256;
 constexpr int RS_THREADS = 256;
 constexpr int RS_ITEMS = 16;
 constexpr int RS_BLOCK = RS_THREADS * RS_ITEMS;
@@ -78,6 +94,12 @@ hipError_t launch_radix_hist(const DrawArgs &a, int pass, hipStream_t s);
 hipError_t launch_radix_scan(const DrawArgs &a, hipStream_t s);
 hipError_t launch_radix_scatter(const DrawArgs &a, int pass, hipStream_t s);
 hipError_t launch_ranges(const DrawArgs &a, hipStream_t s);
+hipError_t launch_span_count(const DrawArgs &a, hipStream_t s);
+hipError_t launch_span_scan(const DrawArgs &a, hipStream_t s);
+hipError_t launch_spans(const DrawArgs &a, hipStream_t s);
+hipError_t launch_frag_count(const DrawArgs &a, hipStream_t s);
+hipError_t launch_frag_scan(const DrawArgs &a, hipStream_t s);
+hipError_t launch_shade(const DrawArgs &a, hipStream_t s);
 hipError_t launch_tile(const DrawArgs &a, hipStream_t s);
 
 hipError_t launch_fill_color(uint32_t *color, int64_t n, uint32_t v, hipStream_t s);

@@ -1,87 +1,39 @@
-// fgl_tile.hip -- the per-tile raster kernel: one workgroup owns one
-// TILE_W x TILE_H tile of the color and depth buffers in LDS and draws the
-// tile's raster triangles in submission order.  Restates rasterize()
-// (context.go:151-281), InterpolateVertexes (vertex.go:18-79), the built-in
-// fragment shaders (shader.go:25-96), BilinearSample (texture.go:41-63) and
-// the framebuffer write (context.go:240-273).
+// fgl_tile.hip -- shading and the ordered framebuffer update.
 //
-// Triangles are consumed in batches.  For each batch:
-//   A. coverage: one lane per (triangle, row) item replays the reference's
-//      forward-differenced edge recurrence for that row (the row recurrence
-//      w00 += b12 from y0, the skip d, then w += a per pixel), so every
-//      barycentric is bit-identical to the serial loop; covered pixels are
-//      appended to an LDS fragment buffer;
-//   B. shading: one lane per fragment computes z, the perspective-correct
-//      interpolants and the shader color;
-//   C. ordered write: fragments are applied in rounds; in each round the
-//      lowest (triangle, row) key per pixel wins, so writes to a pixel
-//      happen in submission order exactly as in the reference.
-// The early depth reject in B is used only when it is provably equal to the
-// reference's test at write time (depth non-increasing: ReadDepth and
-// DepthBias >= 0, or WriteDepth off).
+// k_shade: one thread per coverage span (fgl_spans.hip).  It fetches the
+// raster triangle's record and vertex attributes once, then for each
+// covered pixel of the span replays the reference's per-pixel recurrence
+// from the span's first edge values (context.go:208-213), computes z
+// (context.go:230), the perspective-correct interpolants
+// (context.go:236-238, vertex.go:18-79) and the built-in fragment shader
+// (shader.go:25-96, texture.go:41-63), and writes one Frag in submission
+// order.  Shading has no side effects, so shading a fragment that the
+// ordered update later rejects changes nothing observable.
+//
+// k_rop: one workgroup per TILE_W x TILE_H tile keeps the tile's color and
+// depth in LDS and applies the tile's fragments in submission order
+// (context.go:232-273): in each round every pixel's lowest pending fragment
+// wins and does the early depth test, the Discard check, the locked depth
+// test and the depth / color write exactly as the serial loop would at
+// that moment.
 #include "fgl_internal.h"
 #include "fgl_tiles.h"
 
 namespace fgl {
 
-#ifndef FGL_FRAG_CAP
-#define FGL_FRAG_CAP 1024
+#ifndef FGL_ROP_CHUNK
+#define FGL_ROP_CHUNK 2048
 #endif
-#ifndef FGL_MAX_ITEMS
-#define FGL_MAX_ITEMS 512
-#endif
-constexpr int FRAG_CAP = FGL_FRAG_CAP;
-constexpr int MAX_ITEMS = FGL_MAX_ITEMS;
-
-// Phase timers for profiling builds (-DFGL_TILE_PHASE_TIMERS): thread 0 of
-// each workgroup accumulates shader-clock cycles per phase into the draw
-// counters.  They add no work to the default build.
-#ifdef FGL_TILE_PHASE_TIMERS
-#define PT_DECL uint64_t pt_last = 0, pt_acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-#define PT_START if (tid == 0) pt_last = clock64();
-#define PT_MARK(k)                           \
-    if (tid == 0) {                          \
-        uint64_t now_ = clock64();           \
-        pt_acc[k] += now_ - pt_last;         \
-        pt_last = now_;                      \
-    }
-#define PT_FLUSH \
-    if (tid == 0) for (int k_ = 0; k_ < 8; k_++) atomicAdd(&a.ctr->phase[k_], (unsigned long long)pt_acc[k_]);
-#else
-#define PT_DECL
-#define PT_START
-#define PT_MARK(k)
-#define PT_FLUSH
-#endif
-constexpr int BATCH = TILE_THREADS;
+constexpr int ROP_CHUNK = FGL_ROP_CHUNK;                 // fragments per round set
+constexpr int ROP_PER_THREAD = ROP_CHUNK / TILE_THREADS;
 constexpr uint32_t OWNER_NONE = 0xffffffffu;
-
-constexpr uint16_t F_PIX = 0x0fff;
-constexpr uint16_t F_INB = 0x1000;    // x, y in bounds (SetNRGBA writes)
-constexpr uint16_t F_BLEND = 0x2000;  // AlphaBlend && color.A < 1
-constexpr uint16_t F_DEAD = 0x4000;   // discarded or rejected early
-constexpr uint16_t F_DONE = 0x8000;   // applied
-
-struct TileLDS {
-    uint32_t col[TILE_PIX];
-    double dep[TILE_PIX];
-    uint32_t owner[TILE_PIX];
-    double fb0[FRAG_CAP], fb1[FRAG_CAP], fb2[FRAG_CAP];
-    uint32_t fkey[FRAG_CAP];
-    uint32_t fcol[FRAG_CAP];
-    uint16_t fpix[FRAG_CAP];
-    uint32_t pref[BATCH];
-    uint32_t istart[BATCH + 1];
-    uint8_t iprim[MAX_ITEMS];
-    uint32_t wsum[2][TILE_THREADS / 64];
-    uint32_t nfrag;
-    uint32_t nitems;
-    uint32_t next_lp, next_io;
-    unsigned long long total, updated;
-};
 
 // ---------------------------------------------------------------------------
 // math.Pow (see fgl_device.h)
+
+// x**yf for the fractional part of a non-integer exponent; kept out of line
+// (the library exp/log are large and only non-integer powers need them).
+__device__ __noinline__ double pow_frac(double x, double yf) { return exp(yf * log(x)); }
 
 __device__ double go_pow(double x, double y) {
     if (y == 0 || x == 1) return 1;
@@ -128,7 +80,7 @@ __device__ double go_pow(double x, double y) {
             yf = yf - 1;
             yi = yi + 1;
         }
-        a1 = exp(yf * log(x));
+        a1 = pow_frac(x, yf);
     }
     int xe_i;
     double x1 = frexp(x, &xe_i);
@@ -268,36 +220,52 @@ FGL_D double interp(double a, double b, double c, V4 k) {
     return n * k.w;
 }
 
-// Fragment shaders (shader.go).  Returns the shader's Color.
+// Vertex attributes of one raster triangle, fetched once per span.
+struct ShadeIn {
+    V3 p[3], n[3];        // PhongShader: Position, Normal
+    double u[3], v[3];    // Texture.X, Texture.Y
+    C4 c[3];              // Color (PhongShader with ObjectColor == Discard)
+};
+
+// SH: 1 SolidColorShader, 2 TextureShader, 3 PhongShader, 4 PhongShader
+// with a Texture (its own instantiation keeps the sampler's registers out
+// of the untextured Phong kernel).
 template <int SH>
-FGL_D C4 shade(const DevShader &sh, const AttrSrc &src, uint64_t attr, V4 b) {
+FGL_D void load_in(const DevShader &sh, const AttrSrc &src, uint64_t attr, ShadeIn &in) {
+    const bool uvs = SH == 2 || SH == 4;
+    const bool cols = SH == 3 && sh.obj_is_discard;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        if (SH >= 3) {
+            in.p[k] = fetch_pos(src, attr, k);
+            in.n[k] = fetch_nrm(src, attr, k);
+        }
+        if (uvs) fetch_uv(src, attr, k, in.u[k], in.v[k]);
+        if (cols) in.c[k] = fetch_col(src, attr, k);
+    }
+}
+
+// The fragment shaders (shader.go) on interpolated attributes.
+template <int SH>
+FGL_D C4 eval(const DevShader &sh, const ShadeIn &in, V4 b) {
     if (SH == 1) {  // SolidColorShader.Fragment (shader.go:25-27)
         return c4(sh.color[0], sh.color[1], sh.color[2], sh.color[3]);
     } else if (SH == 2) {  // TextureShader.Fragment (shader.go:44-46)
-        double u0, v0, u1, v1, u2, v2;
-        fetch_uv(src, attr, 0, u0, v0);
-        fetch_uv(src, attr, 1, u1, v1);
-        fetch_uv(src, attr, 2, u2, v2);
-        return bilinear_sample(sh, interp(u0, u1, u2, b), interp(v0, v1, v2, b));
+        return bilinear_sample(sh, interp(in.u[0], in.u[1], in.u[2], b), interp(in.v[0], in.v[1], in.v[2], b));
     } else {  // PhongShader.Fragment (shader.go:75-96)
-        V3 p0 = fetch_pos(src, attr, 0), p1 = fetch_pos(src, attr, 1), p2 = fetch_pos(src, attr, 2);
-        V3 n0 = fetch_nrm(src, attr, 0), n1 = fetch_nrm(src, attr, 1), n2 = fetch_nrm(src, attr, 2);
-        V3 P = v3(interp(p0.x, p1.x, p2.x, b), interp(p0.y, p1.y, p2.y, b), interp(p0.z, p1.z, p2.z, b));
-        V3 N = vnormalize(v3(interp(n0.x, n1.x, n2.x, b), interp(n0.y, n1.y, n2.y, b), interp(n0.z, n1.z, n2.z, b)));
+        V3 P = v3(interp(in.p[0].x, in.p[1].x, in.p[2].x, b), interp(in.p[0].y, in.p[1].y, in.p[2].y, b),
+                  interp(in.p[0].z, in.p[1].z, in.p[2].z, b));
+        V3 N = vnormalize(v3(interp(in.n[0].x, in.n[1].x, in.n[2].x, b), interp(in.n[0].y, in.n[1].y, in.n[2].y, b),
+                             interp(in.n[0].z, in.n[1].z, in.n[2].z, b)));
         C4 light = c4(sh.amb[0], sh.amb[1], sh.amb[2], sh.amb[3]);
         C4 color;
-        if (sh.has_tex) {
-            double u0, v0, u1, v1, u2, v2;
-            fetch_uv(src, attr, 0, u0, v0);
-            fetch_uv(src, attr, 1, u1, v1);
-            fetch_uv(src, attr, 2, u2, v2);
-            color = bilinear_sample(sh, interp(u0, u1, u2, b), interp(v0, v1, v2, b));
+        if (SH == 4) {
+            color = bilinear_sample(sh, interp(in.u[0], in.u[1], in.u[2], b), interp(in.v[0], in.v[1], in.v[2], b));
         } else if (!sh.obj_is_discard) {
             color = c4(sh.obj[0], sh.obj[1], sh.obj[2], sh.obj[3]);
         } else {
-            C4 c0 = fetch_col(src, attr, 0), c1 = fetch_col(src, attr, 1), c2 = fetch_col(src, attr, 2);
-            color = c4(interp(c0.r, c1.r, c2.r, b), interp(c0.g, c1.g, c2.g, b), interp(c0.b, c1.b, c2.b, b),
-                       interp(c0.a, c1.a, c2.a, b));
+            color = c4(interp(in.c[0].r, in.c[1].r, in.c[2].r, b), interp(in.c[0].g, in.c[1].g, in.c[2].g, b),
+                       interp(in.c[0].b, in.c[1].b, in.c[2].b, b), interp(in.c[0].a, in.c[1].a, in.c[2].a, b));
         }
         V3 L = v3(sh.light[0], sh.light[1], sh.light[2]);
         double diffuse = go_max(vdot(N, L), 0);
@@ -317,54 +285,148 @@ FGL_D C4 shade(const DevShader &sh, const AttrSrc &src, uint64_t attr, V4 b) {
     }
 }
 
-// ---------------------------------------------------------------------------
-
 FGL_D const PrimRec *prim_of(const DrawArgs &a, uint32_t ref) {
     return (ref & REF_SIDE) ? a.side_rec + (ref & ~REF_SIDE) : a.rec_main + ref;
 }
 
-// edge(a, b, c) (context.go:147-149)
-FGL_D double edge(double ax, double ay, double bx, double by, double cx, double cy) {
-    return (bx - cx) * (ay - cy) - (by - cy) * (ax - cx);
-}
+// ---------------------------------------------------------------------------
+// fragment offsets: exclusive scan of span lengths
 
-// Appends a fragment to the tile's buffer (one LDS atomic per wave).
-FGL_D void emit_frag(TileLDS &s, bool pred, uint16_t pix, uint32_t key, double b0, double b1, double b2) {
-    uint64_t m = __ballot(pred);
-    if (pred) {
-        uint32_t lane = __lane_id();
-        uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-        int leader = __ffsll((long long)m) - 1;
-        uint32_t base = 0;
-        if ((int)lane == leader) base = atomicAdd(&s.nfrag, (uint32_t)__popcll(m));
-        base = __shfl(base, leader);
-        uint32_t slot = base + rank;
-        if (slot < FRAG_CAP) {
-            s.fpix[slot] = pix;
-            s.fkey[slot] = key;
-            s.fb0[slot] = b0;
-            s.fb1[slot] = b1;
-            s.fb2[slot] = b2;
-        }
+__global__ __launch_bounds__(SPAN_THREADS) void k_frag_count(DrawArgs a) {
+    __shared__ uint64_t wsum[SPAN_THREADS / 64];
+    if (a.ctr->overflow) return;
+    const uint32_t NS = a.ctr->nspans;
+    const uint32_t i = blockIdx.x * SPAN_THREADS + threadIdx.x;
+    uint64_t v = i < NS ? (a.spans[i].n & SPAN_N) : 0u;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t s = 0;
+        for (int w = 0; w < SPAN_THREADS / 64; w++) s += wsum[w];
+        a.frag_blocksum[blockIdx.x] = s;
     }
 }
 
-template <int SH>
-__global__ __launch_bounds__(TILE_THREADS) void k_tile(DrawArgs a) {
-    __shared__ TileLDS s;
+__global__ __launch_bounds__(1024) void k_frag_scan(DrawArgs a) {
+    __shared__ uint64_t part[1024];
+    if (a.ctr->overflow) return;
+    const uint32_t NS = a.ctr->nspans;
+    const int nb = (int)((NS + SPAN_THREADS - 1) / SPAN_THREADS);
     const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
+    const int per = (nb + 1023) / 1024;
+    const int lo = tid * per;
+    const int hi = lo + per < nb ? lo + per : nb;
+    uint64_t s = 0;
+    for (int k = lo; k < hi; k++) s += a.frag_blocksum[k];
+    part[tid] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint64_t v = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    uint64_t run = part[tid] - s;
+    for (int k = lo; k < hi; k++) {
+        uint64_t v = a.frag_blocksum[k];
+        a.frag_blockoff[k] = run;
+        run += v;
+    }
+    if (tid == 1023) {
+        uint64_t total = part[tid];
+        if (total > a.frag_cap) atomicOr(&a.ctr->overflow, 32u);
+        a.ctr->nfrags = total > a.frag_cap ? a.frag_cap : (uint32_t)total;
+        a.ctr->frags_lo = (uint32_t)total;
+        a.ctr->frags_hi = (uint32_t)(total >> 32);
+        a.foff[NS] = total > a.frag_cap ? a.frag_cap : (uint32_t)total;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// shading, one thread per span
+
+template <int SH>
+__global__ __launch_bounds__(SPAN_THREADS) void k_shade(DrawArgs a) {
+    __shared__ uint64_t wsum[SPAN_THREADS / 64];
+    if (a.ctr->overflow) return;
+    const uint32_t NS = a.ctr->nspans;
+    const uint32_t i = blockIdx.x * SPAN_THREADS + threadIdx.x;
+    const Span sp = i < NS ? a.spans[i] : Span{{0, 0, 0}, 0, 0, 0};
+    const uint32_t n = sp.n & SPAN_N;
+    // block-exclusive scan of n
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint64_t v = n;
+    for (int off = 1; off < 64; off <<= 1) {
+        uint64_t y = __shfl_up(v, off);
+        if (lane >= off) v += y;
+    }
+    if (lane == 63) wsum[wave] = v;
+    __syncthreads();
+    uint64_t fo = a.frag_blockoff[blockIdx.x];
+    for (int w = 0; w < wave; w++) fo += wsum[w];
+    fo += v - n;
+    if (i >= NS) return;
+    a.foff[i] = (uint32_t)fo;
+    if (n == 0) return;
+
+    const PrimRec *pr = prim_of(a, sp.ref);
+    const double a12 = pr->sy[2] - pr->sy[1];
+    const double a20 = pr->sy[0] - pr->sy[2];
+    const double a01 = pr->sy[1] - pr->sy[0];
+    const double ra = pr->ra;
+    const double sz0 = pr->sz[0], sz1 = pr->sz[1], sz2 = pr->sz[2];
+    const double r0 = pr->r[0], r1 = pr->r[1], r2 = pr->r[2];
+    AttrSrc src;
+    src.pos = a.in.pos;
+    src.nrm = a.in.nrm;
+    src.uv = a.in.uv;
+    src.col = a.in.col;
+    src.side = a.side_vtx;
+    ShadeIn in;
+    load_in<SH>(a.sh, src, pr->attr, in);
+
+    const uint16_t inb = (sp.n & SPAN_INB) ? FRAG_INB : (uint16_t)0;
+    double w0 = sp.w[0], w1 = sp.w[1], w2 = sp.w[2];
+    Frag *out = a.frags + fo;
+    for (uint32_t j = 0; j < n; j++) {
+        // barycentrics at this pixel, then the step (context.go:208-213)
+        const double b0 = w0 * ra, b1 = w1 * ra, b2 = w2 * ra;
+        w0 += a12;
+        w1 += a20;
+        w2 += a01;
+        Frag f;
+        f.z = b0 * sz0 + b1 * sz1 + b2 * sz2;
+        // perspective-correct interpolation weights (context.go:236-237)
+        V4 bw = v4(b0 * r0, b1 * r1, b2 * r2, 0);
+        bw.w = 1.0 / (bw.x + bw.y + bw.z);
+        C4 c = eval<SH>(a.sh, in, bw);
+        f.color = nrgba_pack(c);
+        f.pix = (uint16_t)(sp.pix + j);
+        f.flags = inb | (c.a < 1 ? FRAG_BLEND : (uint16_t)0) |
+                  ((c.r == 0 && c.g == 0 && c.b == 0 && c.a == 0) ? FRAG_DISCARD : (uint16_t)0);
+        out[j] = f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// the ordered framebuffer update, one workgroup per tile
+
+__global__ __launch_bounds__(TILE_THREADS) void k_rop(DrawArgs a) {
+    __shared__ uint32_t col[TILE_PIX];
+    __shared__ double dep[TILE_PIX];
+    __shared__ uint32_t owner[TILE_PIX];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
     const int t = blockIdx.x;
     if (a.ctr->overflow) return;  // the host reruns the whole draw
     const int tx = t % a.ntx, ty = t / a.ntx;
     const int cx0 = tx * TILE_W, cy0 = ty * TILE_H;
     const int cw = min(TILE_W, a.W - cx0), chh = min(TILE_H, a.H - cy0);
-    const int cx1 = cx0 + cw - 1, cy1 = cy0 + chh - 1;
-    PT_DECL
-    PT_START
-    const uint32_t ls = a.tstart[t], le = a.tend[t];
-    const bool has = le > ls;
-    if (!has) {
+    const uint32_t ps = a.tstart[t], pe = a.tend[t];
+    const uint32_t f0 = pe > ps ? a.foff[a.poff[ps]] : 0u;
+    const uint32_t f1 = pe > ps ? a.foff[a.poff[pe]] : 0u;
+    if (f1 <= f0) {
         if (a.clear_color_pending || a.clear_depth_pending) {
             for (int p = tid; p < TILE_PIX; p += TILE_THREADS) {
                 int lx = p % TILE_W, ly = p / TILE_W;
@@ -377,305 +439,104 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(DrawArgs a) {
         }
         return;
     }
-
     for (int p = tid; p < TILE_PIX; p += TILE_THREADS) {
         int lx = p % TILE_W, ly = p / TILE_W;
         if (lx < cw && ly < chh) {
             int64_t gi = (int64_t)(cy0 + ly) * a.W + cx0 + lx;
-            s.col[p] = a.clear_color_pending ? a.clear_color : a.color[gi];
-            s.dep[p] = a.clear_depth_pending ? a.clear_depth : a.depth[gi];
+            col[p] = a.clear_color_pending ? a.clear_color : a.color[gi];
+            dep[p] = a.clear_depth_pending ? a.clear_depth : a.depth[gi];
         }
-        s.owner[p] = OWNER_NONE;
+        owner[p] = OWNER_NONE;
     }
-    if (tid == 0) {
-        s.total = 0;
-        s.updated = 0;
-    }
-
-    AttrSrc src;
-    src.pos = a.in.pos;
-    src.nrm = a.in.nrm;
-    src.uv = a.in.uv;
-    src.col = a.in.col;
-    src.side = a.side_vtx;
-    const uint32_t *vals = a.vals[a.sorted_buf];
     const double bias = a.st.depth_bias;
-    uint32_t my_total = 0, my_updated = 0;
-
-    uint32_t lp = ls, io = 0;
+    const bool rd = a.st.read_depth != 0, wd = a.st.write_depth != 0, wc = a.st.write_color != 0;
+    const bool ab = a.st.alpha_blend != 0;
+    uint32_t my_updated = 0;
     __syncthreads();
-    PT_MARK(0)
-    while (lp < le) {
-        // ---- batch formation: items of triangles lp, lp+1, ... that fit ----
-        uint32_t idx = lp + (uint32_t)tid;
-        bool valid = idx < le;
-        uint32_t ref = 0, ni = 0, wb = 0;
-        if (valid) {
-            ref = vals[idx];
-            const PrimRec *pr = prim_of(a, ref);
-            ItemGeo g = item_geo(pr->x0, pr->y0, pr->x1, pr->y1, a.W, cx0, cx1, cy0, cy1);
-            uint64_t n64 = (uint64_t)g.nY * (uint64_t)g.nk;
-            if (tid == 0) n64 -= io;
-            ni = n64 > (1u << 20) ? (1u << 20) : (uint32_t)n64;
-            wb = (uint32_t)g.wb;
-        }
-        uint32_t bnd = ni * wb;
-        // inclusive scans of bnd and ni over the block
-        uint32_t sb = bnd, si = ni;
-        for (int off = 1; off < 64; off <<= 1) {
-            uint32_t yb = __shfl_up(sb, off), yi = __shfl_up(si, off);
-            if (lane >= off) {
-                sb += yb;
-                si += yi;
-            }
-        }
-        if (lane == 63) {
-            s.wsum[0][wave] = sb;
-            s.wsum[1][wave] = si;
-        }
-        __syncthreads();
-        for (int w = 0; w < wave; w++) {
-            sb += s.wsum[0][w];
-            si += s.wsum[1][w];
-        }
-        uint32_t be = sb - bnd, ie = si - ni;  // exclusive
-        bool full = valid && sb <= (uint32_t)FRAG_CAP && si <= (uint32_t)MAX_ITEMS;
-        int nfull = __syncthreads_count(full);
-        uint32_t take = full ? ni : 0u;
-        if (tid == nfull && valid) {
-            uint32_t byb = wb ? ((uint32_t)FRAG_CAP - be) / wb : ni;
-            uint32_t byi = (uint32_t)MAX_ITEMS - ie;
-            take = byb < byi ? byb : byi;
-            if (take > ni) take = ni;
-            s.nitems = ie + take;
-            if (nfull == 0) {
-                s.next_lp = lp;
-                s.next_io = io + take;
-            } else {
-                s.next_lp = lp + (uint32_t)nfull;
-                s.next_io = take;
-            }
-        }
-        if (tid == nfull - 1 && (nfull == TILE_THREADS || lp + (uint32_t)nfull >= le)) {
-            s.nitems = si;
-            s.next_lp = lp + (uint32_t)nfull;
-            s.next_io = 0;
-        }
-        if (tid <= nfull && valid) {
-            s.pref[tid] = ref;
-            s.istart[tid] = ie;
-            for (uint32_t m = 0; m < take; m++) s.iprim[ie + m] = (uint8_t)tid;
-        }
-        if (tid == 0) s.nfrag = 0;
-        __syncthreads();
-        const uint32_t nitems = s.nitems;
-        const uint32_t io0 = io;
-        PT_MARK(1)
 
-        // ---- A: coverage walk per (triangle, row) item ----
-        for (uint32_t m = (uint32_t)tid; m < nitems; m += TILE_THREADS) {
-            const bool act = true;
-            uint32_t j = act ? s.iprim[m] : 0u;
-            int32_t y = 0, k = 0, xa = 1, xb = 0, x0 = 0, x1 = 0, y0 = 0, ya = 0;
-            double w00 = 0, w01 = 0, w02 = 0, a12 = 0, a20 = 0, a01 = 0, b12 = 0, b20 = 0, b01 = 0;
-            double ra = 0, ra12 = 0, ra20 = 0, ra01 = 0;
-            if (act) {
-                uint32_t local = m - s.istart[j] + (j == 0 ? io0 : 0u);
-                const PrimRec *pr = prim_of(a, s.pref[j]);
-                x0 = pr->x0; y0 = pr->y0; x1 = pr->x1;
-                ItemGeo g = item_geo(x0, y0, x1, pr->y1, a.W, cx0, cx1, cy0, cy1);
-                ya = g.ya;
-                y = g.ya + (int32_t)(local / (uint32_t)g.nk);
-                k = g.kmin + (int32_t)(local % (uint32_t)g.nk);
-                int32_t py = y + k;
-                int32_t kw = k * a.W;
-                xa = x0 > kw + cx0 ? x0 : kw + cx0;
-                xb = x1 < kw + cx1 ? x1 : kw + cx1;
-                if (py < cy0 || py > cy1) xb = xa - 1;
-                if (xa <= xb) {
-                    double s0x = pr->sx[0], s0y = pr->sy[0];
-                    double s1x = pr->sx[1], s1y = pr->sy[1];
-                    double s2x = pr->sx[2], s2y = pr->sy[2];
-                    // forward differencing variables (context.go:162-181)
-                    double px = (double)x0 + 0.5, pyy = (double)y0 + 0.5;
-                    w00 = edge(s1x, s1y, s2x, s2y, px, pyy);
-                    w01 = edge(s2x, s2y, s0x, s0y, px, pyy);
-                    w02 = edge(s0x, s0y, s1x, s1y, px, pyy);
-                    a01 = s1y - s0y;
-                    b01 = s0x - s1x;
-                    a12 = s2y - s1y;
-                    b12 = s1x - s2x;
-                    a20 = s0y - s2y;
-                    b20 = s2x - s0x;
-                    ra = 1.0 / edge(s0x, s0y, s1x, s1y, s2x, s2y);
-                    ra12 = 1.0 / a12;
-                    ra20 = 1.0 / a20;
-                    ra01 = 1.0 / a01;
-                    // rows y0 .. y-1 of the row recurrence (context.go:275-277)
-                    for (int32_t yy = y0; yy < y; yy++) {
-                        w00 += b12;
-                        w01 += b20;
-                        w02 += b01;
-                    }
-                }
-            }
-            bool walking = act && xa <= xb;
-            double w0 = 0, w1 = 0, w2 = 0;
-            int32_t x = 0;
-            if (walking) {
-                // skip to the first pixel that can be inside (context.go:185-205)
-                double d = 0;
-                double d0 = -w00 * ra12;
-                double d1 = -w01 * ra20;
-                double d2 = -w02 * ra01;
-                if (w00 < 0 && d0 > d) d = d0;
-                if (w01 < 0 && d1 > d) d = d1;
-                if (w02 < 0 && d2 > d) d = d2;
-                d = (double)go_f2i(d);
-                if (d < 0) d = 0;
-                w0 = w00 + a12 * d;
-                w1 = w01 + a20 * d;
-                w2 = w02 + a01 * d;
-                int64_t id = go_f2i(d);
-                x = (id > (int64_t)x1 - x0) ? x1 + 1 : x0 + (int32_t)id;
-            }
-            const uint32_t keyb = (j << 23) | (uint32_t)((y - ya) & 0x7fffff);
-            const int32_t kw = k * a.W;
-            const uint16_t inb = k == 0 ? F_INB : (uint16_t)0;
-            bool was_inside = false;
-            // pixel loop (context.go:207-221); one ballot per step
-            while (__any(walking)) {
-                bool em = false;
-                double b0 = 0, b1 = 0, b2 = 0;
-                if (walking) {
-                    if (x > xb) {
-                        walking = false;
-                    } else {
-                        b0 = w0 * ra;
-                        b1 = w1 * ra;
-                        b2 = w2 * ra;
-                        w0 += a12;
-                        w1 += a20;
-                        w2 += a01;
-                        bool outside = b0 < 0 || b1 < 0 || b2 < 0;
-                        if (outside) {
-                            if (was_inside) walking = false;
-                        } else {
-                            was_inside = true;
-                            em = x >= xa;
-                        }
-                    }
-                }
-                uint16_t pix = 0;
-                if (em) pix = (uint16_t)((y + k - cy0) * TILE_W + (x - kw - cx0));
-                emit_frag(s, em, (uint16_t)(pix | inb), keyb, b0, b1, b2);
-                my_total += em ? 1u : 0u;
-                x++;
+    for (uint32_t fc = f0; fc < f1; fc += ROP_CHUNK) {
+        Frag fr[ROP_PER_THREAD];
+        uint32_t pending = 0;
+#pragma unroll
+        for (int q = 0; q < ROP_PER_THREAD; q++) {
+            uint32_t f = fc + (uint32_t)(q * TILE_THREADS + tid);
+            if (f < f1) {
+                fr[q] = a.frags[f];
+                pending |= 1u << q;
             }
         }
-        __syncthreads();
-        PT_MARK(2)
-
-        // ---- B: shading ----
-        uint32_t nf = s.nfrag;
-        if (nf > (uint32_t)FRAG_CAP) {  // cannot happen: the batch bound caps it
-            if (tid == 0) atomicOr(&a.ctr->overflow, 8u);
-            nf = FRAG_CAP;
-        }
-        for (uint32_t f = (uint32_t)tid; f < nf; f += TILE_THREADS) {
-            uint32_t key = s.fkey[f];
-            uint16_t fp = s.fpix[f];
-            const PrimRec *pr = prim_of(a, s.pref[key >> 23]);
-            double b0 = s.fb0[f], b1 = s.fb1[f], b2 = s.fb2[f];
-            double z = b0 * pr->sz[0] + b1 * pr->sz[1] + b2 * pr->sz[2];
-            double bz = z + bias;
-            if (a.st.early_z && bz > s.dep[fp & F_PIX]) {
-                s.fpix[f] = fp | F_DEAD;
-                continue;
-            }
-            // perspective-correct interpolation (context.go:236-238)
-            V4 bw = v4(b0 * pr->r[0], b1 * pr->r[1], b2 * pr->r[2], 0);
-            bw.w = 1.0 / (bw.x + bw.y + bw.z);
-            C4 c = shade<SH>(a.sh, src, pr->attr, bw);
-            if (c.r == 0 && c.g == 0 && c.b == 0 && c.a == 0) {  // Discard (context.go:241)
-                s.fpix[f] = fp | F_DEAD;
-                continue;
-            }
-            s.fb0[f] = z;
-            s.fb1[f] = bz;
-            s.fcol[f] = nrgba_pack(c);
-            if (a.st.alpha_blend && c.a < 1) fp |= F_BLEND;
-            s.fpix[f] = fp;
-        }
-        __syncthreads();
-        PT_MARK(3)
-
-        // ---- C: ordered write, lowest key per pixel first ----
-        const bool rd = a.st.read_depth != 0, wd = a.st.write_depth != 0, wc = a.st.write_color != 0;
         while (true) {
-            for (uint32_t f = (uint32_t)tid; f < nf; f += TILE_THREADS) {
-                uint16_t fp = s.fpix[f];
-                if (!(fp & (F_DEAD | F_DONE))) atomicMin(&s.owner[fp & F_PIX], s.fkey[f]);
-            }
+#pragma unroll
+            for (int q = 0; q < ROP_PER_THREAD; q++)
+                if (pending & (1u << q)) atomicMin(&owner[fr[q].pix], fc + (uint32_t)(q * TILE_THREADS + tid));
             __syncthreads();
-            int left = 0;
-            for (uint32_t f = (uint32_t)tid; f < nf; f += TILE_THREADS) {
-                uint16_t fp = s.fpix[f];
-                if (fp & (F_DEAD | F_DONE)) continue;
-                uint32_t p = fp & F_PIX;
-                uint32_t key = s.fkey[f];
-                if (s.owner[p] != key) {
-                    left = 1;
-                    continue;
-                }
-                double z = s.fb0[f], bz = s.fb1[f];
-                if (bz <= s.dep[p] || !rd) {
+#pragma unroll
+            for (int q = 0; q < ROP_PER_THREAD; q++) {
+                if (!(pending & (1u << q))) continue;
+                const uint32_t p = fr[q].pix;
+                if (owner[p] != fc + (uint32_t)(q * TILE_THREADS + tid)) continue;
+                // context.go:230-273 at this pixel, in order
+                const double z = fr[q].z;
+                const double bz = z + bias;
+                const uint16_t fl = fr[q].flags;
+                if (!(rd && bz > dep[p]) && !(fl & FRAG_DISCARD) && (bz <= dep[p] || !rd)) {
                     my_updated++;
-                    if (wd) s.dep[p] = z;
+                    if (wd) dep[p] = z;
                     if (wc) {
-                        if (fp & F_BLEND) s.col[p] = blend_over(s.col[p], s.fcol[f]);
-                        else if (fp & F_INB) s.col[p] = s.fcol[f];
+                        if (ab && (fl & FRAG_BLEND)) col[p] = blend_over(col[p], fr[q].color);
+                        else if (fl & FRAG_INB) col[p] = fr[q].color;
                     }
                 }
-                s.fpix[f] = fp | F_DONE;
-                s.owner[p] = OWNER_NONE;
+                pending &= ~(1u << q);
+                owner[p] = OWNER_NONE;
             }
-            if (!__syncthreads_or(left)) break;
+            if (!__syncthreads_or(pending != 0)) break;
         }
-
-        lp = s.next_lp;
-        io = s.next_io;
-        __syncthreads();
-        PT_MARK(4)
     }
 
-    // write the tile back
     for (int p = tid; p < TILE_PIX; p += TILE_THREADS) {
         int lx = p % TILE_W, ly = p / TILE_W;
         if (lx < cw && ly < chh) {
             int64_t gi = (int64_t)(cy0 + ly) * a.W + cx0 + lx;
-            a.color[gi] = s.col[p];
-            a.depth[gi] = s.dep[p];
+            a.color[gi] = col[p];
+            a.depth[gi] = dep[p];
         }
     }
-    atomicAdd(&s.total, (unsigned long long)my_total);
-    atomicAdd(&s.updated, (unsigned long long)my_updated);
-    __syncthreads();
-    PT_MARK(5)
-    PT_FLUSH
-    if (tid == 0) {
-        atomicAdd(&a.ctr->total, s.total);
-        atomicAdd(&a.ctr->updated, s.updated);
-    }
+    // RasterizeInfo: every fragment is an inside pixel with a valid index
+    unsigned long long uu = my_updated;
+    for (int off = 32; off > 0; off >>= 1) uu += __shfl_down(uu, off);
+    if (lane == 0 && uu) atomicAdd(&a.ctr->updated, uu);
+    if (tid == 0) atomicAdd(&a.ctr->total, (unsigned long long)(f1 - f0));
 }
 
-hipError_t launch_tile(const DrawArgs &a, hipStream_t s) {
-    dim3 grid(a.ntiles), block(TILE_THREADS);
+// ---------------------------------------------------------------------------
+
+static unsigned span_grid(const DrawArgs &a) {
+    return (unsigned)(((uint64_t)a.span_cap + SPAN_THREADS - 1) / SPAN_THREADS);
+}
+
+hipError_t launch_frag_count(const DrawArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_frag_count, dim3(span_grid(a)), dim3(SPAN_THREADS), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_frag_scan(const DrawArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_frag_scan, dim3(1), dim3(1024), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_shade(const DrawArgs &a, hipStream_t s) {
+    dim3 grid(span_grid(a)), block(SPAN_THREADS);
     switch (a.sh.kind) {
-        case 1: hipLaunchKernelGGL(k_tile<1>, grid, block, 0, s, a); break;
-        case 2: hipLaunchKernelGGL(k_tile<2>, grid, block, 0, s, a); break;
-        default: hipLaunchKernelGGL(k_tile<3>, grid, block, 0, s, a); break;
+        case 1: hipLaunchKernelGGL(k_shade<1>, grid, block, 0, s, a); break;
+        case 2: hipLaunchKernelGGL(k_shade<2>, grid, block, 0, s, a); break;
+        default:
+            if (a.sh.has_tex) hipLaunchKernelGGL(k_shade<4>, grid, block, 0, s, a);
+            else hipLaunchKernelGGL(k_shade<3>, grid, block, 0, s, a);
+            break;
     }
+    return hipGetLastError();
+}
+hipError_t launch_tile(const DrawArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_rop, dim3(a.ntiles), dim3(TILE_THREADS), 0, s, a);
     return hipGetLastError();
 }
 

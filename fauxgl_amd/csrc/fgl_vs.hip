@@ -71,7 +71,8 @@ FGL_D bool make_raster(const DrawArgs &a, V3 s0, V3 s1, V3 s2, double W0, double
         out->attr = attr;
         out->x0 = (int32_t)x0; out->y0 = (int32_t)y0;
         out->x1 = (int32_t)x1; out->y1 = (int32_t)y1;
-        out->pad = 0;
+        // ra = 1 / edge(s0, s1, s2) (context.go:175)
+        out->ra = 1.0 / ((s1.x - s2.x) * (s0.y - s2.y) - (s1.y - s2.y) * (s0.x - s2.x));
     }
     return true;
 }
@@ -382,8 +383,21 @@ __global__ __launch_bounds__(VS_THREADS) void k_vs_fast(DrawArgs a) {
         V4 o1 = mul_position_w(a.sh.M, P1);
         V4 o2 = mul_position_w(a.sh.M, P2);
         if (woutside(o0) || woutside(o1) || woutside(o2)) {
-            uint32_t slot = atomicAdd(&a.ctr->ndeferred, 1u);
-            a.deferred[slot] = (uint32_t)i;
+            // sutherlandHodgman (clipping.go:28-52) leaves the triangle
+            // untouched by every plane all three vertices are in front of;
+            // if the first plane that is not so has no vertex in front, its
+            // pass emits nothing and ClipTriangle returns no triangles.
+            bool rejected = false;
+            for (int p = 0; p < 6; p++) {
+                bool f0 = point_in_front(p, o0), f1 = point_in_front(p, o1), f2 = point_in_front(p, o2);
+                if (f0 && f1 && f2) continue;
+                rejected = !f0 && !f1 && !f2;
+                break;
+            }
+            if (!rejected) {
+                uint32_t slot = atomicAdd(&a.ctr->ndeferred, 1u);
+                a.deferred[slot] = (uint32_t)i;
+            }
         } else {
             const uint32_t ident[3] = {0, 1, 2};
             Emitter em{a.rec_main + i, 0, 0};
