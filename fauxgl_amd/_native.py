@@ -29,10 +29,15 @@ EXPORTED = [
     "fgl_clear_depth", "fgl_texture_create", "fgl_texture_destroy", "fgl_draw_triangles", "fgl_draw_lines",
     "fgl_mesh_create", "fgl_mesh_destroy", "fgl_draw_mesh", "fgl_read_color", "fgl_read_depth",
     "fgl_write_color", "fgl_write_depth", "fgl_depth_image", "fgl_resolve", "fgl_profile_enable",
-    "fgl_profile_read", "fgl_profile_reset", "fgl_ctx_sync",
+    "fgl_profile_read", "fgl_profile_reset", "fgl_ctx_sync", "fgl_draw_stats",
 ]
 
 _d = ctypes.c_double
+
+
+class FglDrawStat(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("total_pixels", "updated_pixels", "clipped_triangles",
+                                                "side_triangles", "pairs", "spans", "fragments")]
 
 
 class FglState(ctypes.Structure):

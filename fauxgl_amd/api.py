@@ -609,6 +609,13 @@ class Context:
     def Sync(self):
         N.check(N.load().fgl_ctx_sync(self._ptr()), self._ptr())
 
+    def DrawStats(self):
+        """Work counts of the last draw (pairs, spans, fragments, ...); a
+        diagnostic with no reference counterpart."""
+        st = N.FglDrawStat()
+        N.check(N.load().fgl_draw_stats(self._ptr(), ctypes.byref(st)), self._ptr())
+        return {name: int(getattr(st, name)) for name, _ in N.FglDrawStat._fields_}
+
 
 def NewContext(width, height):
     return Context(width, height)

@@ -5,6 +5,7 @@
 // submission order, which is the order the reference draws them in
 // (context.go:413-433 with one worker).
 #include "fgl_internal.h"
+#include "fgl_scan.h"
 #include "fgl_tiles.h"
 
 namespace fgl {
@@ -35,33 +36,11 @@ hipError_t launch_zero(const DrawArgs &a, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // exclusive scan of the vertex-stage block sums (one workgroup)
 
-constexpr int SCAN_THREADS = 1024;
-
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(DrawArgs a) {
-    __shared__ uint64_t part[SCAN_THREADS];
-    const int n = a.nvsblocks;
+    __shared__ uint64_t lds[SCAN_THREADS / 64];
     const int tid = threadIdx.x;
-    const int per = (n + SCAN_THREADS - 1) / SCAN_THREADS;
-    const int lo = tid * per;
-    const int hi = lo + per < n ? lo + per : n;
-    uint64_t s = 0;
-    for (int k = lo; k < hi; k++) s += a.blocksum[k];
-    part[tid] = s;
-    __syncthreads();
-    for (int off = 1; off < SCAN_THREADS; off <<= 1) {
-        uint64_t v = tid >= off ? part[tid - off] : 0;
-        __syncthreads();
-        part[tid] += v;
-        __syncthreads();
-    }
-    uint64_t run = part[tid] - s;  // exclusive
-    for (int k = lo; k < hi; k++) {
-        uint64_t v = a.blocksum[k];
-        a.blockoff[k] = run;
-        run += v;
-    }
+    const uint64_t total = scan_block_sums(a.blocksum, a.blockoff, a.nvsblocks, lds);
     if (tid == SCAN_THREADS - 1) {
-        uint64_t total = part[tid];
         uint32_t ovf = a.ctr->overflow;
         if (total > a.pair_cap) ovf |= 4u;
         a.ctr->npairs = total > a.pair_cap ? a.pair_cap : (uint32_t)total;

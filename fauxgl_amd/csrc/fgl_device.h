@@ -98,7 +98,6 @@ struct DrawCounters {
     uint32_t nfrags;             // fragments (clamped to capacity)
     uint32_t frags_lo, frags_hi; // unclamped fragment count
     uint32_t pad2;
-    unsigned long long phase[8];  // tile-kernel phase cycles (instrumented builds only)
 };
 
 // Coverage span: the covered pixels of one row of one raster triangle in

@@ -85,11 +85,12 @@ struct fgl_ctx {
     int64_t next_tex = 1;
 
     DevBuf ctr, ntl, nrast, sstart, blocksum, blockoff, rec_main, side_rec, side_vtx;
-    DevBuf keys0, vals0, keys1, vals1, hist, dtot, tstart, tend, deferred;
+    DevBuf keys0, vals0, keys1, vals1, hist, dtot, tstart, tend, tfrag, tupd, deferred;
     DevBuf pspans, span_bsum, span_boff, poff, spans;
     DevBuf frag_bsum, frag_boff, foff, frags;
     uint32_t span_cap = 0, frag_cap = 0;
     DevBuf in_pos, in_nrm, in_uv, in_col;
+    DevBuf zeros;
     DevBuf rs_tmp, rs_wx, rs_ox, rs_wy, rs_oy;
     DevBuf dpart, dimg;
     uint32_t side_rec_cap = 0, side_vtx_cap = 0, pair_cap = 0;
@@ -334,6 +335,10 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
     if (c->frag_cap < want_frags) c->frag_cap = (uint32_t)std::min<uint64_t>(want_frags, 0xfffff000u);
 
     HIPCHK(c, c->ctr.ensure(sizeof(DrawCounters)));
+    if (!c->zeros.p) {
+        HIPCHK(c, c->zeros.ensure(16 * sizeof(double)));
+        HIPCHK(c, hipMemsetAsync(c->zeros.p, 0, 16 * sizeof(double), c->stream));
+    }
     HIPCHK(c, c->ntl.ensure((size_t)n * 4));
     HIPCHK(c, c->nrast.ensure((size_t)n * 4));
     HIPCHK(c, c->sstart.ensure((size_t)n * 4));
@@ -344,6 +349,8 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
     HIPCHK(c, c->dtot.ensure(256 * 4));
     HIPCHK(c, c->tstart.ensure((size_t)a.ntiles * 4));
     HIPCHK(c, c->tend.ensure((size_t)a.ntiles * 4));
+    HIPCHK(c, c->tfrag.ensure((size_t)a.ntiles * 8));
+    HIPCHK(c, c->tupd.ensure((size_t)a.ntiles * 4));
 
     for (int attempt = 0; attempt < 6; attempt++) {
         HIPCHK(c, c->side_rec.ensure((size_t)c->side_rec_cap * sizeof(PrimRec)));
@@ -397,11 +404,14 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
         a.deferred = c->deferred.as<uint32_t>();
         a.tstart = c->tstart.as<uint32_t>();
         a.tend = c->tend.as<uint32_t>();
+        a.tfrag = c->tfrag.as<uint2>();
+        a.tupd = c->tupd.as<uint32_t>();
         a.clear_color_pending = c->pend_color;
         a.clear_depth_pending = c->pend_depth;
         a.clear_color = c->clear_color;
         a.clear_depth = c->clear_depth;
         a.ctr = c->ctr.as<DrawCounters>();
+        a.zeros = c->zeros.as<double>();
 
         if ((r = launch(c, "zero", [&] { return launch_zero(a, c->stream); }))) return r;
         if ((r = launch(c, "vertex", [&] { return launch_vs(a, c->stream); }))) return r;
@@ -418,10 +428,11 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
         if ((r = launch(c, "span_count", [&] { return launch_span_count(a, c->stream); }))) return r;
         if ((r = launch(c, "span_scan", [&] { return launch_span_scan(a, c->stream); }))) return r;
         if ((r = launch(c, "spans", [&] { return launch_spans(a, c->stream); }))) return r;
-        if ((r = launch(c, "frag_count", [&] { return launch_frag_count(a, c->stream); }))) return r;
         if ((r = launch(c, "frag_scan", [&] { return launch_frag_scan(a, c->stream); }))) return r;
         if ((r = launch(c, "shade", [&] { return launch_shade(a, c->stream); }))) return r;
+        if ((r = launch(c, "tile_frags", [&] { return launch_tile_frags(a, c->stream); }))) return r;
         if ((r = launch(c, "rop", [&] { return launch_tile(a, c->stream); }))) return r;
+        if ((r = launch(c, "updated", [&] { return launch_updated(a, c->stream); }))) return r;
 
         DrawCounters h;
         HIPCHK(c, hipMemcpyAsync(&h, a.ctr, sizeof h, hipMemcpyDeviceToHost, c->stream));
@@ -954,14 +965,18 @@ int fgl_profile_reset(fgl_ctx *ctx) {
     return FGL_OK;
 }
 
-#ifdef FGL_TILE_PHASE_TIMERS
-// Profiling builds only: tile-kernel phase cycles of the last draw.
-int fgl_debug_phase_cycles(fgl_ctx *ctx, uint64_t out[8]) {
-    if (check_ctx(ctx)) return FGL_E_INVALID;
-    for (int k = 0; k < 8; k++) out[k] = ctx->last.phase[k];
+int fgl_draw_stats(fgl_ctx *ctx, fgl_draw_stat *out) {
+    if (check_ctx(ctx) || !out) return FGL_E_INVALID;
+    const DrawCounters &h = ctx->last;
+    out->total_pixels = h.total;
+    out->updated_pixels = h.updated;
+    out->clipped_triangles = h.ndeferred;
+    out->side_triangles = h.side_rec;
+    out->pairs = (uint64_t)h.pairs_hi << 32 | h.pairs_lo;
+    out->spans = (uint64_t)h.spans_hi << 32 | h.spans_lo;
+    out->fragments = (uint64_t)h.frags_hi << 32 | h.frags_lo;
     return FGL_OK;
 }
-#endif
 
 int fgl_ctx_sync(fgl_ctx *ctx) {
     if (check_ctx(ctx)) return FGL_E_INVALID;

@@ -24,6 +24,7 @@ struct DrawArgs {
     int32_t ntiles;        // ntx * nty
     int32_t kind;          // PrimKind
     DevPrims in;
+    const double *zeros;   // 16 zero doubles standing in for a missing attribute array
     DevState st;
     DevShader sh;
     double S[16];          // Screen(W, H) (matrix.go:119-128)
@@ -61,11 +62,13 @@ struct DrawArgs {
     uint32_t span_cap;
 
     // shaded fragments
-    uint64_t *frag_blocksum;    // per SPAN_THREADS spans
+    uint64_t *frag_blocksum;    // fragments per SPAN_THREADS spans (accumulated by k_spans)
     uint64_t *frag_blockoff;
     uint32_t *foff;             // first fragment of each span (span_cap + 1 entries)
     Frag *frags;
     uint32_t frag_cap;
+    uint2 *tfrag;               // per tile: [first, end) of its fragments
+    uint32_t *tupd;             // per tile: pixels updated
 
     // framebuffer and deferred clears
     uint32_t *color;       // NRGBA8 as little-endian u32
@@ -97,10 +100,11 @@ hipError_t launch_ranges(const DrawArgs &a, hipStream_t s);
 hipError_t launch_span_count(const DrawArgs &a, hipStream_t s);
 hipError_t launch_span_scan(const DrawArgs &a, hipStream_t s);
 hipError_t launch_spans(const DrawArgs &a, hipStream_t s);
-hipError_t launch_frag_count(const DrawArgs &a, hipStream_t s);
 hipError_t launch_frag_scan(const DrawArgs &a, hipStream_t s);
 hipError_t launch_shade(const DrawArgs &a, hipStream_t s);
+hipError_t launch_tile_frags(const DrawArgs &a, hipStream_t s);
 hipError_t launch_tile(const DrawArgs &a, hipStream_t s);
+hipError_t launch_updated(const DrawArgs &a, hipStream_t s);
 
 hipError_t launch_fill_color(uint32_t *color, int64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_fill_depth(double *depth, int64_t n, double v, hipStream_t s);

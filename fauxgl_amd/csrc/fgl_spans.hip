@@ -9,6 +9,7 @@
 // those edge values, so every barycentric it uses is bit-identical to the
 // reference's serial loop.
 #include "fgl_internal.h"
+#include "fgl_scan.h"
 #include "fgl_tiles.h"
 
 namespace fgl {
@@ -57,33 +58,17 @@ __global__ __launch_bounds__(SPAN_THREADS) void k_span_count(DrawArgs a) {
 
 // Exclusive scan of the per-block span sums (one workgroup), total and
 // capacity check.
-__global__ __launch_bounds__(1024) void k_span_scan(DrawArgs a) {
-    __shared__ uint64_t part[1024];
+__global__ __launch_bounds__(SCAN_THREADS) void k_span_scan(DrawArgs a) {
+    __shared__ uint64_t lds[SCAN_THREADS / 64];
     if (a.ctr->overflow) return;
     const uint32_t E = a.ctr->npairs;
     const int nb = (int)((E + SPAN_THREADS - 1) / SPAN_THREADS);
-    const int tid = threadIdx.x;
-    const int per = (nb + 1023) / 1024;
-    const int lo = tid * per;
-    const int hi = lo + per < nb ? lo + per : nb;
-    uint64_t s = 0;
-    for (int k = lo; k < hi; k++) s += a.span_blocksum[k];
-    part[tid] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint64_t v = tid >= off ? part[tid - off] : 0;
-        __syncthreads();
-        part[tid] += v;
-        __syncthreads();
-    }
-    uint64_t run = part[tid] - s;
-    for (int k = lo; k < hi; k++) {
-        uint64_t v = a.span_blocksum[k];
-        a.span_blockoff[k] = run;
-        run += v;
-    }
-    if (tid == 1023) {
-        uint64_t total = part[tid];
+    const uint64_t total = scan_block_sums(a.span_blocksum, a.span_blockoff, nb, lds);
+    // k_spans adds each span block's fragment count into frag_blocksum
+    const uint64_t ns = total > a.span_cap ? a.span_cap : total;
+    const uint32_t nfb = (uint32_t)((ns + SPAN_THREADS - 1) / SPAN_THREADS);
+    for (uint32_t k = threadIdx.x; k < nfb; k += SCAN_THREADS) a.frag_blocksum[k] = 0;
+    if (threadIdx.x == SCAN_THREADS - 1) {
         if (total > a.span_cap) atomicOr(&a.ctr->overflow, 16u);
         a.ctr->nspans = total > a.span_cap ? a.span_cap : (uint32_t)total;
         a.ctr->spans_lo = (uint32_t)total;
@@ -97,31 +82,20 @@ FGL_D double edge(double ax, double ay, double bx, double by, double cx, double 
     return (bx - cx) * (ay - cy) - (by - cy) * (ax - cx);
 }
 
-// One thread per pair: setup once, then the rows in order.
-__global__ __launch_bounds__(SPAN_THREADS) void k_spans(DrawArgs a) {
-    __shared__ uint64_t wsum[SPAN_THREADS / 64];
-    if (a.ctr->overflow) return;
-    const uint32_t E = a.ctr->npairs;
-    const uint32_t *keys = a.keys[a.sorted_buf];
-    const uint32_t *vals = a.vals[a.sorted_buf];
-    const uint32_t p = blockIdx.x * SPAN_THREADS + threadIdx.x;
-    const uint32_t n = p < E ? a.pspans[p] : 0u;
-    // block-exclusive scan of n
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint64_t v = n;
-    for (int off = 1; off < 64; off <<= 1) {
-        uint64_t y = __shfl_up(v, off);
-        if (lane >= off) v += y;
-    }
-    if (lane == 63) wsum[wave] = v;
-    __syncthreads();
-    uint64_t base = a.span_blockoff[blockIdx.x];
-    for (int w = 0; w < wave; w++) base += wsum[w];
-    base += v - n;
-    if (p >= E) return;
-    a.poff[p] = (uint32_t)base;
-    if (n == 0) return;
+// Fragment counts per SPAN_THREADS-span block: collected in LDS for the
+// FB_SLOTS blocks from the workgroup's first span on, in global memory
+// beyond that.
+constexpr uint32_t FB_SLOTS = 64;
 
+FGL_D void add_frags(const DrawArgs &a, uint32_t *lfb, uint32_t blk0, uint32_t blk, uint64_t cnt) {
+    if (!cnt) return;
+    if (blk - blk0 < FB_SLOTS) atomicAdd(&lfb[blk - blk0], (uint32_t)cnt);
+    else atomicAdd((unsigned long long *)&a.frag_blocksum[blk], (unsigned long long)cnt);
+}
+
+// One pair: setup once, then the rows in order (spans written at base..).
+FGL_D void walk_pair(const DrawArgs &a, uint32_t p, uint32_t n, uint64_t base, const uint32_t *keys,
+                     const uint32_t *vals, uint32_t *lfb, uint32_t blk0) {
     const uint32_t ref = vals[p];
     int32_t cx0, cx1, cy0, cy1;
     tile_rect(a, keys[p], cx0, cx1, cy0, cy1);
@@ -148,6 +122,9 @@ __global__ __launch_bounds__(SPAN_THREADS) void k_spans(DrawArgs a) {
     }
     Span *out = a.spans + base;
     uint32_t m = 0;
+    // fragments per SPAN_THREADS-span block, added once per block touched
+    uint32_t fblk = (uint32_t)(base / SPAN_THREADS);
+    uint64_t facc = 0;
     for (int32_t yy = 0; yy < g.nY && m < n; yy++) {
         const int32_t y = g.ya + yy;
         // skip to the first pixel that can be inside (context.go:185-206)
@@ -203,6 +180,13 @@ __global__ __launch_bounds__(SPAN_THREADS) void k_spans(DrawArgs a) {
                     w2 += a01;
                 }
                 sp.n = (uint16_t)(cnt | (k == 0 ? SPAN_INB : 0u));
+                const uint32_t blk = (uint32_t)((base + m) / SPAN_THREADS);
+                if (blk != fblk) {
+                    add_frags(a, lfb, blk0, fblk, facc);
+                    fblk = blk;
+                    facc = 0;
+                }
+                facc += cnt;
             }
             out[m] = sp;
         }
@@ -210,6 +194,42 @@ __global__ __launch_bounds__(SPAN_THREADS) void k_spans(DrawArgs a) {
         w01 += b20;
         w02 += b01;
     }
+    add_frags(a, lfb, blk0, fblk, facc);
+}
+
+// One thread per pair.  Also adds the covered pixels into the fragment
+// count of each span block it writes.
+__global__ __launch_bounds__(SPAN_THREADS) void k_spans(DrawArgs a) {
+    __shared__ uint64_t wsum[SPAN_THREADS / 64];
+    __shared__ uint32_t lfb[FB_SLOTS];
+    if (a.ctr->overflow) return;
+    const uint32_t E = a.ctr->npairs;
+    const uint32_t *keys = a.keys[a.sorted_buf];
+    const uint32_t *vals = a.vals[a.sorted_buf];
+    const uint32_t p = blockIdx.x * SPAN_THREADS + threadIdx.x;
+    const uint32_t n = p < E ? a.pspans[p] : 0u;
+    if (threadIdx.x < FB_SLOTS) lfb[threadIdx.x] = 0;
+    // block-exclusive scan of n
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint64_t v = n;
+    for (int off = 1; off < 64; off <<= 1) {
+        uint64_t y = __shfl_up(v, off);
+        if (lane >= off) v += y;
+    }
+    if (lane == 63) wsum[wave] = v;
+    __syncthreads();
+    const uint64_t bbase = a.span_blockoff[blockIdx.x];
+    const uint32_t blk0 = (uint32_t)(bbase / SPAN_THREADS);
+    uint64_t base = bbase;
+    for (int w = 0; w < wave; w++) base += wsum[w];
+    base += v - n;
+    if (p < E) {
+        a.poff[p] = (uint32_t)base;
+        if (n) walk_pair(a, p, n, base, keys, vals, lfb, blk0);
+    }
+    __syncthreads();
+    if (threadIdx.x < FB_SLOTS && lfb[threadIdx.x])
+        atomicAdd((unsigned long long *)&a.frag_blocksum[blk0 + threadIdx.x], (unsigned long long)lfb[threadIdx.x]);
 }
 
 static unsigned span_blocks(const DrawArgs &a) {
@@ -221,7 +241,7 @@ hipError_t launch_span_count(const DrawArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 hipError_t launch_span_scan(const DrawArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_span_scan, dim3(1), dim3(1024), 0, s, a);
+    hipLaunchKernelGGL(k_span_scan, dim3(1), dim3(SCAN_THREADS), 0, s, a);
     return hipGetLastError();
 }
 hipError_t launch_spans(const DrawArgs &a, hipStream_t s) {

@@ -17,6 +17,7 @@
 // test and the depth / color write exactly as the serial loop would at
 // that moment.
 #include "fgl_internal.h"
+#include "fgl_scan.h"
 #include "fgl_tiles.h"
 
 namespace fgl {
@@ -167,48 +168,56 @@ __device__ __noinline__ C4 bilinear_sample(const DevShader &sh, double u, double
 // ---------------------------------------------------------------------------
 // vertex attribute fetch for the shaders
 
+// Per-vertex attribute arrays.  A missing array (a mesh without normals,
+// texture coordinates or colors) is replaced by a run of zeros read with
+// stride 0, so every fetch is the same branch-free address computation.
 struct AttrSrc {
-    const double *pos, *nrm, *uv, *col;  // per-vertex arrays (stride 3, 3, 2, 4)
+    const double *pos, *nrm, *uv, *col;  // stride 3, 3, 2, 4 doubles
+    int32_t spos, snrm, suv, scol;       // strides (0 for a missing array)
     const double *side;                  // side vertices (stride SIDE_VTX_DOUBLES)
 };
 
-FGL_D void vtx_addr(const AttrSrc &src, uint64_t attr, int k, const double *&base, int64_t &vid, bool &side) {
-    uint32_t kind = attr_kind(attr);
-    uint64_t idx = attr_index(attr);
-    uint32_t sel = attr_sel(attr, k);
-    side = kind == ATTR_SIDE;
-    vid = kind == ATTR_TRI ? (int64_t)(idx * 3 + sel) : kind == ATTR_LINE ? (int64_t)(idx * 2 + sel) : (int64_t)(idx + sel);
-    base = side ? src.side + vid * SIDE_VTX_DOUBLES : nullptr;
+FGL_D AttrSrc attr_src(const DrawArgs &a) {
+    AttrSrc s;
+    s.pos = a.in.pos ? a.in.pos : a.zeros;
+    s.nrm = a.in.nrm ? a.in.nrm : a.zeros;
+    s.uv = a.in.uv ? a.in.uv : a.zeros;
+    s.col = a.in.col ? a.in.col : a.zeros;
+    s.spos = a.in.pos ? 3 : 0;
+    s.snrm = a.in.nrm ? 3 : 0;
+    s.suv = a.in.uv ? 2 : 0;
+    s.scol = a.in.col ? 4 : 0;
+    s.side = a.side_vtx;
+    return s;
+}
+
+// Address of vertex k's attribute: the input array (triangle 3i+sel, line
+// 2i+sel) or the side vertex (i+sel) at offset side_off.
+FGL_D const double *attr_ptr(const AttrSrc &src, uint64_t attr, int k, const double *arr, int32_t stride,
+                             int side_off) {
+    const uint32_t kind = attr_kind(attr);
+    const int64_t idx = (int64_t)attr_index(attr);
+    const int64_t m = kind == ATTR_TRI ? 3 : kind == ATTR_LINE ? 2 : 1;
+    const int64_t vid = idx * m + attr_sel(attr, k);
+    return kind == ATTR_SIDE ? src.side + vid * SIDE_VTX_DOUBLES + side_off : arr + vid * stride;
 }
 
 FGL_D V3 fetch_pos(const AttrSrc &src, uint64_t attr, int k) {
-    const double *b; int64_t vid; bool side;
-    vtx_addr(src, attr, k, b, vid, side);
-    if (side) return v3(b[0], b[1], b[2]);
-    if (!src.pos) return v3(0, 0, 0);
-    return v3(src.pos[3 * vid], src.pos[3 * vid + 1], src.pos[3 * vid + 2]);
+    const double *q = attr_ptr(src, attr, k, src.pos, src.spos, 0);
+    return v3(q[0], q[1], q[2]);
 }
 FGL_D V3 fetch_nrm(const AttrSrc &src, uint64_t attr, int k) {
-    const double *b; int64_t vid; bool side;
-    vtx_addr(src, attr, k, b, vid, side);
-    if (side) return v3(b[3], b[4], b[5]);
-    if (!src.nrm) return v3(0, 0, 0);
-    return v3(src.nrm[3 * vid], src.nrm[3 * vid + 1], src.nrm[3 * vid + 2]);
+    const double *q = attr_ptr(src, attr, k, src.nrm, src.snrm, 3);
+    return v3(q[0], q[1], q[2]);
 }
 FGL_D void fetch_uv(const AttrSrc &src, uint64_t attr, int k, double &u, double &v) {
-    const double *b; int64_t vid; bool side;
-    vtx_addr(src, attr, k, b, vid, side);
-    if (side) { u = b[6]; v = b[7]; return; }
-    if (!src.uv) { u = 0; v = 0; return; }
-    u = src.uv[2 * vid];
-    v = src.uv[2 * vid + 1];
+    const double *q = attr_ptr(src, attr, k, src.uv, src.suv, 6);
+    u = q[0];
+    v = q[1];
 }
 FGL_D C4 fetch_col(const AttrSrc &src, uint64_t attr, int k) {
-    const double *b; int64_t vid; bool side;
-    vtx_addr(src, attr, k, b, vid, side);
-    if (side) return c4(b[8], b[9], b[10], b[11]);
-    if (!src.col) return c4(0, 0, 0, 0);
-    return c4(src.col[4 * vid], src.col[4 * vid + 1], src.col[4 * vid + 2], src.col[4 * vid + 3]);
+    const double *q = attr_ptr(src, attr, k, src.col, src.scol, 8);
+    return c4(q[0], q[1], q[2], q[3]);
 }
 
 // InterpolateFloats-style sum: ((((0 + a*bx) + b*by) + c*bz) * bw)
@@ -290,129 +299,143 @@ FGL_D const PrimRec *prim_of(const DrawArgs &a, uint32_t ref) {
 }
 
 // ---------------------------------------------------------------------------
-// fragment offsets: exclusive scan of span lengths
+// fragment offsets: exclusive scan of the per-block fragment counts k_spans
+// accumulated
 
-__global__ __launch_bounds__(SPAN_THREADS) void k_frag_count(DrawArgs a) {
-    __shared__ uint64_t wsum[SPAN_THREADS / 64];
-    if (a.ctr->overflow) return;
-    const uint32_t NS = a.ctr->nspans;
-    const uint32_t i = blockIdx.x * SPAN_THREADS + threadIdx.x;
-    uint64_t v = i < NS ? (a.spans[i].n & SPAN_N) : 0u;
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
-    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint64_t s = 0;
-        for (int w = 0; w < SPAN_THREADS / 64; w++) s += wsum[w];
-        a.frag_blocksum[blockIdx.x] = s;
-    }
-}
+// Blocks of the span-indexed kernels loop over the SPAN_THREADS-span blocks
+// that exist, so the grid (sized from capacity) need not match the count.
+constexpr uint32_t SPAN_GRID_MAX = 2048;
 
-__global__ __launch_bounds__(1024) void k_frag_scan(DrawArgs a) {
-    __shared__ uint64_t part[1024];
+__global__ __launch_bounds__(SCAN_THREADS) void k_frag_scan(DrawArgs a) {
+    __shared__ uint64_t lds[SCAN_THREADS / 64];
     if (a.ctr->overflow) return;
     const uint32_t NS = a.ctr->nspans;
     const int nb = (int)((NS + SPAN_THREADS - 1) / SPAN_THREADS);
-    const int tid = threadIdx.x;
-    const int per = (nb + 1023) / 1024;
-    const int lo = tid * per;
-    const int hi = lo + per < nb ? lo + per : nb;
-    uint64_t s = 0;
-    for (int k = lo; k < hi; k++) s += a.frag_blocksum[k];
-    part[tid] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint64_t v = tid >= off ? part[tid - off] : 0;
-        __syncthreads();
-        part[tid] += v;
-        __syncthreads();
-    }
-    uint64_t run = part[tid] - s;
-    for (int k = lo; k < hi; k++) {
-        uint64_t v = a.frag_blocksum[k];
-        a.frag_blockoff[k] = run;
-        run += v;
-    }
-    if (tid == 1023) {
-        uint64_t total = part[tid];
+    const uint64_t total = scan_block_sums(a.frag_blocksum, a.frag_blockoff, nb, lds);
+    if (threadIdx.x == SCAN_THREADS - 1) {
         if (total > a.frag_cap) atomicOr(&a.ctr->overflow, 32u);
         a.ctr->nfrags = total > a.frag_cap ? a.frag_cap : (uint32_t)total;
         a.ctr->frags_lo = (uint32_t)total;
         a.ctr->frags_hi = (uint32_t)(total >> 32);
         a.foff[NS] = total > a.frag_cap ? a.frag_cap : (uint32_t)total;
+        // TotalPixels: every fragment is an inside pixel with a valid index
+        a.ctr->total = total;
     }
 }
 
 // ---------------------------------------------------------------------------
-// shading, one thread per span
+// shading
+//
+// Each wave takes 64 consecutive spans (one per lane) and shades their
+// fragments 64 at a time, one fragment per lane: a lane finds the span that
+// owns its fragment by a binary search over the wave's running fragment
+// counts, takes the span's edge values from the owning lane, replays the
+// reference's per-pixel additions up to its pixel and runs the shader.
+
+// Waves per SIMD the untextured Phong kernel is compiled for (its register
+// budget): four hide the attribute-fetch latency better than the three an
+// unconstrained build fits, at the cost of a few spilled registers.
+#ifndef FGL_SHADE_WAVES
+#define FGL_SHADE_WAVES 4
+#endif
 
 template <int SH>
-__global__ __launch_bounds__(SPAN_THREADS) void k_shade(DrawArgs a) {
-    __shared__ uint64_t wsum[SPAN_THREADS / 64];
+__global__ __launch_bounds__(SPAN_THREADS, SH == 3 ? FGL_SHADE_WAVES : 1) void k_shade(DrawArgs a) {
+    __shared__ uint32_t wsum[SPAN_THREADS / 64];
     if (a.ctr->overflow) return;
     const uint32_t NS = a.ctr->nspans;
-    const uint32_t i = blockIdx.x * SPAN_THREADS + threadIdx.x;
-    const Span sp = i < NS ? a.spans[i] : Span{{0, 0, 0}, 0, 0, 0};
-    const uint32_t n = sp.n & SPAN_N;
-    // block-exclusive scan of n
+    const uint32_t nb = (NS + SPAN_THREADS - 1) / SPAN_THREADS;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint64_t v = n;
-    for (int off = 1; off < 64; off <<= 1) {
-        uint64_t y = __shfl_up(v, off);
-        if (lane >= off) v += y;
+    const AttrSrc src = attr_src(a);
+    for (uint32_t blk = blockIdx.x; blk < nb; blk += gridDim.x) {
+        const uint32_t i = blk * SPAN_THREADS + threadIdx.x;
+        const Span sp = i < NS ? a.spans[i] : Span{{0, 0, 0}, 0, 0, 0};
+        const uint32_t n = sp.n & SPAN_N;
+        uint32_t incl = n;  // at most 64 * TILE_W
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(incl, off);
+            if (lane >= off) incl += y;
+        }
+        if (lane == 63) wsum[wave] = incl;
+        __syncthreads();
+        uint64_t fo = a.frag_blockoff[blk];
+        for (int w = 0; w < wave; w++) fo += wsum[w];
+        __syncthreads();
+        const uint32_t excl = incl - n;
+        if (i < NS) a.foff[i] = (uint32_t)(fo + excl);
+        const uint32_t T = __shfl(incl, 63);
+        const uint32_t pn = (uint32_t)sp.pix | ((uint32_t)sp.n << 16);
+        Frag *out = a.frags + fo;
+        for (uint32_t base = 0; base < T; base += 64) {
+            const uint32_t p = base + (uint32_t)lane;
+            // owner: the number of lanes whose running count is <= p
+            int lo = 0;
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1) {
+                const uint32_t v = __shfl(incl, lo + step - 1);
+                if (v <= p) lo += step;
+            }
+            const int ow = lo < 64 ? lo : 63;
+            const uint32_t oexcl = __shfl(excl, ow);
+            const uint32_t oref = __shfl(sp.ref, ow);
+            const uint32_t opn = __shfl(pn, ow);
+            double w0 = __shfl(sp.w[0], ow);
+            double w1 = __shfl(sp.w[1], ow);
+            double w2 = __shfl(sp.w[2], ow);
+            if (p < T) {
+                const uint32_t j = p - oexcl;
+                const PrimRec *pr = prim_of(a, oref);
+                const double a12 = pr->sy[2] - pr->sy[1];
+                const double a20 = pr->sy[0] - pr->sy[2];
+                const double a01 = pr->sy[1] - pr->sy[0];
+                // the reference's per-pixel step, j times (context.go:208-213)
+                for (uint32_t k = 0; k < j; k++) {
+                    w0 += a12;
+                    w1 += a20;
+                    w2 += a01;
+                }
+                const double ra = pr->ra;
+                const double b0 = w0 * ra, b1 = w1 * ra, b2 = w2 * ra;
+                ShadeIn in;
+                load_in<SH>(a.sh, src, pr->attr, in);
+                Frag f;
+                f.z = b0 * pr->sz[0] + b1 * pr->sz[1] + b2 * pr->sz[2];
+                // perspective-correct interpolation weights (context.go:236-237)
+                V4 bw = v4(b0 * pr->r[0], b1 * pr->r[1], b2 * pr->r[2], 0);
+                bw.w = 1.0 / (bw.x + bw.y + bw.z);
+                const C4 c = eval<SH>(a.sh, in, bw);
+                f.color = nrgba_pack(c);
+                f.pix = (uint16_t)((opn & 0xffffu) + j);
+                f.flags = ((opn >> 16) & SPAN_INB ? FRAG_INB : (uint16_t)0) | (c.a < 1 ? FRAG_BLEND : (uint16_t)0) |
+                          ((c.r == 0 && c.g == 0 && c.b == 0 && c.a == 0) ? FRAG_DISCARD : (uint16_t)0);
+                out[p] = f;
+            }
+        }
     }
-    if (lane == 63) wsum[wave] = v;
-    __syncthreads();
-    uint64_t fo = a.frag_blockoff[blockIdx.x];
-    for (int w = 0; w < wave; w++) fo += wsum[w];
-    fo += v - n;
-    if (i >= NS) return;
-    a.foff[i] = (uint32_t)fo;
-    if (n == 0) return;
+}
 
-    const PrimRec *pr = prim_of(a, sp.ref);
-    const double a12 = pr->sy[2] - pr->sy[1];
-    const double a20 = pr->sy[0] - pr->sy[2];
-    const double a01 = pr->sy[1] - pr->sy[0];
-    const double ra = pr->ra;
-    const double sz0 = pr->sz[0], sz1 = pr->sz[1], sz2 = pr->sz[2];
-    const double r0 = pr->r[0], r1 = pr->r[1], r2 = pr->r[2];
-    AttrSrc src;
-    src.pos = a.in.pos;
-    src.nrm = a.in.nrm;
-    src.uv = a.in.uv;
-    src.col = a.in.col;
-    src.side = a.side_vtx;
-    ShadeIn in;
-    load_in<SH>(a.sh, src, pr->attr, in);
-
-    const uint16_t inb = (sp.n & SPAN_INB) ? FRAG_INB : (uint16_t)0;
-    double w0 = sp.w[0], w1 = sp.w[1], w2 = sp.w[2];
-    Frag *out = a.frags + fo;
-    for (uint32_t j = 0; j < n; j++) {
-        // barycentrics at this pixel, then the step (context.go:208-213)
-        const double b0 = w0 * ra, b1 = w1 * ra, b2 = w2 * ra;
-        w0 += a12;
-        w1 += a20;
-        w2 += a01;
-        Frag f;
-        f.z = b0 * sz0 + b1 * sz1 + b2 * sz2;
-        // perspective-correct interpolation weights (context.go:236-237)
-        V4 bw = v4(b0 * r0, b1 * r1, b2 * r2, 0);
-        bw.w = 1.0 / (bw.x + bw.y + bw.z);
-        C4 c = eval<SH>(a.sh, in, bw);
-        f.color = nrgba_pack(c);
-        f.pix = (uint16_t)(sp.pix + j);
-        f.flags = inb | (c.a < 1 ? FRAG_BLEND : (uint16_t)0) |
-                  ((c.r == 0 && c.g == 0 && c.b == 0 && c.a == 0) ? FRAG_DISCARD : (uint16_t)0);
-        out[j] = f;
+// Each tile's fragment range [foff[poff[tstart]], foff[poff[tend]]).
+__global__ __launch_bounds__(256) void k_tile_frags(DrawArgs a) {
+    if (a.ctr->overflow) return;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= a.ntiles) return;
+    const uint32_t ps = a.tstart[t], pe = a.tend[t];
+    uint2 r = make_uint2(0u, 0u);
+    if (pe > ps) {
+        r.x = a.foff[a.poff[ps]];
+        r.y = a.foff[a.poff[pe]];
     }
+    a.tfrag[t] = r;
 }
 
 // ---------------------------------------------------------------------------
 // the ordered framebuffer update, one workgroup per tile
 
-__global__ __launch_bounds__(TILE_THREADS) void k_rop(DrawArgs a) {
+#ifndef FGL_ROP_WAVES
+#define FGL_ROP_WAVES 1
+#endif
+
+__global__ __launch_bounds__(TILE_THREADS, FGL_ROP_WAVES) void k_rop(DrawArgs a) {
     __shared__ uint32_t col[TILE_PIX];
     __shared__ double dep[TILE_PIX];
     __shared__ uint32_t owner[TILE_PIX];
@@ -420,13 +443,14 @@ __global__ __launch_bounds__(TILE_THREADS) void k_rop(DrawArgs a) {
     const int lane = tid & 63;
     const int t = blockIdx.x;
     if (a.ctr->overflow) return;  // the host reruns the whole draw
+    __shared__ uint32_t upd[TILE_THREADS / 64];
     const int tx = t % a.ntx, ty = t / a.ntx;
     const int cx0 = tx * TILE_W, cy0 = ty * TILE_H;
     const int cw = min(TILE_W, a.W - cx0), chh = min(TILE_H, a.H - cy0);
-    const uint32_t ps = a.tstart[t], pe = a.tend[t];
-    const uint32_t f0 = pe > ps ? a.foff[a.poff[ps]] : 0u;
-    const uint32_t f1 = pe > ps ? a.foff[a.poff[pe]] : 0u;
+    const uint2 fr01 = a.tfrag[t];
+    const uint32_t f0 = fr01.x, f1 = fr01.y;
     if (f1 <= f0) {
+        if (tid == 0) a.tupd[t] = 0;
         if (a.clear_color_pending || a.clear_depth_pending) {
             for (int p = tid; p < TILE_PIX; p += TILE_THREADS) {
                 int lx = p % TILE_W, ly = p / TILE_W;
@@ -502,25 +526,44 @@ __global__ __launch_bounds__(TILE_THREADS) void k_rop(DrawArgs a) {
             a.depth[gi] = dep[p];
         }
     }
-    // RasterizeInfo: every fragment is an inside pixel with a valid index
-    unsigned long long uu = my_updated;
+    // UpdatedPixels of this tile (summed by k_updated)
+    uint32_t uu = my_updated;
     for (int off = 32; off > 0; off >>= 1) uu += __shfl_down(uu, off);
-    if (lane == 0 && uu) atomicAdd(&a.ctr->updated, uu);
-    if (tid == 0) atomicAdd(&a.ctr->total, (unsigned long long)(f1 - f0));
+    if (lane == 0) upd[tid >> 6] = uu;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t s = 0;
+        for (int w = 0; w < TILE_THREADS / 64; w++) s += upd[w];
+        a.tupd[t] = s;
+    }
+}
+
+// UpdatedPixels: the sum of the per-tile counts (one workgroup; a
+// device-wide atomic per tile would serialise on one address).
+__global__ __launch_bounds__(1024) void k_updated(DrawArgs a) {
+    __shared__ unsigned long long wsum[1024 / 64];
+    if (a.ctr->overflow) return;
+    unsigned long long v = 0;
+    for (int t = threadIdx.x; t < a.ntiles; t += 1024) v += a.tupd[t];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long s = 0;
+        for (int w = 0; w < 1024 / 64; w++) s += wsum[w];
+        a.ctr->updated = s;
+    }
 }
 
 // ---------------------------------------------------------------------------
 
 static unsigned span_grid(const DrawArgs &a) {
-    return (unsigned)(((uint64_t)a.span_cap + SPAN_THREADS - 1) / SPAN_THREADS);
+    const uint64_t nb = ((uint64_t)a.span_cap + SPAN_THREADS - 1) / SPAN_THREADS;
+    return (unsigned)(nb < SPAN_GRID_MAX ? (nb ? nb : 1) : SPAN_GRID_MAX);
 }
 
-hipError_t launch_frag_count(const DrawArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_frag_count, dim3(span_grid(a)), dim3(SPAN_THREADS), 0, s, a);
-    return hipGetLastError();
-}
 hipError_t launch_frag_scan(const DrawArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_frag_scan, dim3(1), dim3(1024), 0, s, a);
+    hipLaunchKernelGGL(k_frag_scan, dim3(1), dim3(SCAN_THREADS), 0, s, a);
     return hipGetLastError();
 }
 hipError_t launch_shade(const DrawArgs &a, hipStream_t s) {
@@ -535,8 +578,16 @@ hipError_t launch_shade(const DrawArgs &a, hipStream_t s) {
     }
     return hipGetLastError();
 }
+hipError_t launch_tile_frags(const DrawArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_tile_frags, dim3((a.ntiles + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
 hipError_t launch_tile(const DrawArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_rop, dim3(a.ntiles), dim3(TILE_THREADS), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_updated(const DrawArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_updated, dim3(1), dim3(1024), 0, s, a);
     return hipGetLastError();
 }
 

@@ -171,22 +171,23 @@ FGL_D V4 intersect_segment(int p, V4 v0, V4 v1) {                               
     return wadd(v0, wmuls(u, n / d));
 }
 
-// sutherlandHodgman (clipping.go:28-52); a triangle yields at most 9 points.
-__device__ __noinline__ int sutherland_hodgman(const V4 *pts, V4 *out) {
-    V4 buf[2][10];
+// sutherlandHodgman (clipping.go:28-52); a triangle yields at most 9
+// points.  buf[0][0..2] holds the triangle on entry; the polygon is left in
+// buf[*which] and its point count returned.  buf is the caller's (LDS in
+// the clipping kernel).
+__device__ __noinline__ int sutherland_hodgman(V4 (*buf)[10], int *which) {
     int n = 3;
-    for (int i = 0; i < 3; i++) buf[0][i] = pts[i];
     int cur = 0;
     for (int p = 0; p < 6; p++) {
-        if (n == 0) return 0;
+        if (n == 0) break;
         const V4 *input = buf[cur];
         V4 *output = buf[cur ^ 1];
         int m = 0;
         V4 s = input[n - 1];
+        bool sf = point_in_front(p, s);
         for (int k = 0; k < n; k++) {
             V4 e = input[k];
             bool ef = point_in_front(p, e);
-            bool sf = point_in_front(p, s);
             if (ef) {
                 if (!sf && m < 10) output[m++] = intersect_segment(p, s, e);
                 if (m < 10) output[m++] = e;
@@ -194,11 +195,12 @@ __device__ __noinline__ int sutherland_hodgman(const V4 *pts, V4 *out) {
                 if (m < 10) output[m++] = intersect_segment(p, s, e);
             }
             s = e;
+            sf = ef;
         }
         n = m;
         cur ^= 1;
     }
-    for (int i = 0; i < n; i++) out[i] = buf[cur][i];
+    *which = cur;
     return n;
 }
 
@@ -251,18 +253,13 @@ FGL_D void fix_normals(Vtx *V) {
         if (V[k].nrm.x == 0 && V[k].nrm.y == 0 && V[k].nrm.z == 0) V[k].nrm = n;
 }
 
-// ClipTriangle + per-fan drawClippedTriangle.  With em.base == nullptr this
-// only counts; otherwise it also writes the fan vertices at side_vtx[vbase..].
-__device__ __noinline__ void clip_fans(const DrawArgs &a, const Vtx *V, Emitter &em, uint32_t vbase, bool write,
-                                       uint32_t *nfan_out) {
-    V4 pts[3] = {V[0].out, V[1].out, V[2].out};
-    V4 poly[10];
-    int np = sutherland_hodgman(pts, poly);
+// ClipTriangle + per-fan drawClippedTriangle: the fan triangles are built
+// (vertex attributes written at side_vtx[vbase..]) and emitted into em.
+__device__ __noinline__ void clip_fans(const DrawArgs &a, const Vtx *V, const V4 *poly, int np, Emitter &em,
+                                       uint32_t vbase) {
     V3 p1 = v3(V[0].out.x, V[0].out.y, V[0].out.z);
     V3 p2 = v3(V[1].out.x, V[1].out.y, V[1].out.z);
     V3 p3 = v3(V[2].out.x, V[2].out.y, V[2].out.z);
-    uint32_t nfan = np >= 3 ? (uint32_t)(np - 2) : 0u;
-    if (nfan_out) *nfan_out = nfan;
     for (int i = 2; i < np; i++) {
         V4 b1 = barycentric(p1, p2, p3, v3(poly[0].x, poly[0].y, poly[0].z));
         V4 b2 = barycentric(p1, p2, p3, v3(poly[i - 1].x, poly[i - 1].y, poly[i - 1].z));
@@ -273,14 +270,12 @@ __device__ __noinline__ void clip_fans(const DrawArgs &a, const Vtx *V, Emitter 
         F[2] = interpolate_clip(V, b3);
         fix_normals(F);  // NewTriangle
         uint64_t vb = (uint64_t)vbase + 3u * (uint32_t)(i - 2);
-        if (write) {
-            for (int k = 0; k < 3; k++) {
-                double *d = a.side_vtx + (vb + k) * SIDE_VTX_DOUBLES;
-                d[0] = F[k].pos.x; d[1] = F[k].pos.y; d[2] = F[k].pos.z;
-                d[3] = F[k].nrm.x; d[4] = F[k].nrm.y; d[5] = F[k].nrm.z;
-                d[6] = F[k].u; d[7] = F[k].v;
-                d[8] = F[k].col.r; d[9] = F[k].col.g; d[10] = F[k].col.b; d[11] = F[k].col.a;
-            }
+        for (int k = 0; k < 3; k++) {
+            double *d = a.side_vtx + (vb + k) * SIDE_VTX_DOUBLES;
+            d[0] = F[k].pos.x; d[1] = F[k].pos.y; d[2] = F[k].pos.z;
+            d[3] = F[k].nrm.x; d[4] = F[k].nrm.y; d[5] = F[k].nrm.z;
+            d[6] = F[k].u; d[7] = F[k].v;
+            d[8] = F[k].col.r; d[9] = F[k].col.g; d[10] = F[k].col.b; d[11] = F[k].col.a;
         }
         const uint32_t sel[3] = {0, 1, 2};
         emit_clipped_triangle(a, em, ATTR_SIDE, vb, sel, F[0].out, F[1].out, F[2].out);
@@ -317,8 +312,17 @@ FGL_D bool alloc_side(const DrawArgs &a, uint32_t nrec, uint32_t nvtx, uint32_t 
 
 // The clipped branch of DrawTriangle (context.go:376-384):
 // ClipTriangle(NewTriangle(v1, v2, v3)) and each fan triangle in order.
+// Side records are reserved for every fan triangle (six per fan in
+// wireframe) and filled from the first; nr counts the ones written.
 __device__ __noinline__ void vs_triangle_clip(const DrawArgs &a, int64_t i, V4 o0, V4 o1, V4 o2, uint32_t &nr,
-                                              uint32_t &nt, uint32_t &ss) {
+                                              uint32_t &nt, uint32_t &ss, V4 (*buf)[10]) {
+    buf[0][0] = o0;
+    buf[0][1] = o1;
+    buf[0][2] = o2;
+    int which = 0;
+    const int np = sutherland_hodgman(buf, &which);
+    if (np < 3) return;
+    const uint32_t nfan = (uint32_t)(np - 2);
     Vtx V[3];
     V[0] = load_vertex(a.in, 3 * i);
     V[1] = load_vertex(a.in, 3 * i + 1);
@@ -327,14 +331,11 @@ __device__ __noinline__ void vs_triangle_clip(const DrawArgs &a, int64_t i, V4 o
     V[1].out = o1;
     V[2].out = o2;
     fix_normals(V);
-    Emitter cnt{nullptr, 0, 0};
-    uint32_t nfan = 0;
-    clip_fans(a, V, cnt, 0, false, &nfan);
-    if (cnt.n == 0) return;
     uint32_t rb, vb;
-    if (!alloc_side(a, cnt.n, 3 * nfan, rb, vb)) return;
+    if (!alloc_side(a, nfan * (a.st.wireframe ? 6u : 1u), 3 * nfan, rb, vb)) return;
     Emitter em{a.side_rec + rb, 0, 0};
-    clip_fans(a, V, em, vb, true, nullptr);
+    clip_fans(a, V, buf[which], np, em, vb);
+    if (em.n == 0) return;
     nr = em.n | REF_SIDE;
     nt = em.ntiles;
     ss = rb;
@@ -367,7 +368,8 @@ FGL_D void vs_triangle(const DrawArgs &a, int64_t i, uint32_t &nr, uint32_t &nt,
         ss = rb;
         return;
     }
-    vs_triangle_clip(a, i, o0, o1, o2, nr, nt, ss);
+    V4 buf[2][10];
+    vs_triangle_clip(a, i, o0, o1, o2, nr, nt, ss, buf);
 }
 
 // Filled triangles without clipping in one lean pass; triangles with a
@@ -375,8 +377,10 @@ FGL_D void vs_triangle(const DrawArgs &a, int64_t i, uint32_t &nr, uint32_t &nt,
 // in their counts afterwards.
 __global__ __launch_bounds__(VS_THREADS) void k_vs_fast(DrawArgs a) {
     __shared__ uint64_t wsum[VS_THREADS / 64];
+    const int lane = threadIdx.x & 63;
     int64_t i = (int64_t)blockIdx.x * VS_THREADS + threadIdx.x;
     uint32_t nr = 0, nt = 0;
+    bool defer = false;
     if (i < a.in.n) {
         V3 P0 = load_pos(a.in, 3 * i), P1 = load_pos(a.in, 3 * i + 1), P2 = load_pos(a.in, 3 * i + 2);
         V4 o0 = mul_position_w(a.sh.M, P0);
@@ -394,10 +398,7 @@ __global__ __launch_bounds__(VS_THREADS) void k_vs_fast(DrawArgs a) {
                 rejected = !f0 && !f1 && !f2;
                 break;
             }
-            if (!rejected) {
-                uint32_t slot = atomicAdd(&a.ctr->ndeferred, 1u);
-                a.deferred[slot] = (uint32_t)i;
-            }
+            defer = !rejected;
         } else {
             const uint32_t ident[3] = {0, 1, 2};
             Emitter em{a.rec_main + i, 0, 0};
@@ -409,9 +410,18 @@ __global__ __launch_bounds__(VS_THREADS) void k_vs_fast(DrawArgs a) {
         a.ntl[i] = nt;
         a.sstart[i] = 0;
     }
+    // queue the deferred triangles: one counter update per wave
+    const uint64_t m = __ballot(defer);
+    if (m) {
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&a.ctr->ndeferred, (uint32_t)__popcll(m));
+        base = __shfl(base, leader);
+        if (defer) a.deferred[base + __popcll(m & ((1ull << lane) - 1))] = (uint32_t)i;
+    }
     uint64_t v = nt;
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
-    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+    if (lane == 0) wsum[threadIdx.x >> 6] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         uint64_t s = 0;
@@ -420,20 +430,83 @@ __global__ __launch_bounds__(VS_THREADS) void k_vs_fast(DrawArgs a) {
     }
 }
 
+// Exclusive position of this lane's `want` items in a block of the counter
+// reserved for the whole wave (one atomic per wave).  All 64 lanes call it.
+FGL_D uint32_t wave_reserve(uint32_t *counter, uint32_t want) {
+    const int lane = threadIdx.x & 63;
+    uint32_t incl = want;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(incl, off);
+        if (lane >= off) incl += y;
+    }
+    const uint32_t total = __shfl(incl, 63);
+    uint32_t base = 0;
+    if (lane == 63 && total) base = atomicAdd(counter, total);
+    base = __shfl(base, 63);
+    return base + incl - want;
+}
+
+// The triangles k_vs_fast queued, one per lane: clip (polygon in LDS),
+// reserve the side records and vertices of the whole wave at once, then
+// build and emit the fans.
 __global__ __launch_bounds__(64) void k_vs_deferred(DrawArgs a) {
+    __shared__ V4 clipbuf[64][2][10];
     const uint32_t n = a.ctr->ndeferred;
-    for (uint32_t q = blockIdx.x * 64 + threadIdx.x; q < n; q += gridDim.x * 64) {
-        int64_t i = a.deferred[q];
-        V3 P0 = load_pos(a.in, 3 * i), P1 = load_pos(a.in, 3 * i + 1), P2 = load_pos(a.in, 3 * i + 2);
-        V4 o0 = mul_position_w(a.sh.M, P0);
-        V4 o1 = mul_position_w(a.sh.M, P1);
-        V4 o2 = mul_position_w(a.sh.M, P2);
+    V4(*buf)[10] = clipbuf[threadIdx.x];
+    for (uint32_t q0 = blockIdx.x * 64; q0 < n; q0 += gridDim.x * 64) {
+        const uint32_t q = q0 + threadIdx.x;
+        const bool act = q < n;
+        const int64_t i = act ? (int64_t)a.deferred[q] : 0;
+        int np = 0, which = 0;
+        V4 o0, o1, o2;
+        if (act) {
+            V3 P0 = load_pos(a.in, 3 * i), P1 = load_pos(a.in, 3 * i + 1), P2 = load_pos(a.in, 3 * i + 2);
+            o0 = mul_position_w(a.sh.M, P0);
+            o1 = mul_position_w(a.sh.M, P1);
+            o2 = mul_position_w(a.sh.M, P2);
+            buf[0][0] = o0;
+            buf[0][1] = o1;
+            buf[0][2] = o2;
+            np = sutherland_hodgman(buf, &which);
+        }
+        const uint32_t nfan = np >= 3 ? (uint32_t)(np - 2) : 0u;
+        const uint32_t rb = wave_reserve(&a.ctr->side_rec, nfan);
+        const uint32_t vb = wave_reserve(&a.ctr->side_vtx, 3 * nfan);
         uint32_t nr = 0, nt = 0, ss = 0;
-        vs_triangle_clip(a, i, o0, o1, o2, nr, nt, ss);
-        a.nrast[i] = nr;
-        a.ntl[i] = nt;
-        a.sstart[i] = ss;
-        if (nt) atomicAdd((unsigned long long *)&a.blocksum[i / VS_THREADS], (unsigned long long)nt);
+        if (nfan) {
+            bool ok = true;
+            if ((uint64_t)rb + nfan > a.side_rec_cap) {
+                atomicOr(&a.ctr->overflow, 1u);
+                ok = false;
+            }
+            if ((uint64_t)vb + 3 * nfan > a.side_vtx_cap) {
+                atomicOr(&a.ctr->overflow, 2u);
+                ok = false;
+            }
+            if (ok) {
+                Vtx V[3];
+                V[0] = load_vertex(a.in, 3 * i);
+                V[1] = load_vertex(a.in, 3 * i + 1);
+                V[2] = load_vertex(a.in, 3 * i + 2);
+                V[0].out = o0;
+                V[1].out = o1;
+                V[2].out = o2;
+                fix_normals(V);  // NewTriangle (context.go:382)
+                Emitter em{a.side_rec + rb, 0, 0};
+                clip_fans(a, V, buf[which], np, em, vb);
+                if (em.n) {
+                    nr = em.n | REF_SIDE;
+                    nt = em.ntiles;
+                    ss = rb;
+                }
+            }
+        }
+        if (act) {
+            a.nrast[i] = nr;
+            a.ntl[i] = nt;
+            a.sstart[i] = ss;
+            if (nt) atomicAdd((unsigned long long *)&a.blocksum[i / VS_THREADS], (unsigned long long)nt);
+        }
     }
 }
 

@@ -1,9 +1,9 @@
 /*
  * fgl.h -- C ABI of the MI355X (gfx950) draw path behind fauxgl's Go API.
  *
- * A thin cgo shim in package fauxgl binds these entry points (the binding
- * is shown in INTEGRATION.md).  Every entry point names the reference
- * function it replaces (file:line in fogleman/fauxgl).
+ * A thin cgo shim in package fauxgl binds these entry points.  Every entry
+ * point names the reference function it replaces (file:line in
+ * fogleman/fauxgl).
  *
  * Conventions
  *   - Plain C types only.  Host pointers unless the name says "device".
@@ -198,6 +198,19 @@ int fgl_profile_read(fgl_ctx *ctx, fgl_kernel_time *out, int32_t cap, int32_t *n
 int fgl_profile_reset(fgl_ctx *ctx);
 /* Waits for all work queued on ctx. */
 int fgl_ctx_sync(fgl_ctx *ctx);
+
+/* Work counts of the last completed draw on ctx (diagnostics; no reference
+ * counterpart beyond RasterizeInfo's two totals).  clipped_triangles counts
+ * input triangles that crossed a clip plane, side_triangles the raster
+ * triangles written by clipping, wireframe and line expansion, pairs the
+ * (tile, triangle) bin entries, spans the per-row coverage runs and
+ * fragments the covered in-bounds pixels shaded. */
+typedef struct fgl_draw_stat {
+    uint64_t total_pixels, updated_pixels;
+    uint64_t clipped_triangles, side_triangles;
+    uint64_t pairs, spans, fragments;
+} fgl_draw_stat;
+int fgl_draw_stats(fgl_ctx *ctx, fgl_draw_stat *out);
 
 #ifdef __cplusplus
 }
