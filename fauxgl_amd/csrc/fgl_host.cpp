@@ -92,6 +92,8 @@ struct fgl_ctx {
     DevBuf in_pos, in_nrm, in_uv, in_col;
     DevBuf zeros;
     DevBuf rs_tmp, rs_wx, rs_ox, rs_wy, rs_oy;
+    int32_t rs_key[4] = {0, 0, 0, 0};  // source and destination sizes the tent tables are for
+    hipEvent_t xev = nullptr;          // orders another context's stream after this one
     DevBuf dpart, dimg;
     uint32_t side_rec_cap = 0, side_vtx_cap = 0, pair_cap = 0;
     DrawCounters last{};
@@ -109,6 +111,7 @@ struct fgl_ctx {
             (void)hipEventDestroy(p.b);
         }
         for (auto e : pool) (void)hipEventDestroy(e);
+        if (xev) (void)hipEventDestroy(xev);
     }
 };
 
@@ -138,24 +141,30 @@ hipEvent_t ev_get(fgl_ctx *c) {
         c->pool.pop_back();
         return e;
     }
+    // Timing only: without the system-scope release/acquire an event record
+    // does not write back and invalidate the caches between kernels.
     hipEvent_t e = nullptr;
-    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
     return e;
 }
 
-// Collects the durations of finished profiled launches.
+// Collects the durations of finished profiled launches (in launch order;
+// stops at the first one still running).
 void prof_collect(fgl_ctx *c) {
-    for (auto &p : c->pending) {
+    size_t k = 0;
+    for (; k < c->pending.size(); k++) {
+        EvPair &p = c->pending[k];
+        if (hipEventQuery(p.b) == hipErrorNotReady) break;
         float ms = 0;
         if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
-            KStat &k = c->kstats[p.name];
-            k.launches++;
-            k.ms += ms;
+            KStat &st = c->kstats[p.name];
+            st.launches++;
+            st.ms += ms;
         }
         c->pool.push_back(p.a);
         c->pool.push_back(p.b);
     }
-    c->pending.clear();
+    c->pending.erase(c->pending.begin(), c->pending.begin() + (long)k);
 }
 
 // Runs one launch, bracketed by events when profiling.
@@ -216,6 +225,15 @@ void screen_matrix(int32_t w, int32_t h, double *S) {
     double h2 = (double)h / 2;
     const double m[16] = {w2, 0, 0, w2, 0, -h2, 0, h2, 0, 0, 0.5, 0.5, 0, 0, 0, 1};
     std::memcpy(S, m, sizeof m);
+}
+
+// Makes waiter's stream wait for the work queued so far on signaller's.
+int stream_after(fgl_ctx *waiter, fgl_ctx *signaller) {
+    if (waiter->stream == signaller->stream) return FGL_OK;
+    if (!signaller->xev) HIPCHK(signaller, hipEventCreateWithFlags(&signaller->xev, hipEventDisableTiming));
+    HIPCHK(signaller, hipEventRecord(signaller->xev, signaller->stream));
+    HIPCHK(waiter, hipStreamWaitEvent(waiter->stream, signaller->xev, 0));
+    return FGL_OK;
 }
 
 int materialize(fgl_ctx *c) {
@@ -903,21 +921,31 @@ int fgl_resolve(fgl_ctx *src, fgl_ctx *dst, int32_t filter) {
     HIPCHK(src, hipSetDevice(src->device));
     int r = materialize(src);
     if (r) return r;
-    HIPCHK(dst, hipStreamSynchronize(dst->stream));
+    // dst's earlier work finishes before the resolve overwrites its color
+    if ((r = stream_after(src, dst))) return r;
     int32_t flx = 0, fly = 0;
     if (filter == FGL_RESOLVE_TENT) {
-        std::vector<int16_t> wx, wy;
-        std::vector<int32_t> ox, oy;
-        tent_weights(src->W, dst->W, wx, ox, flx);
-        tent_weights(src->H, dst->H, wy, oy, fly);
-        HIPCHK(src, src->rs_wx.ensure(wx.size() * 2));
-        HIPCHK(src, src->rs_ox.ensure(ox.size() * 4));
-        HIPCHK(src, src->rs_wy.ensure(wy.size() * 2));
-        HIPCHK(src, src->rs_oy.ensure(oy.size() * 4));
-        HIPCHK(src, hipMemcpyAsync(src->rs_wx.p, wx.data(), wx.size() * 2, hipMemcpyHostToDevice, src->stream));
-        HIPCHK(src, hipMemcpyAsync(src->rs_ox.p, ox.data(), ox.size() * 4, hipMemcpyHostToDevice, src->stream));
-        HIPCHK(src, hipMemcpyAsync(src->rs_wy.p, wy.data(), wy.size() * 2, hipMemcpyHostToDevice, src->stream));
-        HIPCHK(src, hipMemcpyAsync(src->rs_oy.p, oy.data(), oy.size() * 4, hipMemcpyHostToDevice, src->stream));
+        const size_t fx = 2 * (size_t)(src->W / dst->W), fy = 2 * (size_t)(src->H / dst->H);
+        flx = (int32_t)fx;
+        fly = (int32_t)fy;
+        const int32_t key[4] = {src->W, src->H, dst->W, dst->H};
+        if (std::memcmp(key, src->rs_key, sizeof key) != 0) {
+            std::vector<int16_t> wx, wy;
+            std::vector<int32_t> ox, oy;
+            tent_weights(src->W, dst->W, wx, ox, flx);
+            tent_weights(src->H, dst->H, wy, oy, fly);
+            HIPCHK(src, src->rs_wx.ensure(wx.size() * 2));
+            HIPCHK(src, src->rs_ox.ensure(ox.size() * 4));
+            HIPCHK(src, src->rs_wy.ensure(wy.size() * 2));
+            HIPCHK(src, src->rs_oy.ensure(oy.size() * 4));
+            HIPCHK(src, hipMemcpyAsync(src->rs_wx.p, wx.data(), wx.size() * 2, hipMemcpyHostToDevice, src->stream));
+            HIPCHK(src, hipMemcpyAsync(src->rs_ox.p, ox.data(), ox.size() * 4, hipMemcpyHostToDevice, src->stream));
+            HIPCHK(src, hipMemcpyAsync(src->rs_wy.p, wy.data(), wy.size() * 2, hipMemcpyHostToDevice, src->stream));
+            HIPCHK(src, hipMemcpyAsync(src->rs_oy.p, oy.data(), oy.size() * 4, hipMemcpyHostToDevice, src->stream));
+            // the host vectors go out of scope: finish the copies first
+            HIPCHK(src, hipStreamSynchronize(src->stream));
+            std::memcpy(src->rs_key, key, sizeof key);
+        }
         HIPCHK(src, src->rs_tmp.ensure((size_t)dst->W * src->H * 4));
     }
     r = launch(src, "resolve", [&] {
@@ -927,7 +955,8 @@ int fgl_resolve(fgl_ctx *src, fgl_ctx *dst, int32_t filter) {
                               src->stream);
     });
     if (r) return r;
-    HIPCHK(src, hipStreamSynchronize(src->stream));
+    // later work on dst (reads, draws) runs after the resolve
+    if ((r = stream_after(dst, src))) return r;
     if (src->prof) prof_collect(src);
     dst->pend_color = false;
     return FGL_OK;

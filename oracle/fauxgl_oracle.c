@@ -1152,3 +1152,85 @@ void orc_rasterize_screen(orc_ctx *c, const double s0[3], const double s1[3], co
     info[0] = r.total;
     info[1] = r.updated;
 }
+
+/* ---------------------------------------------------------------------
+ * SSAA resolve (filter defined by this build; see the header).
+ */
+
+static int32_t tent_table(int32_t src, int32_t dst, int16_t *w, int32_t *start) {
+    const double s = (double)src / (double)dst;
+    const double factor = 1.0 / s < 1.0 ? 1.0 / s : 1.0;
+    const int32_t half = (int32_t)ceil(s) < 1 ? 1 : (int32_t)ceil(s);
+    const int32_t taps = 2 * half;
+    if (!w) return taps;
+    for (int32_t o = 0; o < dst; o++) {
+        const double centre = s * ((double)o + 0.5) - 0.5;
+        const int32_t st = (int32_t)centre - half + (ORC_MUTANT == 12 ? 0 : 1);
+        start[o] = st;
+        for (int32_t t = 0; t < taps; t++) {
+            double x = (centre - (double)st - (double)t) * factor;
+            if (x < 0) x = -x;
+            const double k = x <= 1 ? 1 - x : 0;
+            w[(size_t)o * taps + t] = (int16_t)(k * 256);
+        }
+    }
+    return taps;
+}
+
+int32_t orc_tent_weights(int32_t src, int32_t dst, int16_t *w, int32_t *start) {
+    return tent_table(src, dst, w, start);
+}
+
+static int32_t clampi(int32_t v, int32_t lo, int32_t hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+void orc_resolve(const uint8_t *src, int32_t sw, int32_t sh, uint8_t *dst, int32_t dw, int32_t dh, int32_t filter) {
+    if (filter == 2) {
+        const int32_t f = sw / dw, n = f * f;
+        for (int32_t y = 0; y < dh; y++)
+            for (int32_t x = 0; x < dw; x++)
+                for (int ch = 0; ch < 4; ch++) {
+                    uint32_t acc = 0;
+                    for (int32_t j = 0; j < f; j++)
+                        for (int32_t i = 0; i < f; i++) acc += src[((size_t)(y * f + j) * sw + (size_t)(x * f + i)) * 4 + ch];
+                    dst[((size_t)y * dw + x) * 4 + ch] = (uint8_t)((acc + (ORC_MUTANT == 14 ? 0u : (uint32_t)n / 2)) / (uint32_t)n);
+                }
+        return;
+    }
+    const int32_t tx = tent_table(sw, dw, NULL, NULL), ty = tent_table(sh, dh, NULL, NULL);
+    int16_t *wx = malloc(sizeof(int16_t) * (size_t)dw * tx), *wy = malloc(sizeof(int16_t) * (size_t)dh * ty);
+    int32_t *ox = malloc(sizeof(int32_t) * (size_t)dw), *oy = malloc(sizeof(int32_t) * (size_t)dh);
+    uint8_t *mid = malloc((size_t)dw * sh * 4);  /* horizontal pass, premultiplied */
+    tent_table(sw, dw, wx, ox);
+    tent_table(sh, dh, wy, oy);
+    for (int32_t y = 0; y < sh; y++)
+        for (int32_t x = 0; x < dw; x++) {
+            int32_t acc[4] = {0, 0, 0, 0}, wsum = 0;
+            for (int32_t t = 0; t < tx; t++) {
+                const int32_t c = wx[(size_t)x * tx + t];
+                const uint8_t *q = src + ((size_t)y * sw + clampi(ox[x] + t, 0, sw - 1)) * 4;
+                const int32_t a = q[3];
+                for (int ch = 0; ch < 3; ch++) acc[ch] += c * (ORC_MUTANT == 13 ? (int32_t)q[ch] : (int32_t)q[ch] * a / 255);
+                acc[3] += c * a;
+                wsum += c;
+            }
+            for (int ch = 0; ch < 4; ch++)
+                mid[((size_t)y * dw + x) * 4 + ch] = (uint8_t)clampi(wsum ? acc[ch] / wsum : 0, 0, 255);
+        }
+    for (int32_t y = 0; y < dh; y++)
+        for (int32_t x = 0; x < dw; x++) {
+            int32_t acc[4] = {0, 0, 0, 0}, wsum = 0;
+            for (int32_t t = 0; t < ty; t++) {
+                const int32_t c = wy[(size_t)y * ty + t];
+                const uint8_t *q = mid + ((size_t)clampi(oy[y] + t, 0, sh - 1) * dw + x) * 4;
+                for (int ch = 0; ch < 4; ch++) acc[ch] += c * (int32_t)q[ch];
+                wsum += c;
+            }
+            for (int ch = 0; ch < 4; ch++)
+                dst[((size_t)y * dw + x) * 4 + ch] = (uint8_t)clampi(wsum ? acc[ch] / wsum : 0, 0, 255);
+        }
+    free(wx);
+    free(wy);
+    free(ox);
+    free(oy);
+    free(mid);
+}

@@ -143,6 +143,20 @@ void orc_phong_fragment(const orc_shader *s, const double pos[3], const double n
                         const double uv[2], const double col[4], double out[4]);
 /* Rasterize one screen-space triangle with constant attributes through
  * rasterize() (context.go:151-281); returns TotalPixels/UpdatedPixels. */
+/* SSAA resolve (the step after the path; the reference pipeline calls
+ * nfnt/resize's Bilinear there, README.md:109, which is not vendored, so
+ * this is the build's own filter definition, restated independently of
+ * the GPU code).  src is sw x sh NRGBA, dst dw x dh NRGBA, sw = f*dw and
+ * sh = f*dh.  filter 1 = tent, 2 = box.  Tent: separable; per output
+ * coordinate o of an axis of scale s = src/dst, taps t = 0..2*ceil(s)-1 at
+ * source index start+t with start = int(s*(o+0.5)-0.5) - ceil(s) + 1,
+ * weight int16(max(0, 1 - |(s*(o+0.5)-0.5 - start - t) * min(1/s, 1)|) * 256),
+ * indices clamped to [0, n-1]; the horizontal pass premultiplies
+ * (c*a/255, integer) and both passes divide the weighted sum by the weight
+ * sum with truncation.  Box: per channel round-half-up mean of the f x f
+ * block.  orc_tent_weights exposes one axis' table. */
+void orc_resolve(const uint8_t *src, int32_t sw, int32_t sh, uint8_t *dst, int32_t dw, int32_t dh, int32_t filter);
+int32_t orc_tent_weights(int32_t src, int32_t dst, int16_t *w, int32_t *start);
 void orc_rasterize_screen(orc_ctx *c, const double s0[3], const double s1[3],
                           const double s2[3], const double w[3], uint64_t info[2]);
 

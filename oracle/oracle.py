@@ -72,6 +72,10 @@ def lib():
         "orc_color": (_P(ctypes.c_uint8), [vp]),
         "orc_depth": (dp, [vp]),
         "orc_depth_image": (None, [vp, _P(ctypes.c_uint16)]),
+        "orc_resolve": (None, [_P(ctypes.c_uint8), ctypes.c_int32, ctypes.c_int32, _P(ctypes.c_uint8), ctypes.c_int32,
+                               ctypes.c_int32, ctypes.c_int32]),
+        "orc_tent_weights": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int32, _P(ctypes.c_int16),
+                                              _P(ctypes.c_int32)]),
         "orc_counters": (None, [vp, u64p]),
         "orc_reset_counters": (None, [vp]),
         "orc_mesh_new": (vp, [ctypes.c_int64, dp, dp, dp, dp, ctypes.c_int64, dp, dp, dp, dp]),
@@ -373,3 +377,22 @@ def phong_fragment(shader, pos, nrm, uv=(0, 0), col=(0, 0, 0, 0)):
     lib().orc_phong_fragment(ctypes.byref(shader), _dptr(darr(pos, 3)), _dptr(darr(nrm, 3)), _dptr(darr(uv, 2)),
                              _dptr(darr(col, 4)), _dptr(out))
     return out
+
+
+def resolve(src, dw, dh, filter=1):
+    """src: (sh, sw, 4) uint8 NRGBA -> (dh, dw, 4) uint8 (filter 1 tent, 2 box)."""
+    src = np.ascontiguousarray(src, dtype=np.uint8)
+    sh, sw = src.shape[:2]
+    out = np.zeros((dh, dw, 4), dtype=np.uint8)
+    lib().orc_resolve(src.ctypes.data_as(_P(ctypes.c_uint8)), sw, sh, out.ctypes.data_as(_P(ctypes.c_uint8)), dw, dh,
+                      filter)
+    return out
+
+
+def tent_weights(src, dst):
+    """(weights (dst, taps) int16, start (dst,) int32) of one axis."""
+    taps = lib().orc_tent_weights(src, dst, None, None)
+    w = np.zeros((dst, taps), dtype=np.int16)
+    st = np.zeros(dst, dtype=np.int32)
+    lib().orc_tent_weights(src, dst, w.ctypes.data_as(_P(ctypes.c_int16)), st.ctypes.data_as(_P(ctypes.c_int32)))
+    return w, st

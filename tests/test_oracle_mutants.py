@@ -26,6 +26,9 @@ MUTANTS = {
     9: ("blend drops the 0x101 factor", "blend"),
     10: ("drawClippedTriangle never swaps v0/v2", "windings"),
     11: ("the near clip plane tests against P instead of N", "clip_polygon"),
+    12: ("tent taps start one source pixel early", "resolve"),
+    13: ("tent horizontal pass does not premultiply by alpha", "resolve"),
+    14: ("box resolve truncates instead of rounding", "resolve"),
 }
 
 

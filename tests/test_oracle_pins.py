@@ -541,3 +541,79 @@ def test_threaded_mirror_matches_on_opaque_unique_depths():
         depths.append(ctx.depth())
     assert infos[0][0] == infos[1][0]
     assert np.array_equal(depths[0], depths[1])
+
+
+# ---------------------------------------------------------------------------
+# SSAA resolve (the build's own filter; pinned by its closed forms)
+
+def _tent_exact(src, dst):
+    """The tent table in exact rationals: centre c = s*(o+1/2) - 1/2,
+    start = trunc(c) - ceil(s) + 1, weight floor(max(0, 1 - |c - start - t|/s) * 256)."""
+    s = F(src, dst)
+    half = max(math.ceil(s), 1)
+    w = np.zeros((dst, 2 * half), dtype=np.int64)
+    st = np.zeros(dst, dtype=np.int64)
+    for o in range(dst):
+        c = s * (o + F(1, 2)) - F(1, 2)
+        st[o] = int(c) - half + 1  # int() truncates toward zero like Go/C
+        for t in range(2 * half):
+            x = abs(c - st[o] - t) / s
+            w[o, t] = math.floor(max(F(0), 1 - x) * 256)
+    return w, st
+
+
+@pytest.mark.parametrize("f,dst", [(1, 5), (2, 7), (3, 4), (4, 6), (8, 3)])
+def test_resolve_tent_weights_closed_form(f, dst):
+    w, st = O.tent_weights(f * dst, dst)
+    we, ste = _tent_exact(f * dst, dst)
+    assert np.array_equal(st, ste)
+    assert np.array_equal(w, we)
+    if f == 4:
+        assert w[0].tolist() == [32, 96, 160, 224, 224, 160, 96, 32]
+        assert st.tolist() == [4 * o - 2 for o in range(dst)]
+
+
+@pytest.mark.parametrize("filt", [1, 2])
+@pytest.mark.parametrize("a", [255, 128, 7, 0])
+def test_resolve_constant_image(filt, a):
+    c = np.array([200, 31, 97, a], np.uint8)
+    img = np.broadcast_to(c, (12, 20, 4)).copy()
+    out = O.resolve(img, 5, 3, filt)
+    if filt == 2:
+        exp = c  # the mean of a constant
+    else:
+        # tent output is premultiplied: c * a / 255 with integer division
+        exp = np.array([200 * a // 255, 31 * a // 255, 97 * a // 255, a], np.uint8)
+    assert (out == exp).all()
+
+
+def test_resolve_box_is_the_rounded_block_mean():
+    rng = np.random.Generator(np.random.PCG64(21))
+    for f in (2, 3, 4):
+        img = rng.integers(0, 256, (3 * f, 5 * f, 4), dtype=np.uint8)
+        out = O.resolve(img, 5, 3, 2)
+        blocks = img.reshape(3, f, 5, f, 4).astype(np.int64).sum(axis=(1, 3))
+        exp = (blocks + (f * f) // 2) // (f * f)
+        assert np.array_equal(out, exp.astype(np.uint8))
+
+
+def test_resolve_tent_separable_hand_case():
+    # opaque image varying along x only: every output row is the 1-D
+    # horizontal filter; 4x with taps [32,96,160,224,224,160,96,32] and
+    # edge clamping, computed by hand for the first output pixel:
+    # sources -2..5 clamp to 0,0,0,1,2,3,4,5: 16960 // 1024 = 16
+    f, dw, dh = 4, 3, 2
+    row = np.array([0, 10, 20, 30, 40, 50, 60, 70, 80, 90, 100, 110], np.int64)
+    img = np.zeros((dh * f, dw * f, 4), np.uint8)
+    img[..., 0] = row[None, :]
+    img[..., 3] = 255
+    out = O.resolve(img, dw, dh, 1)
+    taps = [32, 96, 160, 224, 224, 160, 96, 32]
+    first = sum(t * row[max(0, min(11, -2 + k))] for k, t in enumerate(taps)) // 1024
+    assert first == (32 * 0 + 96 * 0 + 160 * 0 + 224 * 10 + 224 * 20 + 160 * 30 + 96 * 40 + 32 * 50) // 1024 == 16
+    assert (out[:, 0, 0] == first).all()
+    mid = sum(t * row[max(0, min(11, 2 + k))] for k, t in enumerate(taps)) // 1024
+    assert (out[:, 1, 0] == mid).all() and mid == 55
+    # the same image transposed: the vertical pass gives the same numbers
+    outT = O.resolve(np.ascontiguousarray(img.transpose(1, 0, 2)), dh, dw, 1)
+    assert np.array_equal(outT[:, :, 0].T, out[:, :, 0])
