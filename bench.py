@@ -24,7 +24,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-FP64_ISSUE_PEAK = 39.3e12  # separate mul/add issue on MI355X (78.6 TFLOP/s counts an FMA as 2)
+FP64_PEAK_TFLOPS = 78.6     # MI355X FP64 (vector and matrix), an FMA counted as 2
+FP64_ISSUE_PEAK = 39.3e12   # the same pipe issuing separate mul / add (no contraction in this path)
+PHONG_OPS_PER_FRAGMENT = 122  # FP64 ops per shaded Phong fragment: interpolants, normalize, pow(.,32)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_16x.json")
 
 
 def log(*a):
@@ -70,9 +73,10 @@ def gpu_frames(mesh, W, H, ssaa, steps, warmup):
     prof = big.ProfileRead()
     big.ProfileEnable(False)
     ms = (t1 - t0) * 1000.0 / steps
+    stats = big.DrawStats()
     img = (out if out is not None else big).Image()
     dm.close()
-    return ms, info, prof, img
+    return ms, info, prof, img, stats
 
 
 def cpu_frames(mesh, W, H, ssaa, frames, threads):
@@ -98,14 +102,55 @@ def cpu_frames(mesh, W, H, ssaa, frames, threads):
     return float(np.median(times)), info
 
 
-def algorithmic(W, H, ssaa, ntri, counters=None):
-    """Bytes (and FP64 ops) one frame must move at minimum: 48 B of Phong
-    vertex payload (position + normal) per triangle vertex read once, and
-    12 B per framebuffer pixel (NRGBA8 + float64 depth) written once, the
-    clear being fused into the tile writes."""
+def algorithmic(W, H, ssaa, ntri):
+    """Bytes one frame must move at minimum: 48 B of Phong vertex payload
+    (position + normal) per triangle vertex read once, and 12 B per
+    framebuffer pixel (NRGBA8 + float64 depth) written once, the clear being
+    fused into the tile writes."""
     s = int(round(ssaa ** 0.5))
     npix = W * s * H * s
     return 3 * ntri * 48 + 12 * npix
+
+
+def kernel_roofline(name, avg_ms, stats, W, H, ssaa):
+    """Roofline of one kernel from its algorithmic work per launch."""
+    s = int(round(ssaa ** 0.5))
+    npix = W * s * H * s
+    t = avg_ms / 1000.0
+    if name == "shade":
+        flops = PHONG_OPS_PER_FRAGMENT * stats["fragments"]
+        ach = flops / t / 1e12
+        return {"bound": "mfma", "pipe": "FP64 VALU (no matrix-shaped work; FP64 peak)", "achieved": round(ach, 3),
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFLOPS, 4),
+                "work": f"{PHONG_OPS_PER_FRAGMENT} FP64 ops x {stats['fragments']} fragments = {flops} per launch"}
+    if name == "rop":
+        nbytes = 16 * stats["fragments"] + 12 * npix
+        unit_desc = f"16 B x {stats['fragments']} fragments read + 12 B x {npix} pixels written"
+    elif name == "resolve":
+        nbytes = 4 * npix + 4 * W * H
+        unit_desc = f"4 B x {npix} source pixels + 4 B x {W * H} resolved pixels"
+    elif name == "spans":
+        nbytes = 32 * stats["spans"] + 8 * stats["pairs"]
+        unit_desc = f"32 B x {stats['spans']} spans + 8 B x {stats['pairs']} pairs"
+    else:
+        return None
+    ach = nbytes / t / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "work": f"{unit_desc} = {nbytes} B per launch"}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE, each scaled by a plain copy's factor)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = d.get("kernels", {}).get(kernel)
+    if not k:
+        return None
+    return int(k["fetch_bytes"] + k["write_bytes"])
 
 
 def main():
@@ -113,7 +158,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-frames", type=int, default=3, help="timed CPU frames per resolution (0 = skip)")
+    ap.add_argument("--cpu-frames", type=int, default=5, help="timed CPU frames per resolution (0 = skip)")
+    ap.add_argument("--cpu-frames-1t", type=int, default=3, help="timed single-thread CPU frames at 1x (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = CPUs in this process's affinity mask")
     ap.add_argument("--only", choices=["16x", "1x"], default=None,
                     help="time one resolution only and print its kernel table (for profiler runs)")
@@ -129,29 +175,31 @@ def main():
 
     if args.only:
         ss = SSAA if args.only == "16x" else 1
-        ms, info, prof, _ = gpu_frames(mesh, W, H, ss, args.steps, args.warmup)
+        ms, info, prof, _, stats = gpu_frames(mesh, W, H, ss, args.steps, args.warmup)
         table = {k: {"launches": v[0], "avg_ms": round(v[1] / max(v[0], 1), 4),
                      "ms_per_frame": round(v[1] / args.steps, 4)} for k, v in sorted(prof.items())}
         print(json.dumps({"workload": args.only, "ms_per_frame": round(ms, 4), "info": list(info),
-                          "kernels": table}), flush=True)
+                          "draw_stats": stats, "kernels": table}), flush=True)
         return
 
-    ms16, info16, prof16, img16 = gpu_frames(mesh, W, H, SSAA, args.steps, args.warmup)
+    ms16, info16, prof16, img16, st16 = gpu_frames(mesh, W, H, SSAA, args.steps, args.warmup)
     log(f"[bench] 16x SSAA: {ms16:.3f} ms/frame, info {tuple(info16)}")
-    ms1, info1, prof1, img1 = gpu_frames(mesh, W, H, 1, args.steps, args.warmup)
+    ms1, info1, prof1, img1, st1 = gpu_frames(mesh, W, H, 1, args.steps, args.warmup)
     log(f"[bench] 1x: {ms1:.3f} ms/frame, info {tuple(info1)}")
 
     # dominant kernel of the headline step and its roofline
     per_kernel = {k: {"launches": v[0], "avg_ms": v[1] / max(v[0], 1), "ms_per_frame": v[1] / args.steps}
                   for k, v in prof16.items()}
     dom = max(per_kernel, key=lambda k: per_kernel[k]["ms_per_frame"])
-    bytes_tile = algorithmic(W, H, SSAA, ntri)
-    avg_s = per_kernel[dom]["avg_ms"] / 1000.0
-    achieved = bytes_tile / avg_s / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-                "kernel_avg_ms": round(per_kernel[dom]["avg_ms"], 4),
-                "algorithmic_bytes": bytes_tile}
+    roofline = kernel_roofline(dom, per_kernel[dom]["avg_ms"], st16, W, H, SSAA) or {}
+    roofline.update({"traffic": measured_traffic(dom), "kernel": dom,
+                     "kernel_avg_ms": round(per_kernel[dom]["avg_ms"], 4)})
+    # the whole frame against max(bytes / HBM peak, shading FP64 ops / issue peak)
+    bytes_frame = algorithmic(W, H, SSAA, ntri) + 4 * W * H * SSAA + 4 * W * H  # + resolve
+    t_roof = max(bytes_frame / (HBM_PEAK_GBS * 1e9), PHONG_OPS_PER_FRAGMENT * st16["fragments"] / FP64_ISSUE_PEAK)
+    roofline_frame = {"t_roof_ms": round(t_roof * 1000, 4), "t_meas_ms": round(ms16, 4),
+                      "frac": round(t_roof * 1000 / ms16, 4), "bytes": bytes_frame,
+                      "fp64_ops_shading": PHONG_OPS_PER_FRAGMENT * st16["fragments"]}
 
     cpu = None
     if args.cpu_frames > 0:
@@ -163,10 +211,16 @@ def main():
         cpu = {"value": round(1.0 / c16, 4), "unit": "frames/s", "cores": threads, "kind": "port",
                "sample": (f"median of {args.cpu_frames} frames after 1 warm-up: clear + DrawMesh of the same "
                           f"{ntri}-triangle scene at 7680x4320 by the oracle's all-core mirror of "
-                          f"DrawTriangles (wn={threads}); no resolve"),
+                          f"DrawTriangles (wn={threads} workers, i % wn, 256 striped locks); no resolve"),
                "frames_per_s_1x": round(1.0 / c1, 3), "ms_per_frame": round(c16 * 1000, 2),
-               "ms_per_frame_1x": round(c1 * 1000, 2), "cpu_model": cpu_model(),
-               "info_matches_gpu": list(cinfo16) == list(info16) and list(cinfo1) == list(info1)}
+               "ms_per_frame_1x": round(c1 * 1000, 2), "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+               "gomaxprocs_equivalent": threads,
+               # the threaded mirror's UpdatedPixels depends on thread timing; TotalPixels does not
+               "total_pixels_match_gpu": cinfo16[0] == info16[0] and cinfo1[0] == info1[0]}
+        if args.cpu_frames_1t > 0:
+            s1, _ = cpu_frames(mesh, W, H, 1, args.cpu_frames_1t, 1)
+            log(f"[bench] CPU 1x single thread: {s1 * 1000:.1f} ms/frame")
+            cpu["ms_per_frame_1x_single_thread"] = round(s1 * 1000, 2)
 
     fps16 = 1000.0 / ms16
     line = {
@@ -192,7 +246,9 @@ def main():
         "rasterize_info_16x": list(info16),
         "kernels_16x": {k: {"avg_ms": round(v["avg_ms"], 4), "ms_per_frame": round(v["ms_per_frame"], 4),
                             "launches": v["launches"]} for k, v in sorted(per_kernel.items())},
+        "draw_stats_16x": st16,
         "roofline": roofline,
+        "roofline_frame": roofline_frame,
         "cpu_baseline": cpu,
     }
     if cpu is not None:

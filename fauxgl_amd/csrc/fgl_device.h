@@ -74,6 +74,13 @@ struct DevShader {
     double light[3], cam[3];
     double obj[4], amb[4], dif[4], spc[4];
     double power;
+    // math.Pow's case analysis of y = power, made once per draw
+    // (math.Pow's Modf / fraction adjustment, shader.go:91): yi and yf
+    // after the "yf > 0.5" step, pw_frac = the fractional part is nonzero,
+    // pw_big = yi >= 2^63.
+    uint64_t pw_yi;
+    double pw_yf;
+    int32_t pw_frac, pw_big;
 };
 
 struct DevState {

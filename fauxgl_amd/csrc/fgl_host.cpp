@@ -87,7 +87,7 @@ struct fgl_ctx {
     DevBuf ctr, ntl, nrast, sstart, blocksum, blockoff, rec_main, side_rec, side_vtx;
     DevBuf keys0, vals0, keys1, vals1, hist, dtot, tstart, tend, tfrag, tupd, deferred;
     DevBuf pspans, span_bsum, span_boff, poff, spans;
-    DevBuf frag_bsum, frag_boff, foff, frags;
+    DevBuf frag_bsum, frag_boff, frag_gsum, foff, frags;
     uint32_t span_cap = 0, frag_cap = 0;
     DevBuf in_pos, in_nrm, in_uv, in_col;
     DevBuf zeros;
@@ -273,6 +273,20 @@ int fill_shader(fgl_ctx *c, const fgl_shader *sh, DevShader &d) {
     std::memcpy(d.dif, sh->diffuse, sizeof d.dif);
     std::memcpy(d.spc, sh->specular, sizeof d.spc);
     d.power = sh->specular_power;
+    {
+        // the y-only part of math.Pow(x, y), y = SpecularPower
+        const double ay = std::fabs(d.power);
+        double yi = std::trunc(ay);
+        double yf = ay - yi;
+        d.pw_frac = yf != 0;
+        d.pw_big = yi >= 9223372036854775808.0;
+        if (yf != 0 && yf > 0.5) {
+            yf = yf - 1;
+            yi = yi + 1;
+        }
+        d.pw_yf = yf;
+        d.pw_yi = (std::isfinite(yi) && yi < 9223372036854775808.0) ? (uint64_t)yi : 0;
+    }
     // ObjectColor != Discard compares all four channels with 0 (shader.go:76)
     d.obj_is_discard = (sh->object_color[0] == 0 && sh->object_color[1] == 0 && sh->object_color[2] == 0 &&
                         sh->object_color[3] == 0);
@@ -293,6 +307,9 @@ int fill_shader(fgl_ctx *c, const fgl_shader *sh, DevShader &d) {
 }
 
 constexpr int64_t kMaxPrims = (int64_t)1 << 26;
+// Side vertices (from clipping) per draw; keeps the shading kernels'
+// attribute offsets within 32 bits.
+constexpr uint64_t kMaxSideVtx = (uint64_t)1 << 28;
 
 int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, const DevPrims &in, uint64_t info[2]) {
     if (info) info[0] = info[1] = 0;
@@ -345,7 +362,7 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
     uint64_t want_vtx = kind == PRIM_LINE ? 256 : (uint64_t)n / 4 + 4096;
     uint64_t want_pairs = (uint64_t)2 * n + (uint64_t)a.ntiles + 65536;
     if (c->side_rec_cap < want_side) c->side_rec_cap = (uint32_t)std::min<uint64_t>(want_side, 0x7fffffffu);
-    if (c->side_vtx_cap < want_vtx) c->side_vtx_cap = (uint32_t)std::min<uint64_t>(want_vtx, 0x7fffffffu);
+    if (c->side_vtx_cap < want_vtx) c->side_vtx_cap = (uint32_t)std::min<uint64_t>(want_vtx, kMaxSideVtx);
     if (c->pair_cap < want_pairs) c->pair_cap = (uint32_t)std::min<uint64_t>(want_pairs, 0xfffff000u);
     uint64_t want_spans = (uint64_t)3 * c->pair_cap;
     if (c->span_cap < want_spans) c->span_cap = (uint32_t)std::min<uint64_t>(want_spans, 0xfffff000u);
@@ -394,10 +411,12 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
         const size_t nfb = ((size_t)c->span_cap + SPAN_THREADS - 1) / SPAN_THREADS + 1;
         HIPCHK(c, c->frag_bsum.ensure(nfb * 8));
         HIPCHK(c, c->frag_boff.ensure(nfb * 8));
+        HIPCHK(c, c->frag_gsum.ensure((nfb / 1024 + 2) * 8));
         HIPCHK(c, c->foff.ensure(((size_t)c->span_cap + 1) * 4));
         HIPCHK(c, c->frags.ensure((size_t)c->frag_cap * sizeof(Frag)));
         a.frag_blocksum = c->frag_bsum.as<uint64_t>();
         a.frag_blockoff = c->frag_boff.as<uint64_t>();
+        a.frag_gsum = c->frag_gsum.as<uint64_t>();
         a.foff = c->foff.as<uint32_t>();
         a.frags = c->frags.as<Frag>();
         a.frag_cap = c->frag_cap;
@@ -483,7 +502,13 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
             c->frag_cap = (uint32_t)need;
         }
         if (h.overflow & 1u) c->side_rec_cap = (uint32_t)std::min<uint64_t>((uint64_t)h.side_rec * 5 / 4 + 4096, 0x7fffffffu);
-        if (h.overflow & 2u) c->side_vtx_cap = (uint32_t)std::min<uint64_t>((uint64_t)h.side_vtx * 5 / 4 + 4096, 0x7fffffffu);
+        if (h.overflow & 2u) {
+            if (c->side_vtx_cap >= kMaxSideVtx) {
+                set_err(c->err, "draw needs more than %llu clipped vertices", (unsigned long long)kMaxSideVtx);
+                return FGL_E_UNSUPPORTED;
+            }
+            c->side_vtx_cap = (uint32_t)std::min<uint64_t>((uint64_t)h.side_vtx * 5 / 4 + 4096, kMaxSideVtx);
+        }
         if (h.overflow & 4u) {
             uint64_t need = ((uint64_t)h.pairs_hi << 32 | h.pairs_lo) * 5 / 4 + 65536;
             if (need > 0xfffff000ull) {

@@ -63,7 +63,8 @@ struct DrawArgs {
 
     // shaded fragments
     uint64_t *frag_blocksum;    // fragments per SPAN_THREADS spans (accumulated by k_spans)
-    uint64_t *frag_blockoff;
+    uint64_t *frag_blockoff;    // offset of each span block within its group of 1024 blocks
+    uint64_t *frag_gsum;        // fragments per group of 1024 span blocks
     uint32_t *foff;             // first fragment of each span (span_cap + 1 entries)
     Frag *frags;
     uint32_t frag_cap;

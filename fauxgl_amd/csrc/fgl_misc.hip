@@ -191,12 +191,110 @@ __global__ __launch_bounds__(256) void k_resolve_tent(const uint32_t *src, int32
     }
 }
 
+// Integer factors up to RS_F: the workgroup stages the whole source window
+// its RS_OW x RS_OH outputs read (rows and columns clamped, premultiplied
+// once per pixel) in LDS with coalesced loads, then runs both passes from
+// LDS with the same integer steps as k_resolve_h / k_resolve_v.
+constexpr int RS_F = 4, RS_OW = 64, RS_OH = 4;
+constexpr int RS_ROWS = RS_OH * RS_F + RS_F, RS_COLS = RS_OW * RS_F + RS_F;
+constexpr int RS_LOADS = (RS_ROWS * RS_COLS + 255) / 256;
+
+FGL_D uint32_t premultiply(uint32_t p) {
+    const uint32_t a = p >> 24;
+    const uint32_t r = (p & 0xff) * a / 0xff, g = ((p >> 8) & 0xff) * a / 0xff, b = ((p >> 16) & 0xff) * a / 0xff;
+    return r | (g << 8) | (b << 16) | (a << 24);
+}
+
+FGL_D uint32_t tent_tap_sum(const int32_t acc[4], int32_t sum) {
+    uint32_t o = 0;
+    for (int ch = 0; ch < 4; ch++) {
+        int32_t v = sum ? acc[ch] / sum : 0;
+        v = v < 0 ? 0 : (v > 255 ? 255 : v);
+        o |= (uint32_t)v << (8 * ch);
+    }
+    return o;
+}
+
+__global__ __launch_bounds__(256) void k_resolve_tent_lds(const uint32_t *src, int32_t sw, int32_t sh, uint32_t *dst,
+                                                          int32_t dw, int32_t dh, const int16_t *wx, const int32_t *ox,
+                                                          int32_t flx, const int16_t *wy, const int32_t *oy,
+                                                          int32_t fly) {
+    __shared__ uint32_t s[RS_ROWS][RS_COLS];
+    __shared__ uint32_t hrow[RS_ROWS][RS_OW];
+    const int x0 = blockIdx.x * RS_OW, y0 = blockIdx.y * RS_OH;
+    const int nx = min(RS_OW, dw - x0), ny = min(RS_OH, dh - y0);
+    const int r0 = oy[y0], c0 = ox[x0];
+    const int nrows = oy[y0 + ny - 1] + fly - r0;
+    const int ncols = ox[x0 + nx - 1] + flx - c0;
+    // stage: all loads first (independent), then the premultiplied stores
+    uint32_t v[RS_LOADS];
+#pragma unroll
+    for (int k = 0; k < RS_LOADS; k++) {
+        const int q = k * 256 + (int)threadIdx.x;
+        const int j = q / RS_COLS, c = q % RS_COLS;
+        v[k] = 0;
+        if (j < nrows && c < ncols) {
+            const int ry = min(max(r0 + j, 0), sh - 1);
+            const int cx = min(max(c0 + c, 0), sw - 1);
+            v[k] = src[(int64_t)ry * sw + cx];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < RS_LOADS; k++) {
+        const int q = k * 256 + (int)threadIdx.x;
+        if (q < RS_ROWS * RS_COLS) s[q / RS_COLS][q % RS_COLS] = premultiply(v[k]);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < nrows * RS_OW; q += 256) {
+        const int j = q / RS_OW, x = q % RS_OW;
+        if (x >= nx) continue;
+        const int gx = x0 + x;
+        const int base = ox[gx] - c0;
+        int32_t acc[4] = {0, 0, 0, 0};
+        int32_t sum = 0;
+        for (int32_t k = 0; k < flx; k++) {
+            const int32_t c = wx[gx * flx + k];
+            const uint32_t p = s[j][base + k];
+            acc[0] += c * (int32_t)(p & 0xff);
+            acc[1] += c * (int32_t)((p >> 8) & 0xff);
+            acc[2] += c * (int32_t)((p >> 16) & 0xff);
+            acc[3] += c * (int32_t)(p >> 24);
+            sum += c;
+        }
+        hrow[j][x] = tent_tap_sum(acc, sum);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < RS_OH * RS_OW; q += 256) {
+        const int yy = q / RS_OW, x = q % RS_OW;
+        if (x >= nx || yy >= ny) continue;
+        const int gy = y0 + yy;
+        const int base = oy[gy] - r0;
+        int32_t acc[4] = {0, 0, 0, 0};
+        int32_t sum = 0;
+        for (int32_t k = 0; k < fly; k++) {
+            const int32_t c = wy[gy * fly + k];
+            const uint32_t p = hrow[base + k][x];
+            for (int ch = 0; ch < 4; ch++) acc[ch] += c * (int32_t)((p >> (8 * ch)) & 0xff);
+            sum += c;
+        }
+        dst[(int64_t)gy * dw + x0 + x] = tent_tap_sum(acc, sum);
+    }
+}
+
 hipError_t launch_resolve(const uint32_t *src, int32_t sw, int32_t sh, uint32_t *tmp, uint32_t *dst, int32_t dw,
                           int32_t dh, int32_t filter, const int16_t *wx, const int32_t *ox, int32_t flx,
                           const int16_t *wy, const int32_t *oy, int32_t fly, hipStream_t s) {
     if (filter == 2) {
         hipLaunchKernelGGL(k_resolve_box, dim3(grid_for((int64_t)dw * dh)), dim3(256), 0, s, src, sw, dst, dw, dh,
                            sw / dw);
+        return hipGetLastError();
+    }
+    // integer factor up to RS_F: the source window fits the LDS stage
+    if (sw % dw == 0 && sh % dh == 0 && sw / dw == sh / dh && sw / dw <= RS_F && flx == 2 * (sw / dw) &&
+        fly == 2 * (sh / dh)) {
+        dim3 grid((dw + RS_OW - 1) / RS_OW, (dh + RS_OH - 1) / RS_OH);
+        hipLaunchKernelGGL(k_resolve_tent_lds, grid, dim3(256), 0, s, src, sw, sh, dst, dw, dh, wx, ox, flx, wy, oy,
+                           fly);
         return hipGetLastError();
     }
     // rows one output tile's vertical taps reach: (RT_OH - 1) * step + fly

@@ -36,7 +36,12 @@ constexpr uint32_t OWNER_NONE = 0xffffffffu;
 // (the library exp/log are large and only non-integer powers need them).
 __device__ __noinline__ double pow_frac(double x, double yf) { return exp(yf * log(x)); }
 
-__device__ double go_pow(double x, double y) {
+// math.Pow(x, sh.power).  The y-only steps (Modf, the yf > 0.5 adjustment)
+// were made on the host, so the squaring loop runs over the bits of yi in
+// scalar registers; xe and ae stay within +-2^19 here (the loop stops once
+// |xe| > 2^12), so 32-bit integers hold Go's int values exactly.
+__device__ double go_pow(double x, const DevShader &sh) {
+    const double y = sh.power;
     if (y == 0 || x == 1) return 1;
     if (y == 1) return x;
     if (__builtin_isnan(x) || __builtin_isnan(y)) return __builtin_nan("");
@@ -65,28 +70,17 @@ __device__ double go_pow(double x, double y) {
     }
     if (y == 0.5) return __builtin_sqrt(x);
     if (y == -0.5) return 1.0 / __builtin_sqrt(x);
-    double ay = __builtin_fabs(y);
-    double yi = __builtin_trunc(ay);
-    double yf = ay - yi;
-    if (yf != 0 && x < 0) return __builtin_nan("");
-    if (yi >= 9223372036854775808.0) {
+    if (sh.pw_frac && x < 0) return __builtin_nan("");
+    if (sh.pw_big) {
         if (x == -1) return 1;
         if ((__builtin_fabs(x) < 1) == (y > 0)) return 0;
         return __builtin_inf();
     }
-    double a1 = 1.0;
-    int64_t ae = 0;
-    if (yf != 0) {
-        if (yf > 0.5) {
-            yf = yf - 1;
-            yi = yi + 1;
-        }
-        a1 = pow_frac(x, yf);
-    }
-    int xe_i;
-    double x1 = frexp(x, &xe_i);
-    int64_t xe = xe_i;
-    for (int64_t i = (int64_t)yi; i != 0; i >>= 1) {
+    double a1 = sh.pw_frac ? pow_frac(x, sh.pw_yf) : 1.0;
+    int32_t ae = 0;
+    int xe;
+    double x1 = frexp(x, &xe);
+    for (uint64_t i = sh.pw_yi; i != 0; i >>= 1) {
         if (xe < -(1 << 12) || (1 << 12) < xe) {
             ae += xe;
             break;
@@ -106,9 +100,7 @@ __device__ double go_pow(double x, double y) {
         a1 = 1.0 / a1;
         ae = -ae;
     }
-    if (ae > 100000) ae = 100000;
-    if (ae < -100000) ae = -100000;
-    return ldexp(a1, (int)ae);
+    return ldexp(a1, ae);
 }
 
 // ---------------------------------------------------------------------------
@@ -193,13 +185,16 @@ FGL_D AttrSrc attr_src(const DrawArgs &a) {
 
 // Address of vertex k's attribute: the input array (triangle 3i+sel, line
 // 2i+sel) or the side vertex (i+sel) at offset side_off.
+// Offsets fit 32 bits: at most 2^26 input primitives (3 * 2^26 vertices
+// times 4 doubles) and 2^28 side vertices (times SIDE_VTX_DOUBLES).
 FGL_D const double *attr_ptr(const AttrSrc &src, uint64_t attr, int k, const double *arr, int32_t stride,
                              int side_off) {
     const uint32_t kind = attr_kind(attr);
-    const int64_t idx = (int64_t)attr_index(attr);
-    const int64_t m = kind == ATTR_TRI ? 3 : kind == ATTR_LINE ? 2 : 1;
-    const int64_t vid = idx * m + attr_sel(attr, k);
-    return kind == ATTR_SIDE ? src.side + vid * SIDE_VTX_DOUBLES + side_off : arr + vid * stride;
+    const uint32_t idx = (uint32_t)attr_index(attr);
+    const uint32_t m = kind == ATTR_TRI ? 3u : kind == ATTR_LINE ? 2u : 1u;
+    const uint32_t vid = idx * m + attr_sel(attr, k);
+    return kind == ATTR_SIDE ? src.side + (vid * (uint32_t)SIDE_VTX_DOUBLES + (uint32_t)side_off)
+                             : arr + vid * (uint32_t)stride;
 }
 
 FGL_D V3 fetch_pos(const AttrSrc &src, uint64_t attr, int k) {
@@ -285,7 +280,7 @@ FGL_D C4 eval(const DevShader &sh, const ShadeIn &in, V4 b) {
             V3 reflected = vsub(i, vmuls(N, 2 * vdot(N, i)));
             double specular = go_max(vdot(camera, reflected), 0);
             if (specular > 0) {
-                specular = go_pow(specular, sh.power);
+                specular = go_pow(specular, sh);
                 light = cadd_scaled(light, c4(sh.spc[0], sh.spc[1], sh.spc[2], sh.spc[3]), specular);
             }
         }
@@ -306,21 +301,36 @@ FGL_D const PrimRec *prim_of(const DrawArgs &a, uint32_t ref) {
 // that exist, so the grid (sized from capacity) need not match the count.
 constexpr uint32_t SPAN_GRID_MAX = 2048;
 
-__global__ __launch_bounds__(SCAN_THREADS) void k_frag_scan(DrawArgs a) {
-    __shared__ uint64_t lds[SCAN_THREADS / 64];
+// Two levels: each workgroup scans FRAG_GROUP consecutive span-block sums
+// (coalesced) into group-local offsets and writes the group's total; the
+// shading kernel adds the totals of the groups before its block.
+constexpr int FRAG_GROUP = 1024;
+
+__global__ __launch_bounds__(FRAG_GROUP) void k_frag_scan(DrawArgs a) {
+    __shared__ uint64_t wsum[FRAG_GROUP / 64];
     if (a.ctr->overflow) return;
     const uint32_t NS = a.ctr->nspans;
-    const int nb = (int)((NS + SPAN_THREADS - 1) / SPAN_THREADS);
-    const uint64_t total = scan_block_sums(a.frag_blocksum, a.frag_blockoff, nb, lds);
-    if (threadIdx.x == SCAN_THREADS - 1) {
-        if (total > a.frag_cap) atomicOr(&a.ctr->overflow, 32u);
-        a.ctr->nfrags = total > a.frag_cap ? a.frag_cap : (uint32_t)total;
-        a.ctr->frags_lo = (uint32_t)total;
-        a.ctr->frags_hi = (uint32_t)(total >> 32);
-        a.foff[NS] = total > a.frag_cap ? a.frag_cap : (uint32_t)total;
-        // TotalPixels: every fragment is an inside pixel with a valid index
-        a.ctr->total = total;
+    const uint32_t nb = (NS + SPAN_THREADS - 1) / SPAN_THREADS;
+    const uint32_t g = blockIdx.x;
+    if ((uint64_t)g * FRAG_GROUP >= nb) return;
+    const uint32_t b = g * FRAG_GROUP + threadIdx.x;
+    const uint64_t v = b < nb ? a.frag_blocksum[b] : 0;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint64_t incl = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint64_t y = __shfl_up(incl, off);
+        if (lane >= off) incl += y;
     }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    uint64_t base = 0, total = 0;
+    for (int w = 0; w < FRAG_GROUP / 64; w++) {
+        const uint64_t x = wsum[w];
+        if (w < wave) base += x;
+        total += x;
+    }
+    if (b < nb) a.frag_blockoff[b] = base + incl - v;
+    if (threadIdx.x == 0) a.frag_gsum[g] = total;
 }
 
 // ---------------------------------------------------------------------------
@@ -345,6 +355,20 @@ __global__ __launch_bounds__(SPAN_THREADS, SH == 3 ? FGL_SHADE_WAVES : 1) void k
     if (a.ctr->overflow) return;
     const uint32_t NS = a.ctr->nspans;
     const uint32_t nb = (NS + SPAN_THREADS - 1) / SPAN_THREADS;
+    const uint32_t ngroups = (nb + FRAG_GROUP - 1) / FRAG_GROUP;
+    // fragment total from the group sums; every workgroup sees the same
+    uint64_t total = 0;
+    for (uint32_t g = 0; g < ngroups; g++) total += a.frag_gsum[g];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (total > a.frag_cap) atomicOr(&a.ctr->overflow, 32u);
+        a.ctr->nfrags = total > a.frag_cap ? a.frag_cap : (uint32_t)total;
+        a.ctr->frags_lo = (uint32_t)total;
+        a.ctr->frags_hi = (uint32_t)(total >> 32);
+        a.foff[NS] = total > a.frag_cap ? a.frag_cap : (uint32_t)total;
+        // TotalPixels: every fragment is an inside pixel with a valid index
+        a.ctr->total = total;
+    }
+    if (total > a.frag_cap) return;  // the host grows the fragment buffer and reruns
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const AttrSrc src = attr_src(a);
     for (uint32_t blk = blockIdx.x; blk < nb; blk += gridDim.x) {
@@ -359,6 +383,7 @@ __global__ __launch_bounds__(SPAN_THREADS, SH == 3 ? FGL_SHADE_WAVES : 1) void k
         if (lane == 63) wsum[wave] = incl;
         __syncthreads();
         uint64_t fo = a.frag_blockoff[blk];
+        for (uint32_t g = 0; g < blk / FRAG_GROUP; g++) fo += a.frag_gsum[g];
         for (int w = 0; w < wave; w++) fo += wsum[w];
         __syncthreads();
         const uint32_t excl = incl - n;
@@ -563,7 +588,9 @@ static unsigned span_grid(const DrawArgs &a) {
 }
 
 hipError_t launch_frag_scan(const DrawArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_frag_scan, dim3(1), dim3(SCAN_THREADS), 0, s, a);
+    const uint64_t nb = ((uint64_t)a.span_cap + SPAN_THREADS - 1) / SPAN_THREADS;
+    const unsigned groups = (unsigned)((nb + FRAG_GROUP - 1) / FRAG_GROUP);
+    hipLaunchKernelGGL(k_frag_scan, dim3(groups ? groups : 1), dim3(FRAG_GROUP), 0, s, a);
     return hipGetLastError();
 }
 hipError_t launch_shade(const DrawArgs &a, hipStream_t s) {

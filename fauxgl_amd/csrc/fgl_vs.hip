@@ -229,56 +229,62 @@ FGL_D double ip(double a, double b, double c, V4 k) {
     n = n + c * k.z;
     return n * k.w;
 }
-FGL_D Vtx interpolate_clip(const Vtx *V, V4 b) {
+FGL_D Vtx interpolate_clip(const Vtx &A, const Vtx &B, const Vtx &C, V4 b) {
     Vtx o;
-    o.pos = v3(ip(V[0].pos.x, V[1].pos.x, V[2].pos.x, b), ip(V[0].pos.y, V[1].pos.y, V[2].pos.y, b),
-               ip(V[0].pos.z, V[1].pos.z, V[2].pos.z, b));
-    o.nrm = vnormalize(v3(ip(V[0].nrm.x, V[1].nrm.x, V[2].nrm.x, b), ip(V[0].nrm.y, V[1].nrm.y, V[2].nrm.y, b),
-                          ip(V[0].nrm.z, V[1].nrm.z, V[2].nrm.z, b)));
-    o.u = ip(V[0].u, V[1].u, V[2].u, b);
-    o.v = ip(V[0].v, V[1].v, V[2].v, b);
-    o.col = c4(ip(V[0].col.r, V[1].col.r, V[2].col.r, b), ip(V[0].col.g, V[1].col.g, V[2].col.g, b),
-               ip(V[0].col.b, V[1].col.b, V[2].col.b, b), ip(V[0].col.a, V[1].col.a, V[2].col.a, b));
-    o.out = v4(ip(V[0].out.x, V[1].out.x, V[2].out.x, b), ip(V[0].out.y, V[1].out.y, V[2].out.y, b),
-               ip(V[0].out.z, V[1].out.z, V[2].out.z, b), ip(V[0].out.w, V[1].out.w, V[2].out.w, b));
+    o.pos = v3(ip(A.pos.x, B.pos.x, C.pos.x, b), ip(A.pos.y, B.pos.y, C.pos.y, b), ip(A.pos.z, B.pos.z, C.pos.z, b));
+    o.nrm = vnormalize(
+        v3(ip(A.nrm.x, B.nrm.x, C.nrm.x, b), ip(A.nrm.y, B.nrm.y, C.nrm.y, b), ip(A.nrm.z, B.nrm.z, C.nrm.z, b)));
+    o.u = ip(A.u, B.u, C.u, b);
+    o.v = ip(A.v, B.v, C.v, b);
+    o.col = c4(ip(A.col.r, B.col.r, C.col.r, b), ip(A.col.g, B.col.g, C.col.g, b), ip(A.col.b, B.col.b, C.col.b, b),
+               ip(A.col.a, B.col.a, C.col.a, b));
+    o.out = v4(ip(A.out.x, B.out.x, C.out.x, b), ip(A.out.y, B.out.y, C.out.y, b), ip(A.out.z, B.out.z, C.out.z, b),
+               ip(A.out.w, B.out.w, C.out.w, b));
     return o;
 }
 
 // Triangle.FixNormals (triangle.go:33-37, 46-58)
-FGL_D void fix_normals(Vtx *V) {
-    V3 e1 = vsub(V[1].pos, V[0].pos);
-    V3 e2 = vsub(V[2].pos, V[0].pos);
+FGL_D void fix_normals(Vtx &A, Vtx &B, Vtx &C) {
+    V3 e1 = vsub(B.pos, A.pos);
+    V3 e2 = vsub(C.pos, A.pos);
     V3 n = vnormalize(vcross(e1, e2));
-    for (int k = 0; k < 3; k++)
-        if (V[k].nrm.x == 0 && V[k].nrm.y == 0 && V[k].nrm.z == 0) V[k].nrm = n;
+    if (A.nrm.x == 0 && A.nrm.y == 0 && A.nrm.z == 0) A.nrm = n;
+    if (B.nrm.x == 0 && B.nrm.y == 0 && B.nrm.z == 0) B.nrm = n;
+    if (C.nrm.x == 0 && C.nrm.y == 0 && C.nrm.z == 0) C.nrm = n;
+}
+
+FGL_D void store_side_vertex(const DrawArgs &a, uint64_t v, const Vtx &F) {
+    double *d = a.side_vtx + v * SIDE_VTX_DOUBLES;
+    d[0] = F.pos.x; d[1] = F.pos.y; d[2] = F.pos.z;
+    d[3] = F.nrm.x; d[4] = F.nrm.y; d[5] = F.nrm.z;
+    d[6] = F.u; d[7] = F.v;
+    d[8] = F.col.r; d[9] = F.col.g; d[10] = F.col.b; d[11] = F.col.a;
 }
 
 // ClipTriangle + per-fan drawClippedTriangle: the fan triangles are built
 // (vertex attributes written at side_vtx[vbase..]) and emitted into em.
-__device__ __noinline__ void clip_fans(const DrawArgs &a, const Vtx *V, const V4 *poly, int np, Emitter &em,
-                                       uint32_t vbase) {
-    V3 p1 = v3(V[0].out.x, V[0].out.y, V[0].out.z);
-    V3 p2 = v3(V[1].out.x, V[1].out.y, V[1].out.z);
-    V3 p3 = v3(V[2].out.x, V[2].out.y, V[2].out.z);
+// Barycentric + InterpolateVertexes of a polygon point give the same
+// vertex every time the reference recomputes them (clipping.go:64-70), so
+// each point is interpolated once; NewTriangle's FixNormals is applied per
+// fan to copies.
+FGL_D void clip_fans(const DrawArgs &a, const Vtx &V0, const Vtx &V1, const Vtx &V2, const V4 *poly, int np,
+                     Emitter &em, uint32_t vbase) {
+    const V3 p1 = v3(V0.out.x, V0.out.y, V0.out.z);
+    const V3 p2 = v3(V1.out.x, V1.out.y, V1.out.z);
+    const V3 p3 = v3(V2.out.x, V2.out.y, V2.out.z);
+    const Vtx F0 = interpolate_clip(V0, V1, V2, barycentric(p1, p2, p3, v3(poly[0].x, poly[0].y, poly[0].z)));
+    Vtx Fp = interpolate_clip(V0, V1, V2, barycentric(p1, p2, p3, v3(poly[1].x, poly[1].y, poly[1].z)));
     for (int i = 2; i < np; i++) {
-        V4 b1 = barycentric(p1, p2, p3, v3(poly[0].x, poly[0].y, poly[0].z));
-        V4 b2 = barycentric(p1, p2, p3, v3(poly[i - 1].x, poly[i - 1].y, poly[i - 1].z));
-        V4 b3 = barycentric(p1, p2, p3, v3(poly[i].x, poly[i].y, poly[i].z));
-        Vtx F[3];
-        F[0] = interpolate_clip(V, b1);
-        F[1] = interpolate_clip(V, b2);
-        F[2] = interpolate_clip(V, b3);
-        fix_normals(F);  // NewTriangle
-        uint64_t vb = (uint64_t)vbase + 3u * (uint32_t)(i - 2);
-        for (int k = 0; k < 3; k++) {
-            double *d = a.side_vtx + (vb + k) * SIDE_VTX_DOUBLES;
-            d[0] = F[k].pos.x; d[1] = F[k].pos.y; d[2] = F[k].pos.z;
-            d[3] = F[k].nrm.x; d[4] = F[k].nrm.y; d[5] = F[k].nrm.z;
-            d[6] = F[k].u; d[7] = F[k].v;
-            d[8] = F[k].col.r; d[9] = F[k].col.g; d[10] = F[k].col.b; d[11] = F[k].col.a;
-        }
+        const Vtx Fi = interpolate_clip(V0, V1, V2, barycentric(p1, p2, p3, v3(poly[i].x, poly[i].y, poly[i].z)));
+        Vtx T0 = F0, T1 = Fp, T2 = Fi;
+        fix_normals(T0, T1, T2);  // NewTriangle
+        const uint64_t vb = (uint64_t)vbase + 3u * (uint32_t)(i - 2);
+        store_side_vertex(a, vb, T0);
+        store_side_vertex(a, vb + 1, T1);
+        store_side_vertex(a, vb + 2, T2);
         const uint32_t sel[3] = {0, 1, 2};
-        emit_clipped_triangle(a, em, ATTR_SIDE, vb, sel, F[0].out, F[1].out, F[2].out);
+        emit_clipped_triangle(a, em, ATTR_SIDE, vb, sel, T0.out, T1.out, T2.out);
+        Fp = Fi;
     }
 }
 
@@ -323,18 +329,15 @@ __device__ __noinline__ void vs_triangle_clip(const DrawArgs &a, int64_t i, V4 o
     const int np = sutherland_hodgman(buf, &which);
     if (np < 3) return;
     const uint32_t nfan = (uint32_t)(np - 2);
-    Vtx V[3];
-    V[0] = load_vertex(a.in, 3 * i);
-    V[1] = load_vertex(a.in, 3 * i + 1);
-    V[2] = load_vertex(a.in, 3 * i + 2);
-    V[0].out = o0;
-    V[1].out = o1;
-    V[2].out = o2;
-    fix_normals(V);
+    Vtx V0 = load_vertex(a.in, 3 * i), V1 = load_vertex(a.in, 3 * i + 1), V2 = load_vertex(a.in, 3 * i + 2);
+    V0.out = o0;
+    V1.out = o1;
+    V2.out = o2;
+    fix_normals(V0, V1, V2);  // NewTriangle (context.go:382)
     uint32_t rb, vb;
     if (!alloc_side(a, nfan * (a.st.wireframe ? 6u : 1u), 3 * nfan, rb, vb)) return;
     Emitter em{a.side_rec + rb, 0, 0};
-    clip_fans(a, V, buf[which], np, em, vb);
+    clip_fans(a, V0, V1, V2, buf[which], np, em, vb);
     if (em.n == 0) return;
     nr = em.n | REF_SIDE;
     nt = em.ntiles;
@@ -484,16 +487,14 @@ __global__ __launch_bounds__(64) void k_vs_deferred(DrawArgs a) {
                 ok = false;
             }
             if (ok) {
-                Vtx V[3];
-                V[0] = load_vertex(a.in, 3 * i);
-                V[1] = load_vertex(a.in, 3 * i + 1);
-                V[2] = load_vertex(a.in, 3 * i + 2);
-                V[0].out = o0;
-                V[1].out = o1;
-                V[2].out = o2;
-                fix_normals(V);  // NewTriangle (context.go:382)
+                Vtx V0 = load_vertex(a.in, 3 * i), V1 = load_vertex(a.in, 3 * i + 1),
+                    V2 = load_vertex(a.in, 3 * i + 2);
+                V0.out = o0;
+                V1.out = o1;
+                V2.out = o2;
+                fix_normals(V0, V1, V2);  // NewTriangle (context.go:382)
                 Emitter em{a.side_rec + rb, 0, 0};
-                clip_fans(a, V, buf[which], np, em, vb);
+                clip_fans(a, V0, V1, V2, buf[which], np, em, vb);
                 if (em.n) {
                     nr = em.n | REF_SIDE;
                     nt = em.ntiles;

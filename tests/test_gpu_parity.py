@@ -327,3 +327,12 @@ def test_soup_blend_texture_full_1080p():
     mesh, sh = M.soup_scene(W, H)
     ops = [("clear_color", M.BACKGROUND), ("set", {"DepthBias": 1e-4}), ("shader", sh), ("mesh", mesh)]
     check("soup_blend_texture_200k_1920x1080", W, H, ops, resident=True)
+
+
+def test_noise_sphere_full_16x():
+    """The headline draw itself: 871,306 triangles at 7680x4320 (4x4 SSAA
+    internal size), bit-exact against the single-threaded oracle."""
+    W, H = 7680, 4320
+    mesh = M.noise_sphere_mesh()
+    ops = [("clear_color", M.BACKGROUND), ("shader", M.phong_shader(1920, 1080)), ("mesh", mesh)]
+    check("noise_sphere_871k_7680x4320", W, H, ops, resident=True)
