@@ -81,6 +81,8 @@ struct DevShader {
     uint64_t pw_yi;
     double pw_yf;
     int32_t pw_frac, pw_big;
+    int32_t pw_niter;  // bit length of pw_yi (squaring steps)
+    int32_t pw_pad;
 };
 
 struct DevState {

@@ -286,6 +286,8 @@ int fill_shader(fgl_ctx *c, const fgl_shader *sh, DevShader &d) {
         }
         d.pw_yf = yf;
         d.pw_yi = (std::isfinite(yi) && yi < 9223372036854775808.0) ? (uint64_t)yi : 0;
+        d.pw_niter = 0;
+        for (uint64_t v = d.pw_yi; v; v >>= 1) d.pw_niter++;
     }
     // ObjectColor != Discard compares all four channels with 0 (shader.go:76)
     d.obj_is_discard = (sh->object_color[0] == 0 && sh->object_color[1] == 0 && sh->object_color[2] == 0 &&

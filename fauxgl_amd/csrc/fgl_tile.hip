@@ -80,20 +80,39 @@ __device__ double go_pow(double x, const DevShader &sh) {
     int32_t ae = 0;
     int xe;
     double x1 = frexp(x, &xe);
-    for (uint64_t i = sh.pw_yi; i != 0; i >>= 1) {
-        if (xe < -(1 << 12) || (1 << 12) < xe) {
-            ae += xe;
-            break;
+    // |xe| at most doubles (plus one) per step, so when (|xe| + 1) shifted
+    // by the step count stays within 2^12 the overflow exit cannot be taken
+    // and the loop runs without it.
+    const int32_t axe = xe < 0 ? -xe : xe;
+    if (sh.pw_niter <= 13 && axe + 1 <= ((1 << 12) >> (sh.pw_niter > 0 ? sh.pw_niter - 1 : 0))) {
+        for (uint64_t i = sh.pw_yi; i != 0; i >>= 1) {
+            if (i & 1) {
+                a1 *= x1;
+                ae += xe;
+            }
+            x1 *= x1;
+            xe <<= 1;
+            if (x1 < .5) {
+                x1 += x1;
+                xe--;
+            }
         }
-        if (i & 1) {
-            a1 *= x1;
-            ae += xe;
-        }
-        x1 *= x1;
-        xe <<= 1;
-        if (x1 < .5) {
-            x1 += x1;
-            xe--;
+    } else {
+        for (uint64_t i = sh.pw_yi; i != 0; i >>= 1) {
+            if (xe < -(1 << 12) || (1 << 12) < xe) {
+                ae += xe;
+                break;
+            }
+            if (i & 1) {
+                a1 *= x1;
+                ae += xe;
+            }
+            x1 *= x1;
+            xe <<= 1;
+            if (x1 < .5) {
+                x1 += x1;
+                xe--;
+            }
         }
     }
     if (y < 0) {
@@ -289,6 +308,42 @@ FGL_D C4 eval(const DevShader &sh, const ShadeIn &in, V4 b) {
     }
 }
 
+// PhongShader.Fragment with the vertex attributes fetched where they are
+// first needed (positions, then normals, then colors), with scheduling
+// barriers between the steps so the three fetches are not hoisted together:
+// fewer values live at once leaves registers for more waves per SIMD.
+FGL_D C4 eval_phong_staged(const DevShader &sh, const AttrSrc &src, uint64_t attr, V4 b) {
+    V3 p0 = fetch_pos(src, attr, 0), p1 = fetch_pos(src, attr, 1), p2 = fetch_pos(src, attr, 2);
+    const V3 P = v3(interp(p0.x, p1.x, p2.x, b), interp(p0.y, p1.y, p2.y, b), interp(p0.z, p1.z, p2.z, b));
+    __builtin_amdgcn_sched_barrier(0);
+    V3 n0 = fetch_nrm(src, attr, 0), n1 = fetch_nrm(src, attr, 1), n2 = fetch_nrm(src, attr, 2);
+    const V3 N = vnormalize(v3(interp(n0.x, n1.x, n2.x, b), interp(n0.y, n1.y, n2.y, b), interp(n0.z, n1.z, n2.z, b)));
+    __builtin_amdgcn_sched_barrier(0);
+    C4 light = c4(sh.amb[0], sh.amb[1], sh.amb[2], sh.amb[3]);
+    C4 color;
+    if (!sh.obj_is_discard) {
+        color = c4(sh.obj[0], sh.obj[1], sh.obj[2], sh.obj[3]);
+    } else {
+        const C4 c0 = fetch_col(src, attr, 0), c1 = fetch_col(src, attr, 1), c2 = fetch_col(src, attr, 2);
+        color = c4(interp(c0.r, c1.r, c2.r, b), interp(c0.g, c1.g, c2.g, b), interp(c0.b, c1.b, c2.b, b),
+                   interp(c0.a, c1.a, c2.a, b));
+    }
+    const V3 L = v3(sh.light[0], sh.light[1], sh.light[2]);
+    const double diffuse = go_max(vdot(N, L), 0);
+    light = cadd_scaled(light, c4(sh.dif[0], sh.dif[1], sh.dif[2], sh.dif[3]), diffuse);
+    if (diffuse > 0 && sh.power > 0) {
+        const V3 camera = vnormalize(vsub(v3(sh.cam[0], sh.cam[1], sh.cam[2]), P));
+        const V3 i = v3(-L.x, -L.y, -L.z);
+        const V3 reflected = vsub(i, vmuls(N, 2 * vdot(N, i)));
+        double specular = go_max(vdot(camera, reflected), 0);
+        if (specular > 0) {
+            specular = go_pow(specular, sh);
+            light = cadd_scaled(light, c4(sh.spc[0], sh.spc[1], sh.spc[2], sh.spc[3]), specular);
+        }
+    }
+    return c4(go_min(color.r * light.r, 1), go_min(color.g * light.g, 1), go_min(color.b * light.b, 1), color.a);
+}
+
 FGL_D const PrimRec *prim_of(const DrawArgs &a, uint32_t ref) {
     return (ref & REF_SIDE) ? a.side_rec + (ref & ~REF_SIDE) : a.rec_main + ref;
 }
@@ -345,13 +400,32 @@ __global__ __launch_bounds__(FRAG_GROUP) void k_frag_scan(DrawArgs a) {
 // Waves per SIMD the untextured Phong kernel is compiled for (its register
 // budget): four hide the attribute-fetch latency better than the three an
 // unconstrained build fits, at the cost of a few spilled registers.
+// Inclusive max-scan over the 64 lanes of a wave with DPP row shifts and
+// row broadcasts (values >= 0; lanes with no source contribute 0).
+FGL_D uint32_t wave_incl_max(uint32_t v) {
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false));  // row_shr:1
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false));  // row_shr:2
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false));  // row_shr:4
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false));  // row_shr:8
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+    return v;
+}
+
 #ifndef FGL_SHADE_WAVES
 #define FGL_SHADE_WAVES 4
+#endif
+#ifndef FGL_SHADE_STAGED
+#define FGL_SHADE_STAGED 0
+#endif
+#ifndef FGL_SHADE_OWNER_DPP
+#define FGL_SHADE_OWNER_DPP 1
 #endif
 
 template <int SH>
 __global__ __launch_bounds__(SPAN_THREADS, SH == 3 ? FGL_SHADE_WAVES : 1) void k_shade(DrawArgs a) {
     __shared__ uint32_t wsum[SPAN_THREADS / 64];
+    __shared__ uint32_t ostart[SPAN_THREADS / 64][64];
     if (a.ctr->overflow) return;
     const uint32_t NS = a.ctr->nspans;
     const uint32_t nb = (NS + SPAN_THREADS - 1) / SPAN_THREADS;
@@ -391,8 +465,23 @@ __global__ __launch_bounds__(SPAN_THREADS, SH == 3 ? FGL_SHADE_WAVES : 1) void k
         const uint32_t T = __shfl(incl, 63);
         const uint32_t pn = (uint32_t)sp.pix | ((uint32_t)sp.n << 16);
         Frag *out = a.frags + fo;
+#if FGL_SHADE_OWNER_DPP
+        uint32_t carry = 0;  // owner lane + 1 of the last fragment of the previous window
+#endif
         for (uint32_t base = 0; base < T; base += 64) {
             const uint32_t p = base + (uint32_t)lane;
+#if FGL_SHADE_OWNER_DPP
+            // owner: each span starting in this window marks its start with
+            // its lane (+1); a max-scan carries it over the span's pixels
+            ostart[wave][lane] = 0;
+            if (n && excl >= base && excl - base < 64) ostart[wave][excl - base] = (uint32_t)lane + 1;
+            __builtin_amdgcn_wave_barrier();
+            uint32_t om = wave_incl_max(ostart[wave][lane]);
+            om = om ? om : carry;
+            carry = __builtin_amdgcn_readlane(om, 63);
+            __builtin_amdgcn_wave_barrier();
+            const int ow = (int)om - 1 < 0 ? 0 : (int)om - 1;
+#else
             // owner: the number of lanes whose running count is <= p
             int lo = 0;
 #pragma unroll
@@ -401,6 +490,7 @@ __global__ __launch_bounds__(SPAN_THREADS, SH == 3 ? FGL_SHADE_WAVES : 1) void k
                 if (v <= p) lo += step;
             }
             const int ow = lo < 64 ? lo : 63;
+#endif
             const uint32_t oexcl = __shfl(excl, ow);
             const uint32_t oref = __shfl(sp.ref, ow);
             const uint32_t opn = __shfl(pn, ow);
@@ -414,6 +504,7 @@ __global__ __launch_bounds__(SPAN_THREADS, SH == 3 ? FGL_SHADE_WAVES : 1) void k
                 const double a20 = pr->sy[0] - pr->sy[2];
                 const double a01 = pr->sy[1] - pr->sy[0];
                 // the reference's per-pixel step, j times (context.go:208-213)
+#pragma unroll 4
                 for (uint32_t k = 0; k < j; k++) {
                     w0 += a12;
                     w1 += a20;
@@ -421,14 +512,19 @@ __global__ __launch_bounds__(SPAN_THREADS, SH == 3 ? FGL_SHADE_WAVES : 1) void k
                 }
                 const double ra = pr->ra;
                 const double b0 = w0 * ra, b1 = w1 * ra, b2 = w2 * ra;
-                ShadeIn in;
-                load_in<SH>(a.sh, src, pr->attr, in);
                 Frag f;
                 f.z = b0 * pr->sz[0] + b1 * pr->sz[1] + b2 * pr->sz[2];
                 // perspective-correct interpolation weights (context.go:236-237)
                 V4 bw = v4(b0 * pr->r[0], b1 * pr->r[1], b2 * pr->r[2], 0);
                 bw.w = 1.0 / (bw.x + bw.y + bw.z);
-                const C4 c = eval<SH>(a.sh, in, bw);
+                C4 c;
+                if (SH == 3 && FGL_SHADE_STAGED) {
+                    c = eval_phong_staged(a.sh, src, pr->attr, bw);
+                } else {
+                    ShadeIn in;
+                    load_in<SH>(a.sh, src, pr->attr, in);
+                    c = eval<SH>(a.sh, in, bw);
+                }
                 f.color = nrgba_pack(c);
                 f.pix = (uint16_t)((opn & 0xffffu) + j);
                 f.flags = ((opn >> 16) & SPAN_INB ? FRAG_INB : (uint16_t)0) | (c.a < 1 ? FRAG_BLEND : (uint16_t)0) |
@@ -595,14 +691,16 @@ hipError_t launch_frag_scan(const DrawArgs &a, hipStream_t s) {
 }
 hipError_t launch_shade(const DrawArgs &a, hipStream_t s) {
     dim3 grid(span_grid(a)), block(SPAN_THREADS);
+#define K_SHADE k_shade
     switch (a.sh.kind) {
-        case 1: hipLaunchKernelGGL(k_shade<1>, grid, block, 0, s, a); break;
-        case 2: hipLaunchKernelGGL(k_shade<2>, grid, block, 0, s, a); break;
+        case 1: hipLaunchKernelGGL(K_SHADE<1>, grid, block, 0, s, a); break;
+        case 2: hipLaunchKernelGGL(K_SHADE<2>, grid, block, 0, s, a); break;
         default:
-            if (a.sh.has_tex) hipLaunchKernelGGL(k_shade<4>, grid, block, 0, s, a);
-            else hipLaunchKernelGGL(k_shade<3>, grid, block, 0, s, a);
+            if (a.sh.has_tex) hipLaunchKernelGGL(K_SHADE<4>, grid, block, 0, s, a);
+            else hipLaunchKernelGGL(K_SHADE<3>, grid, block, 0, s, a);
             break;
     }
+#undef K_SHADE
     return hipGetLastError();
 }
 hipError_t launch_tile_frags(const DrawArgs &a, hipStream_t s) {
