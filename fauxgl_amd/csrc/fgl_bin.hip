@@ -119,9 +119,22 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(DrawArgs a, int pass)
     h[threadIdx.x] = 0;
     __syncthreads();
     const uint64_t base = (uint64_t)blockIdx.x * RS_BLOCK;
+    if (base >= E) {
+        __syncthreads();
+        a.hist[(uint64_t)threadIdx.x * a.nrsblocks + blockIdx.x] = 0;
+        return;
+    }
+    // unconditional loads (clamped to the last pair) issue back to back
+    uint32_t k[RS_ITEMS];
+#pragma unroll
     for (int it = 0; it < RS_ITEMS; it++) {
-        uint64_t idx = base + (uint64_t)it * RS_THREADS + threadIdx.x;
-        if (idx < E) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
+        const uint64_t idx = base + (uint64_t)it * RS_THREADS + threadIdx.x;
+        k[it] = keys[idx < E ? idx : E - 1];
+    }
+#pragma unroll
+    for (int it = 0; it < RS_ITEMS; it++) {
+        const uint64_t idx = base + (uint64_t)it * RS_THREADS + threadIdx.x;
+        if (idx < E) atomicAdd(&h[(k[it] >> shift) & 255u], 1u);
     }
     __syncthreads();
     a.hist[(uint64_t)threadIdx.x * a.nrsblocks + blockIdx.x] = h[threadIdx.x];
@@ -187,13 +200,24 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(DrawArgs a, int pa
         run[tid] = dbase + a.hist[(uint64_t)tid * a.nrsblocks + blockIdx.x];
     }
     const uint64_t base = (uint64_t)blockIdx.x * RS_BLOCK;
+    if (base >= E) return;
+    // all of this thread's pairs, loaded up front (clamped, unconditional)
+    uint32_t kk[RS_ITEMS], vv[RS_ITEMS];
+#pragma unroll
+    for (int it = 0; it < RS_ITEMS; it++) {
+        const uint64_t idx = base + (uint64_t)it * RS_THREADS + tid;
+        const uint64_t ci = idx < E ? idx : E - 1;
+        kk[it] = kin[ci];
+        vv[it] = vin[ci];
+    }
+#pragma unroll
     for (int it = 0; it < RS_ITEMS; it++) {
         for (int w = 0; w < RS_THREADS / 64; w++) wcnt[w][tid] = 0;
         __syncthreads();
         uint64_t idx = base + (uint64_t)it * RS_THREADS + tid;
         bool valid = idx < E;
-        uint32_t key = valid ? kin[idx] : 0u;
-        uint32_t val = valid ? vin[idx] : 0u;
+        uint32_t key = valid ? kk[it] : 0u;
+        uint32_t val = vv[it];
         uint32_t digit = (key >> shift) & 255u;
         uint64_t peers = __ballot(valid);
 #pragma unroll

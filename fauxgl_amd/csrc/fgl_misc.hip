@@ -221,6 +221,9 @@ __global__ __launch_bounds__(256) void k_resolve_tent_lds(const uint32_t *src, i
                                                           int32_t fly) {
     __shared__ uint32_t s[RS_ROWS][RS_COLS];
     __shared__ uint32_t hrow[RS_ROWS][RS_OW];
+    __shared__ int16_t lwx[RS_OW][2 * RS_F], lwy[RS_OH][2 * RS_F];
+    __shared__ int32_t lox[RS_OW], loy[RS_OH];
+    const int tid = threadIdx.x;
     const int x0 = blockIdx.x * RS_OW, y0 = blockIdx.y * RS_OH;
     const int nx = min(RS_OW, dw - x0), ny = min(RS_OH, dh - y0);
     const int r0 = oy[y0], c0 = ox[x0];
@@ -230,30 +233,37 @@ __global__ __launch_bounds__(256) void k_resolve_tent_lds(const uint32_t *src, i
     uint32_t v[RS_LOADS];
 #pragma unroll
     for (int k = 0; k < RS_LOADS; k++) {
-        const int q = k * 256 + (int)threadIdx.x;
+        const int q = k * 256 + tid;
         const int j = q / RS_COLS, c = q % RS_COLS;
-        v[k] = 0;
-        if (j < nrows && c < ncols) {
-            const int ry = min(max(r0 + j, 0), sh - 1);
-            const int cx = min(max(c0 + c, 0), sw - 1);
-            v[k] = src[(int64_t)ry * sw + cx];
-        }
+        const int ry = min(max(r0 + min(j, nrows - 1), 0), sh - 1);
+        const int cx = min(max(c0 + min(c, ncols - 1), 0), sw - 1);
+        v[k] = src[(int64_t)ry * sw + cx];
     }
+    // this workgroup's rows of the weight tables
+    for (int q = tid; q < RS_OW * 2 * RS_F; q += 256) {
+        const int x = q / (2 * RS_F), k = q % (2 * RS_F);
+        lwx[x][k] = (x < nx && k < flx) ? wx[(x0 + x) * flx + k] : (int16_t)0;
+    }
+    if (tid < RS_OH * 2 * RS_F) {
+        const int y = tid / (2 * RS_F), k = tid % (2 * RS_F);
+        lwy[y][k] = (y < ny && k < fly) ? wy[(y0 + y) * fly + k] : (int16_t)0;
+    }
+    if (tid < RS_OW) lox[tid] = tid < nx ? ox[x0 + tid] - c0 : 0;
+    if (tid < RS_OH) loy[tid] = tid < ny ? oy[y0 + tid] - r0 : 0;
 #pragma unroll
     for (int k = 0; k < RS_LOADS; k++) {
-        const int q = k * 256 + (int)threadIdx.x;
+        const int q = k * 256 + tid;
         if (q < RS_ROWS * RS_COLS) s[q / RS_COLS][q % RS_COLS] = premultiply(v[k]);
     }
     __syncthreads();
-    for (int q = threadIdx.x; q < nrows * RS_OW; q += 256) {
+    for (int q = tid; q < nrows * RS_OW; q += 256) {
         const int j = q / RS_OW, x = q % RS_OW;
         if (x >= nx) continue;
-        const int gx = x0 + x;
-        const int base = ox[gx] - c0;
+        const int base = lox[x];
         int32_t acc[4] = {0, 0, 0, 0};
         int32_t sum = 0;
         for (int32_t k = 0; k < flx; k++) {
-            const int32_t c = wx[gx * flx + k];
+            const int32_t c = lwx[x][k];
             const uint32_t p = s[j][base + k];
             acc[0] += c * (int32_t)(p & 0xff);
             acc[1] += c * (int32_t)((p >> 8) & 0xff);
@@ -264,20 +274,19 @@ __global__ __launch_bounds__(256) void k_resolve_tent_lds(const uint32_t *src, i
         hrow[j][x] = tent_tap_sum(acc, sum);
     }
     __syncthreads();
-    for (int q = threadIdx.x; q < RS_OH * RS_OW; q += 256) {
+    for (int q = tid; q < RS_OH * RS_OW; q += 256) {
         const int yy = q / RS_OW, x = q % RS_OW;
         if (x >= nx || yy >= ny) continue;
-        const int gy = y0 + yy;
-        const int base = oy[gy] - r0;
+        const int base = loy[yy];
         int32_t acc[4] = {0, 0, 0, 0};
         int32_t sum = 0;
         for (int32_t k = 0; k < fly; k++) {
-            const int32_t c = wy[gy * fly + k];
+            const int32_t c = lwy[yy][k];
             const uint32_t p = hrow[base + k][x];
             for (int ch = 0; ch < 4; ch++) acc[ch] += c * (int32_t)((p >> (8 * ch)) & 0xff);
             sum += c;
         }
-        dst[(int64_t)gy * dw + x0 + x] = tent_tap_sum(acc, sum);
+        dst[(int64_t)(y0 + yy) * dw + x0 + x] = tent_tap_sum(acc, sum);
     }
 }
 

@@ -584,14 +584,29 @@ __global__ __launch_bounds__(TILE_THREADS, FGL_ROP_WAVES) void k_rop(DrawArgs a)
         }
         return;
     }
-    for (int p = tid; p < TILE_PIX; p += TILE_THREADS) {
-        int lx = p % TILE_W, ly = p / TILE_W;
-        if (lx < cw && ly < chh) {
-            int64_t gi = (int64_t)(cy0 + ly) * a.W + cx0 + lx;
-            col[p] = a.clear_color_pending ? a.clear_color : a.color[gi];
-            dep[p] = a.clear_depth_pending ? a.clear_depth : a.depth[gi];
+    {
+        // the tile's current color and depth (or the pending clear values);
+        // loads are unconditional (clamped to the tile origin outside the
+        // framebuffer) so they issue back to back
+        constexpr int PPT = TILE_PIX / TILE_THREADS;
+        uint32_t cv[PPT];
+        double dv[PPT];
+        const int64_t g0 = (int64_t)cy0 * a.W + cx0;
+#pragma unroll
+        for (int k = 0; k < PPT; k++) {
+            const int p = k * TILE_THREADS + tid;
+            const int lx = p % TILE_W, ly = p / TILE_W;
+            const int64_t gi = (lx < cw && ly < chh) ? (int64_t)(cy0 + ly) * a.W + cx0 + lx : g0;
+            cv[k] = a.clear_color_pending ? a.clear_color : a.color[gi];
+            dv[k] = a.clear_depth_pending ? a.clear_depth : a.depth[gi];
         }
-        owner[p] = OWNER_NONE;
+#pragma unroll
+        for (int k = 0; k < PPT; k++) {
+            const int p = k * TILE_THREADS + tid;
+            col[p] = cv[k];
+            dep[p] = dv[k];
+            owner[p] = OWNER_NONE;
+        }
     }
     const double bias = a.st.depth_bias;
     const bool rd = a.st.read_depth != 0, wd = a.st.write_depth != 0, wc = a.st.write_color != 0;
@@ -602,13 +617,13 @@ __global__ __launch_bounds__(TILE_THREADS, FGL_ROP_WAVES) void k_rop(DrawArgs a)
     for (uint32_t fc = f0; fc < f1; fc += ROP_CHUNK) {
         Frag fr[ROP_PER_THREAD];
         uint32_t pending = 0;
+        // unconditional loads (clamped to the tile's last fragment) so they
+        // issue back to back; the pending mask drops the clamped copies
 #pragma unroll
         for (int q = 0; q < ROP_PER_THREAD; q++) {
-            uint32_t f = fc + (uint32_t)(q * TILE_THREADS + tid);
-            if (f < f1) {
-                fr[q] = a.frags[f];
-                pending |= 1u << q;
-            }
+            const uint32_t f = fc + (uint32_t)(q * TILE_THREADS + tid);
+            fr[q] = a.frags[f < f1 ? f : f1 - 1];
+            if (f < f1) pending |= 1u << q;
         }
         while (true) {
 #pragma unroll
