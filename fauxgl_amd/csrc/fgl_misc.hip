@@ -195,9 +195,13 @@ __global__ __launch_bounds__(256) void k_resolve_tent(const uint32_t *src, int32
 // its RS_OW x RS_OH outputs read (rows and columns clamped, premultiplied
 // once per pixel) in LDS with coalesced loads, then runs both passes from
 // LDS with the same integer steps as k_resolve_h / k_resolve_v.
-constexpr int RS_F = 4, RS_OW = 64, RS_OH = 4;
+#ifndef FGL_RS_OH
+#define FGL_RS_OH 4
+#endif
+constexpr int RS_F = 4, RS_OW = 64, RS_OH = FGL_RS_OH;
 constexpr int RS_ROWS = RS_OH * RS_F + RS_F, RS_COLS = RS_OW * RS_F + RS_F;
 constexpr int RS_LOADS = (RS_ROWS * RS_COLS + 255) / 256;
+static_assert(RS_COLS % 4 == 0, "staged rows must be 16-byte multiples");
 
 FGL_D uint32_t premultiply(uint32_t p) {
     const uint32_t a = p >> 24;
@@ -205,10 +209,14 @@ FGL_D uint32_t premultiply(uint32_t p) {
     return r | (g << 8) | (b << 16) | (a << 24);
 }
 
+// Weighted sums are non-negative, so when the weight sum is a power of two
+// (every integer factor 2^k) the truncating division is a shift.
 FGL_D uint32_t tent_tap_sum(const int32_t acc[4], int32_t sum) {
+    const bool pow2 = sum > 0 && (sum & (sum - 1)) == 0;
+    const int sh = pow2 ? __builtin_ctz((uint32_t)sum) : 0;
     uint32_t o = 0;
     for (int ch = 0; ch < 4; ch++) {
-        int32_t v = sum ? acc[ch] / sum : 0;
+        int32_t v = pow2 ? (acc[ch] >> sh) : (sum ? acc[ch] / sum : 0);
         v = v < 0 ? 0 : (v > 255 ? 255 : v);
         o |= (uint32_t)v << (8 * ch);
     }
@@ -219,7 +227,7 @@ __global__ __launch_bounds__(256) void k_resolve_tent_lds(const uint32_t *src, i
                                                           int32_t dw, int32_t dh, const int16_t *wx, const int32_t *ox,
                                                           int32_t flx, const int16_t *wy, const int32_t *oy,
                                                           int32_t fly) {
-    __shared__ uint32_t s[RS_ROWS][RS_COLS];
+    __shared__ __align__(16) uint32_t s[RS_ROWS][RS_COLS];
     __shared__ uint32_t hrow[RS_ROWS][RS_OW];
     __shared__ int16_t lwx[RS_OW][2 * RS_F], lwy[RS_OH][2 * RS_F];
     __shared__ int32_t lox[RS_OW], loy[RS_OH];
@@ -262,14 +270,32 @@ __global__ __launch_bounds__(256) void k_resolve_tent_lds(const uint32_t *src, i
         const int base = lox[x];
         int32_t acc[4] = {0, 0, 0, 0};
         int32_t sum = 0;
-        for (int32_t k = 0; k < flx; k++) {
-            const int32_t c = lwx[x][k];
-            const uint32_t p = s[j][base + k];
-            acc[0] += c * (int32_t)(p & 0xff);
-            acc[1] += c * (int32_t)((p >> 8) & 0xff);
-            acc[2] += c * (int32_t)((p >> 16) & 0xff);
-            acc[3] += c * (int32_t)(p >> 24);
-            sum += c;
+        if (flx == 2 * RS_F && (base & 3) == 0) {
+            // the 8 taps as two aligned 16-byte LDS reads (rows are 16-byte
+            // multiples, base = RS_F * x)
+            const uint4 q0 = *reinterpret_cast<const uint4 *>(&s[j][base]);
+            const uint4 q1 = *reinterpret_cast<const uint4 *>(&s[j][base + 4]);
+            const uint32_t t[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int32_t c = lwx[x][k];
+                const uint32_t p = t[k];
+                acc[0] += c * (int32_t)(p & 0xff);
+                acc[1] += c * (int32_t)((p >> 8) & 0xff);
+                acc[2] += c * (int32_t)((p >> 16) & 0xff);
+                acc[3] += c * (int32_t)(p >> 24);
+                sum += c;
+            }
+        } else {
+            for (int32_t k = 0; k < flx; k++) {
+                const int32_t c = lwx[x][k];
+                const uint32_t p = s[j][base + k];
+                acc[0] += c * (int32_t)(p & 0xff);
+                acc[1] += c * (int32_t)((p >> 8) & 0xff);
+                acc[2] += c * (int32_t)((p >> 16) & 0xff);
+                acc[3] += c * (int32_t)(p >> 24);
+                sum += c;
+            }
         }
         hrow[j][x] = tent_tap_sum(acc, sum);
     }
