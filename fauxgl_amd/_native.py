@@ -29,7 +29,7 @@ EXPORTED = [
     "fgl_clear_depth", "fgl_texture_create", "fgl_texture_destroy", "fgl_draw_triangles", "fgl_draw_lines",
     "fgl_mesh_create", "fgl_mesh_destroy", "fgl_draw_mesh", "fgl_read_color", "fgl_read_depth",
     "fgl_write_color", "fgl_write_depth", "fgl_depth_image", "fgl_resolve", "fgl_profile_enable",
-    "fgl_profile_read", "fgl_profile_reset", "fgl_ctx_sync", "fgl_draw_stats",
+    "fgl_profile_read", "fgl_profile_reset", "fgl_ctx_sync", "fgl_draw_stats", "fgl_profile_only",
 ]
 
 _d = ctypes.c_double
@@ -110,6 +110,8 @@ def load():
         "fgl_profile_read": (i32, [vp, ctypes.POINTER(FglKernelTime), i32, ctypes.POINTER(i32)]),
         "fgl_profile_reset": (i32, [vp]),
         "fgl_ctx_sync": (i32, [vp]),
+        "fgl_profile_only": (i32, [vp, ctypes.c_char_p]),
+        "fgl_draw_stats": (i32, [vp, ctypes.POINTER(FglDrawStat)]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)

@@ -596,6 +596,10 @@ class Context:
     def ProfileEnable(self, on=True):
         N.check(N.load().fgl_profile_enable(self._ptr(), 1 if on else 0), self._ptr())
 
+    def ProfileOnly(self, kernel=None):
+        """Times only launches of `kernel` (None: every launch)."""
+        N.check(N.load().fgl_profile_only(self._ptr(), kernel.encode() if kernel else None), self._ptr())
+
     def ProfileReset(self):
         N.check(N.load().fgl_profile_reset(self._ptr()), self._ptr())
 

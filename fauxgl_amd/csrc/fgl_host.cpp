@@ -99,6 +99,7 @@ struct fgl_ctx {
     DrawCounters last{};
 
     bool prof = false;
+    std::string prof_only;  // time only launches of this kernel (empty: all)
     std::map<std::string, KStat> kstats;
     std::vector<EvPair> pending;
     std::vector<hipEvent_t> pool;
@@ -171,7 +172,8 @@ void prof_collect(fgl_ctx *c) {
 template <class F>
 int launch(fgl_ctx *c, const char *name, F &&f) {
     hipEvent_t a = nullptr, b = nullptr;
-    if (c->prof) {
+    const bool timed = c->prof && (c->prof_only.empty() || c->prof_only == name);
+    if (timed) {
         a = ev_get(c);
         b = ev_get(c);
         if (a) (void)hipEventRecord(a, c->stream);
@@ -181,7 +183,7 @@ int launch(fgl_ctx *c, const char *name, F &&f) {
         set_err(c->err, "launch of %s failed: %s", name, hipGetErrorString(e));
         return FGL_E_DEVICE;
     }
-    if (c->prof && a && b) {
+    if (timed && a && b) {
         (void)hipEventRecord(b, c->stream);
         c->pending.push_back(EvPair{name, a, b});
     }
@@ -992,6 +994,12 @@ int fgl_resolve(fgl_ctx *src, fgl_ctx *dst, int32_t filter) {
 int fgl_profile_enable(fgl_ctx *ctx, int32_t enable) {
     if (check_ctx(ctx)) return FGL_E_INVALID;
     ctx->prof = enable != 0;
+    return FGL_OK;
+}
+
+int fgl_profile_only(fgl_ctx *ctx, const char *kernel) {
+    if (check_ctx(ctx)) return FGL_E_INVALID;
+    ctx->prof_only = kernel ? kernel : "";
     return FGL_OK;
 }
 

@@ -187,6 +187,10 @@ int fgl_resolve(fgl_ctx *src, fgl_ctx *dst, int32_t filter);
 /* When enabled, every kernel launch on ctx is bracketed by HIP events on the
  * context's stream and its duration accumulated per kernel. */
 int fgl_profile_enable(fgl_ctx *ctx, int32_t enable);
+/* Restricts the timing to launches of one kernel (its name as reported by
+ * fgl_profile_read); NULL or "" times every launch again.  Two events per
+ * frame perturb a timed loop far less than two per launch. */
+int fgl_profile_only(fgl_ctx *ctx, const char *kernel);
 /* Copies up to `cap` records: name (NUL-terminated, 48 bytes), launch count
  * and total milliseconds.  Returns the number of kernels via *n. */
 typedef struct fgl_kernel_time {
