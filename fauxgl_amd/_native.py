@@ -30,6 +30,8 @@ EXPORTED = [
     "fgl_mesh_create", "fgl_mesh_destroy", "fgl_draw_mesh", "fgl_read_color", "fgl_read_depth",
     "fgl_write_color", "fgl_write_depth", "fgl_depth_image", "fgl_resolve", "fgl_profile_enable",
     "fgl_profile_read", "fgl_profile_reset", "fgl_ctx_sync", "fgl_draw_stats", "fgl_profile_only",
+    "fgl_nn_create", "fgl_nn_destroy", "fgl_nn_param_count", "fgl_nn_get_params", "fgl_nn_set_params",
+    "fgl_nn_infer", "fgl_nn_residual", "fgl_nn_grad", "fgl_nn_train_step",
 ]
 
 _d = ctypes.c_double

@@ -1050,3 +1050,348 @@ int fgl_ctx_sync(fgl_ctx *ctx) {
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Learned anti-aliasing: parameters, forward, backward and Adam on the
+// context's device (fgl_nn.hip).
+
+struct fgl_nn {
+    fgl_ctx *home = nullptr;  // device owner (parameters live with it)
+    int32_t cin[NN_LAYERS] = {NN_FEAT_C, NN_C, NN_C, NN_C, NN_C, NN_C};
+    int32_t cout[NN_LAYERS] = {NN_C, NN_C, NN_C, NN_C, NN_C, NN_OUT_C};
+    int64_t woff[NN_LAYERS] = {}, boff[NN_LAYERS] = {};
+    int64_t nparam = 0;
+    DevBuf p, g, m, v;                        // f32 master weights, gradient, Adam moments
+    DevBuf wf[NN_LAYERS], wt[NN_LAYERS];      // bf16 forward / flipped-transposed copies
+    DevBuf feat, act[NN_LAYERS - 1], out, gout, dy[2], dwp, dbp, lpart, range, dpart;
+    int64_t step = 0;
+    int32_t H = 0, W = 0;
+};
+
+namespace {
+
+uint64_t splitmix(uint64_t &s) {
+    uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+double gauss(uint64_t &s) {
+    const double u1 = ((double)(splitmix(s) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    const double u2 = (double)(splitmix(s) >> 11) * (1.0 / 9007199254740992.0);
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+}
+
+int nn_pack(fgl_nn *nn) {
+    fgl_ctx *c = nn->home;
+    for (int l = 0; l < NN_LAYERS; l++) {
+        int r = launch(c, "nn_pack", [&] {
+            return launch_nn_pack(nn->p.as<float>() + nn->woff[l], nn->cin[l], nn->cout[l], nn->wf[l].as<uint16_t>(),
+                                  nn->wt[l].as<uint16_t>(), c->stream);
+        });
+        if (r) return r;
+    }
+    return FGL_OK;
+}
+
+int nn_buffers(fgl_nn *nn, int32_t H, int32_t W) {
+    fgl_ctx *c = nn->home;
+    const size_t px = (size_t)H * W;
+    HIPCHK(c, nn->feat.ensure(px * NN_FEAT_C * 2));
+    for (int l = 0; l < NN_LAYERS - 1; l++) HIPCHK(c, nn->act[l].ensure(px * NN_C * 2));
+    HIPCHK(c, nn->out.ensure(px * NN_OUT_C * 4));
+    HIPCHK(c, nn->range.ensure(16));
+    HIPCHK(c, nn->dpart.ensure(512 * 16));
+    nn->H = H;
+    nn->W = W;
+    return FGL_OK;
+}
+
+// features and the six convolutions of src's color and depth
+int nn_forward(fgl_nn *nn, fgl_ctx *src) {
+    fgl_ctx *c = nn->home;
+    int r = materialize(src);
+    if (r) return r;
+    if ((r = stream_after(c, src))) return r;
+    if ((r = nn_buffers(nn, src->H, src->W))) return r;
+    const int32_t H = src->H, W = src->W;
+    const int64_t n = (int64_t)H * W;
+    if ((r = launch(c, "nn_depth_range", [&] {
+             return launch_depth_minmax(src->depth.as<double>(), n, nn->dpart.as<double>(), 512, c->stream);
+         })))
+        return r;
+    if ((r = launch(c, "nn_depth_range", [&] {
+             return launch_nn_range(nn->dpart.as<double>(), 512, nn->range.as<double>(), c->stream);
+         })))
+        return r;
+    if ((r = launch(c, "nn_features", [&] {
+             return launch_nn_features(src->color.as<uint32_t>(), src->depth.as<double>(), nn->range.as<double>(), H,
+                                       W, nn->feat.as<uint16_t>(), c->stream);
+         })))
+        return r;
+    const uint16_t *in = nn->feat.as<uint16_t>();
+    for (int l = 0; l < NN_LAYERS; l++) {
+        const bool last = l == NN_LAYERS - 1;
+        void *o = last ? nn->out.p : nn->act[l].p;
+        if ((r = launch(c, "nn_conv", [&] {
+                 return launch_nn_conv(last ? 1 : 0, in, nn->cin[l], nn->wf[l].as<uint16_t>(),
+                                       nn->p.as<float>() + nn->boff[l], H, W, nn->cout[l], o, nullptr, c->stream);
+             })))
+            return r;
+        if (!last) in = nn->act[l].as<uint16_t>();
+    }
+    return FGL_OK;
+}
+
+// loss against target's color, gradient of every parameter into nn->g
+int nn_backward(fgl_nn *nn, fgl_ctx *src, fgl_ctx *target, float *loss) {
+    fgl_ctx *c = nn->home;
+    int r = materialize(target);
+    if (r) return r;
+    if ((r = stream_after(c, target))) return r;
+    const int32_t H = nn->H, W = nn->W;
+    const size_t px = (size_t)H * W;
+    const int32_t nlb = 256;
+    HIPCHK(c, nn->gout.ensure(px * NN_OUT_C * 2));
+    HIPCHK(c, nn->dy[0].ensure(px * NN_C * 2));
+    HIPCHK(c, nn->dy[1].ensure(px * NN_C * 2));
+    HIPCHK(c, nn->lpart.ensure(nlb * 4));
+    const int64_t nw = nn_dw_waves(H, W);
+    HIPCHK(c, nn->dwp.ensure((size_t)nw * 9 * NN_C * NN_C * 4));
+    HIPCHK(c, nn->dbp.ensure((size_t)nw * NN_C * 4));
+    if ((r = launch(c, "nn_loss", [&] {
+             return launch_nn_loss(nn->out.as<float>(), src->color.as<uint32_t>(), target->color.as<uint32_t>(), H, W,
+                                   nn->gout.as<uint16_t>(), nn->lpart.as<float>(), nlb, c->stream);
+         })))
+        return r;
+    const uint16_t *dy = nn->gout.as<uint16_t>();
+    for (int l = NN_LAYERS - 1; l >= 0; l--) {
+        const uint16_t *x = l == 0 ? nn->feat.as<uint16_t>() : nn->act[l - 1].as<uint16_t>();
+        const int32_t ci = nn->cin[l], co = nn->cout[l];
+        if ((r = launch(c, "nn_dw", [&] {
+                 return launch_nn_dw(x, ci, dy, co, H, W, nn->dwp.as<float>(), nn->dbp.as<float>(), c->stream);
+             })))
+            return r;
+        if ((r = launch(c, "nn_reduce", [&] {
+                 return launch_nn_reduce(nn->dwp.as<float>(), nw, 9LL * co * ci, nn->g.as<float>() + nn->woff[l],
+                                         c->stream);
+             })))
+            return r;
+        if ((r = launch(c, "nn_reduce", [&] {
+                 return launch_nn_reduce(nn->dbp.as<float>(), nw, co, nn->g.as<float>() + nn->boff[l], c->stream);
+             })))
+            return r;
+        if (l == 0) break;
+        uint16_t *dx = nn->dy[l & 1].as<uint16_t>();
+        if ((r = launch(c, "nn_conv_dx", [&] {
+                 return launch_nn_conv(2, dy, co, nn->wt[l].as<uint16_t>(), nullptr, H, W, ci, dx, x, c->stream);
+             })))
+            return r;
+        dy = dx;
+    }
+    std::vector<float> lp(nlb);
+    HIPCHK(c, hipMemcpyAsync(lp.data(), nn->lpart.p, nlb * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->prof) prof_collect(c);
+    double s = 0;
+    for (float x : lp) s += x;
+    if (loss) *loss = (float)s;
+    return FGL_OK;
+}
+
+int check_nn(fgl_nn *nn) {
+    if (!nn || !nn->home) {
+        g_err = "null network";
+        return FGL_E_INVALID;
+    }
+    return FGL_OK;
+}
+
+int check_same_size(fgl_ctx *a, fgl_ctx *b, const char *what) {
+    if (a->W != b->W || a->H != b->H) {
+        set_err(a->err, "%s: contexts differ in size (%dx%d vs %dx%d)", what, a->W, a->H, b->W, b->H);
+        return FGL_E_INVALID;
+    }
+    if (a->device != b->device) {
+        set_err(a->err, "%s: contexts on different devices", what);
+        return FGL_E_INVALID;
+    }
+    return FGL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fgl_nn_create(fgl_ctx *ctx, uint64_t seed, fgl_nn **out) {
+    if (check_ctx(ctx) || !out) return FGL_E_INVALID;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    fgl_nn *nn = new fgl_nn;
+    nn->home = ctx;
+    int64_t off = 0;
+    for (int l = 0; l < NN_LAYERS; l++) {
+        nn->woff[l] = off;
+        off += 9LL * nn->cin[l] * nn->cout[l];
+        nn->boff[l] = off;
+        off += nn->cout[l];
+    }
+    nn->nparam = off;
+    std::vector<float> h((size_t)off, 0.0f);
+    uint64_t s = seed;
+    for (int l = 0; l < NN_LAYERS; l++) {
+        const int real_in = l == 0 ? 8 : nn->cin[l];
+        const int real_out = l == NN_LAYERS - 1 ? 3 : nn->cout[l];
+        const double sd = std::sqrt(2.0 / (9.0 * real_in)) * (l == NN_LAYERS - 1 ? 0.1 : 1.0);
+        for (int tap = 0; tap < 9; tap++)
+            for (int co = 0; co < nn->cout[l]; co++)
+                for (int ci = 0; ci < nn->cin[l]; ci++) {
+                    const double gv = gauss(s);
+                    if (co < real_out && ci < real_in)
+                        h[(size_t)(nn->woff[l] + ((int64_t)tap * nn->cout[l] + co) * nn->cin[l] + ci)] = (float)(gv * sd);
+                }
+    }
+    int r = FGL_OK;
+    auto fail = [&](const char *m) {
+        set_err(ctx->err, "fgl_nn_create: %s", m);
+        delete nn;
+        return FGL_E_NOMEM;
+    };
+    if (nn->p.ensure(off * 4) || nn->g.ensure(off * 4) || nn->m.ensure(off * 4) || nn->v.ensure(off * 4))
+        return fail("device allocation failed");
+    for (int l = 0; l < NN_LAYERS; l++)
+        if (nn->wf[l].ensure(9LL * nn->cin[l] * nn->cout[l] * 2) || nn->wt[l].ensure(9LL * nn->cin[l] * nn->cout[l] * 2))
+            return fail("device allocation failed");
+    if (hipMemcpyAsync(nn->p.p, h.data(), off * 4, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+        hipMemsetAsync(nn->g.p, 0, off * 4, ctx->stream) != hipSuccess ||
+        hipMemsetAsync(nn->m.p, 0, off * 4, ctx->stream) != hipSuccess ||
+        hipMemsetAsync(nn->v.p, 0, off * 4, ctx->stream) != hipSuccess) {
+        set_err(ctx->err, "fgl_nn_create: upload failed");
+        delete nn;
+        return FGL_E_DEVICE;
+    }
+    if ((r = nn_pack(nn))) {
+        delete nn;
+        return r;
+    }
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        set_err(ctx->err, "fgl_nn_create: synchronize failed");
+        delete nn;
+        return FGL_E_DEVICE;
+    }
+    *out = nn;
+    return FGL_OK;
+}
+
+void fgl_nn_destroy(fgl_nn *nn) {
+    if (!nn) return;
+    if (nn->home) (void)hipStreamSynchronize(nn->home->stream);
+    delete nn;
+}
+
+int fgl_nn_param_count(const fgl_nn *nn, int64_t *n) {
+    if (!nn || !n) return FGL_E_INVALID;
+    *n = nn->nparam;
+    return FGL_OK;
+}
+
+int fgl_nn_get_params(fgl_nn *nn, float *host, int64_t n) {
+    if (check_nn(nn) || !host || n != nn->nparam) return FGL_E_INVALID;
+    fgl_ctx *c = nn->home;
+    HIPCHK(c, hipMemcpyAsync(host, nn->p.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FGL_OK;
+}
+
+int fgl_nn_set_params(fgl_nn *nn, const float *host, int64_t n) {
+    if (check_nn(nn) || !host || n != nn->nparam) return FGL_E_INVALID;
+    fgl_ctx *c = nn->home;
+    HIPCHK(c, hipMemcpyAsync(nn->p.p, host, n * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(nn->m.p, 0, n * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(nn->v.p, 0, n * 4, c->stream));
+    nn->step = 0;
+    int r = nn_pack(nn);
+    if (r) return r;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FGL_OK;
+}
+
+int fgl_nn_infer(fgl_nn *nn, fgl_ctx *src, fgl_ctx *dst) {
+    if (check_nn(nn) || check_ctx(src) || check_ctx(dst)) return FGL_E_INVALID;
+    fgl_ctx *c = nn->home;
+    int r = check_same_size(src, dst, "fgl_nn_infer");
+    if (r) return r;
+    if (src->device != c->device) {
+        set_err(src->err, "fgl_nn_infer: context and network on different devices");
+        return FGL_E_INVALID;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    if ((r = nn_forward(nn, src))) return r;
+    if ((r = stream_after(c, dst))) return r;
+    if ((r = launch(c, "nn_output", [&] {
+             return launch_nn_output(nn->out.as<float>(), src->color.as<uint32_t>(), src->H, src->W,
+                                     dst->color.as<uint32_t>(), c->stream);
+         })))
+        return r;
+    if ((r = stream_after(dst, c))) return r;
+    if ((r = stream_after(src, c))) return r;
+    dst->pend_color = false;
+    return FGL_OK;
+}
+
+int fgl_nn_residual(fgl_nn *nn, fgl_ctx *src, float *host, int64_t n) {
+    if (check_nn(nn) || check_ctx(src) || !host) return FGL_E_INVALID;
+    fgl_ctx *c = nn->home;
+    if (src->device != c->device || n != (int64_t)src->W * src->H * NN_OUT_C) {
+        set_err(src->err, "fgl_nn_residual: needs %lld floats on the network's device",
+                (long long)src->W * src->H * NN_OUT_C);
+        return FGL_E_INVALID;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    int r = nn_forward(nn, src);
+    if (r) return r;
+    HIPCHK(c, hipMemcpyAsync(host, nn->out.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FGL_OK;
+}
+
+int fgl_nn_grad(fgl_nn *nn, fgl_ctx *src, fgl_ctx *target, float *loss, float *grad, int64_t n) {
+    if (check_nn(nn) || check_ctx(src) || check_ctx(target)) return FGL_E_INVALID;
+    int r = check_same_size(src, target, "fgl_nn_grad");
+    if (r) return r;
+    fgl_ctx *c = nn->home;
+    if (src->device != c->device) {
+        set_err(src->err, "fgl_nn_grad: context and network on different devices");
+        return FGL_E_INVALID;
+    }
+    if (grad && n != nn->nparam) {
+        set_err(src->err, "fgl_nn_grad: gradient buffer holds %lld floats, the network %lld", (long long)n,
+                (long long)nn->nparam);
+        return FGL_E_INVALID;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    if ((r = nn_forward(nn, src))) return r;
+    if ((r = nn_backward(nn, src, target, loss))) return r;
+    if (grad) {
+        HIPCHK(c, hipMemcpyAsync(grad, nn->g.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return FGL_OK;
+}
+
+int fgl_nn_train_step(fgl_nn *nn, fgl_ctx *src, fgl_ctx *target, float lr, float *loss) {
+    int r = fgl_nn_grad(nn, src, target, loss, nullptr, 0);
+    if (r) return r;
+    fgl_ctx *c = nn->home;
+    nn->step++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.0f - (float)std::pow((double)b1, (double)nn->step);
+    const float c2 = 1.0f - (float)std::pow((double)b2, (double)nn->step);
+    if ((r = launch(c, "nn_adam", [&] {
+             return launch_nn_adam(nn->p.as<float>(), nn->g.as<float>(), nn->m.as<float>(), nn->v.as<float>(),
+                                   nn->nparam, lr, b1, b2, 1e-8f, c1, c2, c->stream);
+         })))
+        return r;
+    return nn_pack(nn);
+}
+
+}  // extern "C"

@@ -216,6 +216,33 @@ typedef struct fgl_draw_stat {
 } fgl_draw_stat;
 int fgl_draw_stats(fgl_ctx *ctx, fgl_draw_stat *out);
 
+/* ---- learned anti-aliasing (no reference counterpart: it replaces the
+ * 16x SSAA render + resolve that the reference pipeline would run) ----
+ *
+ * A six-layer 3x3 convolutional network (BF16 MFMA, FP32 accumulate) maps
+ * a 1x render's color, depth and depth-derived normals to an estimate of
+ * the 4x4-supersampled image.  Parameters are FP32, laid out per layer as
+ * weights [9][Cout][Cin] then bias [Cout], layers 16->32, 32->32 (x4),
+ * 32->16 (input channels 8..15 and output channels 3..15 are padding). */
+typedef struct fgl_nn fgl_nn;
+/* Allocates the network on ctx's device with seeded He initialisation. */
+int fgl_nn_create(fgl_ctx *ctx, uint64_t seed, fgl_nn **out);
+void fgl_nn_destroy(fgl_nn *nn);
+int fgl_nn_param_count(const fgl_nn *nn, int64_t *n);
+int fgl_nn_get_params(fgl_nn *nn, float *params, int64_t n);
+/* Also resets the optimiser state. */
+int fgl_nn_set_params(fgl_nn *nn, const float *params, int64_t n);
+/* dst color = network(src); src and dst have the same size. */
+int fgl_nn_infer(fgl_nn *nn, fgl_ctx *src, fgl_ctx *dst);
+/* The network's raw output for src: H*W*16 floats, pixel-major; channels
+ * 0..2 are the residual added to the input color (in units of 255). */
+int fgl_nn_residual(fgl_nn *nn, fgl_ctx *src, float *out, int64_t n);
+/* Mean squared error of network(src) against target's color (same size)
+ * and, when grad is not NULL, the gradient of every parameter (n floats). */
+int fgl_nn_grad(fgl_nn *nn, fgl_ctx *src, fgl_ctx *target, float *loss, float *grad, int64_t n);
+/* One Adam step (beta 0.9 / 0.999) on that loss. */
+int fgl_nn_train_step(fgl_nn *nn, fgl_ctx *src, fgl_ctx *target, float lr, float *loss);
+
 #ifdef __cplusplus
 }
 #endif
