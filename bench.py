@@ -92,6 +92,70 @@ def gpu_frames(mesh, W, H, ssaa, steps, warmup):
     return ms, info, table, img, stats, dom, live
 
 
+def nn_frames(mesh, W, H, steps, warmup):
+    """1x draw followed by the learned anti-aliasing pass (fauxgl_amd.nnaa
+    with the committed weights): the frame the network's output replaces the
+    16x SSAA frame with."""
+    from fauxgl_amd import api as A
+    from fauxgl_amd import meshes as M
+    from fauxgl_amd import nnaa
+    lo = A.NewContext(W, H)
+    out = A.NewContext(W, H)
+    lo.Shader = M.phong_shader(W, H)
+    dm = lo.Upload(mesh)
+    net = nnaa.Network(lo, weights=nnaa.DEFAULT_WEIGHTS)
+
+    def frame():
+        lo.ClearColorBufferWith(M.BACKGROUND)
+        lo.ClearDepthBuffer()
+        lo.DrawMesh(dm)
+        net.Infer(lo, out)
+
+    for _ in range(warmup):
+        frame()
+    lo.Sync()
+    lo.ProfileReset()
+    lo.ProfileEnable(True)
+    for _ in range(PROFILE_FRAMES):
+        frame()
+    lo.Sync()
+    table = lo.ProfileRead()
+    lo.ProfileEnable(False)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        frame()
+    out.Sync()
+    lo.Sync()
+    ms = (time.perf_counter() - t0) * 1000.0 / steps
+    img = out.Image()
+    net.close()
+    dm.close()
+    return ms, img, table
+
+
+def psnr(a, b):
+    d = a[..., :3].astype(np.float64) - b[..., :3].astype(np.float64)
+    mse = float(np.mean(d * d))
+    return float("inf") if mse == 0 else 10.0 * np.log10(255.0 * 255.0 / mse)
+
+
+def ssim(a, b):
+    """Mean SSIM over R, G, B (Gaussian window sigma 1.5, K1 0.01, K2 0.03,
+    L 255; Wang et al. 2004)."""
+    from scipy.ndimage import gaussian_filter
+    c1, c2 = (0.01 * 255) ** 2, (0.03 * 255) ** 2
+    vals = []
+    for ch in range(3):
+        x = a[..., ch].astype(np.float64)
+        y = b[..., ch].astype(np.float64)
+        f = lambda z: gaussian_filter(z, 1.5, truncate=3.5)
+        mx, my = f(x), f(y)
+        sxx, syy, sxy = f(x * x) - mx * mx, f(y * y) - my * my, f(x * y) - mx * my
+        m = ((2 * mx * my + c1) * (2 * sxy + c2)) / ((mx * mx + my * my + c1) * (sxx + syy + c2))
+        vals.append(float(m.mean()))
+    return float(np.mean(vals))
+
+
 def cpu_frames(mesh, W, H, ssaa, frames, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -199,6 +263,24 @@ def main():
     log(f"[bench] 16x SSAA: {ms16:.3f} ms/frame, info {tuple(info16)}")
     ms1, info1, prof1, img1, st1, _, _ = gpu_frames(mesh, W, H, 1, args.steps, args.warmup)
     log(f"[bench] 1x: {ms1:.3f} ms/frame, info {tuple(info1)}")
+    ms4, _, _, img4, _, _, _ = gpu_frames(mesh, W, H, 4, args.steps, args.warmup)
+    log(f"[bench] 4x SSAA: {ms4:.3f} ms/frame")
+    msn, imgn, profn = nn_frames(mesh, W, H, args.steps, args.warmup)
+    log(f"[bench] 1x + neural AA: {msn:.3f} ms/frame")
+    nn_kernels = {k: {"launches": v[0] // PROFILE_FRAMES, "ms_per_frame": round(v[1] / PROFILE_FRAMES, 4)}
+                  for k, v in sorted(profn.items()) if k.startswith("nn_")}
+    neural_aa = {
+        "ms_per_frame": round(msn, 4), "frames_per_s": round(1000.0 / msn, 3),
+        "speedup_vs_16x": round(ms16 / msn, 3),
+        "psnr_db_vs_16x": round(psnr(imgn, img16), 3), "ssim_vs_16x": round(ssim(imgn, img16), 5),
+        "psnr_db_1x_vs_16x": round(psnr(img1, img16), 3), "ssim_1x_vs_16x": round(ssim(img1, img16), 5),
+        "psnr_db_4x_ssaa_vs_16x": round(psnr(img4, img16), 3), "ms_per_frame_4x_ssaa": round(ms4, 4),
+        "nn_ms_per_frame": round(sum(v["ms_per_frame"] for v in nn_kernels.values()), 4),
+        "nn_kernels": nn_kernels,
+        "network": "6 x (3x3 conv), 32 channels, ReLU, BF16 operands, FP32 accumulation (MFMA 16x16x32)",
+        "weights": "fauxgl_amd/nnaa_weights.npy (trained by fauxgl_amd/train_nnaa.py on engine renders; "
+                   "this camera is not a training view)",
+    }
 
     # dominant kernel of the headline step and its roofline
     per_kernel = {k: {"launches": v[0], "avg_ms": v[1] / max(v[0], 1), "ms_per_frame": v[1] / PROFILE_FRAMES}
@@ -264,6 +346,7 @@ def main():
         "roofline": roofline,
         "roofline_frame": roofline_frame,
         "cpu_baseline": cpu,
+        "neural_aa": neural_aa,
     }
     if cpu is not None:
         line["speedup_vs_cpu_16x"] = round(fps16 / cpu["value"], 1)

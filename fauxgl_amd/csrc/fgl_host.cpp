@@ -1063,7 +1063,7 @@ struct fgl_nn {
     int64_t nparam = 0;
     DevBuf p, g, m, v;                        // f32 master weights, gradient, Adam moments
     DevBuf wf[NN_LAYERS], wt[NN_LAYERS];      // bf16 forward / flipped-transposed copies
-    DevBuf feat, act[NN_LAYERS - 1], out, gout, dy[2], dwp, dbp, lpart, range, dpart;
+    DevBuf feat, act[NN_LAYERS - 1], out, gout, dy[2], dwp, dbp, rtmp, lpart, range, dpart;
     int64_t step = 0;
     int32_t H = 0, W = 0;
 };
@@ -1107,8 +1107,11 @@ int nn_buffers(fgl_nn *nn, int32_t H, int32_t W) {
     return FGL_OK;
 }
 
-// features and the six convolutions of src's color and depth
-int nn_forward(fgl_nn *nn, fgl_ctx *src) {
+// features and the six convolutions of src's color and depth; the last
+// layer writes the raw output (nn->out) or, with dst, the final color.
+// keep_feat: the features go through nn->feat (the weight gradient of
+// layer 0 reads them); otherwise layer 0 computes them as it stages tiles.
+int nn_forward(fgl_nn *nn, fgl_ctx *src, fgl_ctx *dst, bool keep_feat) {
     fgl_ctx *c = nn->home;
     int r = materialize(src);
     if (r) return r;
@@ -1116,7 +1119,7 @@ int nn_forward(fgl_nn *nn, fgl_ctx *src) {
     if ((r = nn_buffers(nn, src->H, src->W))) return r;
     const int32_t H = src->H, W = src->W;
     const int64_t n = (int64_t)H * W;
-    if ((r = launch(c, "nn_depth_range", [&] {
+    if ((r = launch(c, "nn_depth_minmax", [&] {
              return launch_depth_minmax(src->depth.as<double>(), n, nn->dpart.as<double>(), 512, c->stream);
          })))
         return r;
@@ -1124,18 +1127,23 @@ int nn_forward(fgl_nn *nn, fgl_ctx *src) {
              return launch_nn_range(nn->dpart.as<double>(), 512, nn->range.as<double>(), c->stream);
          })))
         return r;
-    if ((r = launch(c, "nn_features", [&] {
-             return launch_nn_features(src->color.as<uint32_t>(), src->depth.as<double>(), nn->range.as<double>(), H,
-                                       W, nn->feat.as<uint16_t>(), c->stream);
-         })))
+    if (keep_feat && (r = launch(c, "nn_features", [&] {
+                          return launch_nn_features(src->color.as<uint32_t>(), src->depth.as<double>(),
+                                                    nn->range.as<double>(), H, W, nn->feat.as<uint16_t>(), c->stream);
+                      })))
         return r;
-    const uint16_t *in = nn->feat.as<uint16_t>();
+    const uint16_t *in = keep_feat ? nn->feat.as<uint16_t>() : nullptr;
+    if (dst && (r = stream_after(c, dst))) return r;
     for (int l = 0; l < NN_LAYERS; l++) {
         const bool last = l == NN_LAYERS - 1;
-        void *o = last ? nn->out.p : nn->act[l].p;
-        if ((r = launch(c, "nn_conv", [&] {
-                 return launch_nn_conv(last ? 1 : 0, in, nn->cin[l], nn->wf[l].as<uint16_t>(),
-                                       nn->p.as<float>() + nn->boff[l], H, W, nn->cout[l], o, nullptr, c->stream);
+        void *o = last ? (dst ? dst->color.p : nn->out.p) : nn->act[l].p;
+        const int epi = last ? (dst ? 3 : 1) : 0;
+        static const char *const kConvName[NN_LAYERS] = {"nn_conv0", "nn_conv1", "nn_conv2",
+                                                          "nn_conv3", "nn_conv4", "nn_conv5"};
+        if ((r = launch(c, kConvName[l], [&] {
+                 return launch_nn_conv(epi, in, nn->cin[l], nn->wf[l].as<uint16_t>(), nn->p.as<float>() + nn->boff[l],
+                                       H, W, nn->cout[l], o, nullptr, src->color.as<uint32_t>(),
+                                       src->depth.as<double>(), nn->range.as<double>(), c->stream);
              })))
             return r;
         if (!last) in = nn->act[l].as<uint16_t>();
@@ -1159,6 +1167,7 @@ int nn_backward(fgl_nn *nn, fgl_ctx *src, fgl_ctx *target, float *loss) {
     const int64_t nw = nn_dw_waves(H, W);
     HIPCHK(c, nn->dwp.ensure((size_t)nw * 9 * NN_C * NN_C * 4));
     HIPCHK(c, nn->dbp.ensure((size_t)nw * NN_C * 4));
+    HIPCHK(c, nn->rtmp.ensure((size_t)NN_REDUCE_SPLIT * 9 * NN_C * NN_C * 4));
     if ((r = launch(c, "nn_loss", [&] {
              return launch_nn_loss(nn->out.as<float>(), src->color.as<uint32_t>(), target->color.as<uint32_t>(), H, W,
                                    nn->gout.as<uint16_t>(), nn->lpart.as<float>(), nlb, c->stream);
@@ -1174,17 +1183,19 @@ int nn_backward(fgl_nn *nn, fgl_ctx *src, fgl_ctx *target, float *loss) {
             return r;
         if ((r = launch(c, "nn_reduce", [&] {
                  return launch_nn_reduce(nn->dwp.as<float>(), nw, 9LL * co * ci, nn->g.as<float>() + nn->woff[l],
-                                         c->stream);
+                                         nn->rtmp.as<float>(), c->stream);
              })))
             return r;
         if ((r = launch(c, "nn_reduce", [&] {
-                 return launch_nn_reduce(nn->dbp.as<float>(), nw, co, nn->g.as<float>() + nn->boff[l], c->stream);
+                 return launch_nn_reduce(nn->dbp.as<float>(), nw, co, nn->g.as<float>() + nn->boff[l],
+                                         nn->rtmp.as<float>(), c->stream);
              })))
             return r;
         if (l == 0) break;
         uint16_t *dx = nn->dy[l & 1].as<uint16_t>();
         if ((r = launch(c, "nn_conv_dx", [&] {
-                 return launch_nn_conv(2, dy, co, nn->wt[l].as<uint16_t>(), nullptr, H, W, ci, dx, x, c->stream);
+                 return launch_nn_conv(2, dy, co, nn->wt[l].as<uint16_t>(), nullptr, H, W, ci, dx, x, nullptr,
+                                       nullptr, nullptr, c->stream);
              })))
             return r;
         dy = dx;
@@ -1325,13 +1336,7 @@ int fgl_nn_infer(fgl_nn *nn, fgl_ctx *src, fgl_ctx *dst) {
         return FGL_E_INVALID;
     }
     HIPCHK(c, hipSetDevice(c->device));
-    if ((r = nn_forward(nn, src))) return r;
-    if ((r = stream_after(c, dst))) return r;
-    if ((r = launch(c, "nn_output", [&] {
-             return launch_nn_output(nn->out.as<float>(), src->color.as<uint32_t>(), src->H, src->W,
-                                     dst->color.as<uint32_t>(), c->stream);
-         })))
-        return r;
+    if ((r = nn_forward(nn, src, dst, false))) return r;
     if ((r = stream_after(dst, c))) return r;
     if ((r = stream_after(src, c))) return r;
     dst->pend_color = false;
@@ -1347,7 +1352,7 @@ int fgl_nn_residual(fgl_nn *nn, fgl_ctx *src, float *host, int64_t n) {
         return FGL_E_INVALID;
     }
     HIPCHK(c, hipSetDevice(c->device));
-    int r = nn_forward(nn, src);
+    int r = nn_forward(nn, src, nullptr, false);
     if (r) return r;
     HIPCHK(c, hipMemcpyAsync(host, nn->out.p, n * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1369,7 +1374,7 @@ int fgl_nn_grad(fgl_nn *nn, fgl_ctx *src, fgl_ctx *target, float *loss, float *g
         return FGL_E_INVALID;
     }
     HIPCHK(c, hipSetDevice(c->device));
-    if ((r = nn_forward(nn, src))) return r;
+    if ((r = nn_forward(nn, src, nullptr, true))) return r;
     if ((r = nn_backward(nn, src, target, loss))) return r;
     if (grad) {
         HIPCHK(c, hipMemcpyAsync(grad, nn->g.p, n * 4, hipMemcpyDeviceToHost, c->stream));

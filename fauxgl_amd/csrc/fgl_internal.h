@@ -120,22 +120,24 @@ hipError_t launch_depth_image(const double *depth, int64_t n, const double *part
 constexpr int NN_C = 32;         // hidden channels (widest layer)
 constexpr int NN_FEAT_C = 16;    // input feature channels (8 used)
 constexpr int NN_OUT_C = 16;     // output channels (3 used)
-constexpr int NN_DW_ROWS = 8;    // rows of 16 pixels per weight-gradient wave
+constexpr int NN_DW_ROWS = 32;   // rows of 16 pixels per weight-gradient wave
+constexpr int NN_REDUCE_SPLIT = 64;  // first-pass rows of the partial-sum reduction
 constexpr int NN_LAYERS = 6;
 
 hipError_t launch_nn_range(const double *partial, int32_t nblocks, double *range, hipStream_t s);
 hipError_t launch_nn_features(const uint32_t *color, const double *depth, const double *range, int32_t H, int32_t W,
                               uint16_t *feat, hipStream_t s);
+// in == nullptr: layer 0 with the features computed from color, depth and
+// range (Cin 16)
 hipError_t launch_nn_conv(int epi, const uint16_t *in, int32_t Cin, const uint16_t *w, const float *bias, int32_t H,
-                          int32_t W, int32_t Cout, void *out, const uint16_t *mask, hipStream_t s);
+                          int32_t W, int32_t Cout, void *out, const uint16_t *mask, const uint32_t *color,
+                          const double *depth, const double *range, hipStream_t s);
 int64_t nn_dw_waves(int32_t H, int32_t W);
 hipError_t launch_nn_dw(const uint16_t *x, int32_t Cin, const uint16_t *dy, int32_t Cout, int32_t H, int32_t W,
                         float *partial, float *bpartial, hipStream_t s);
-hipError_t launch_nn_reduce(const float *part, int64_t n, int64_t len, float *out, hipStream_t s);
+hipError_t launch_nn_reduce(const float *part, int64_t n, int64_t len, float *out, float *tmp, hipStream_t s);
 hipError_t launch_nn_loss(const float *out, const uint32_t *color, const uint32_t *target, int32_t H, int32_t W,
                           uint16_t *gout, float *lpart, int32_t nblocks, hipStream_t s);
-hipError_t launch_nn_output(const float *out, const uint32_t *color, int32_t H, int32_t W, uint32_t *dst,
-                            hipStream_t s);
 hipError_t launch_nn_adam(float *p, const float *g, float *m, float *v, int64_t n, float lr, float b1, float b2,
                           float eps, float c1, float c2, hipStream_t s);
 hipError_t launch_nn_pack(const float *wf32, int32_t Cin, int32_t Cout, uint16_t *wf, uint16_t *wt, hipStream_t s);

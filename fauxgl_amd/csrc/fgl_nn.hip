@@ -35,16 +35,23 @@ constexpr double kMaxF64 = 1.79769313486231570814527423731704356798070e+308;
 // normalisation; 1 where cleared), the normal of the depth surface from
 // central differences (nx, ny, nz), coverage mask; channels 8..15 zero.
 
-__global__ void k_nn_range(const double *partial, int32_t nblocks, double *range) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// one wave: min / max over the depth kernel's per-block partials
+__global__ __launch_bounds__(64) void k_nn_range(const double *partial, int32_t nblocks, double *range) {
     double lo = kMaxF64, hi = -kMaxF64;
-    for (int32_t b = 0; b < nblocks; b++) {
+    for (int32_t b = threadIdx.x; b < nblocks; b += 64) {
         const double l = partial[2 * b], h = partial[2 * b + 1];
-        if (l < lo) lo = l;
-        if (h > hi) hi = h;
+        lo = l < lo ? l : lo;
+        hi = h > hi ? h : hi;
     }
-    range[0] = lo;
-    range[1] = hi;
+    for (int off = 32; off > 0; off >>= 1) {
+        const double l = __shfl_xor(lo, off), h = __shfl_xor(hi, off);
+        lo = l < lo ? l : lo;
+        hi = h > hi ? h : hi;
+    }
+    if (threadIdx.x == 0) {
+        range[0] = lo;
+        range[1] = hi;
+    }
 }
 
 FGL_D float depth_t(const double *depth, int64_t i, double lo, double hi) {
@@ -53,107 +60,260 @@ FGL_D float depth_t(const double *depth, int64_t i, double lo, double hi) {
     return (float)((d - lo) / (hi - lo));
 }
 
+// the 8 feature channels of pixel (x, y) as bf16 pairs
+FGL_D uint4 pixel_features(const uint32_t *color, const double *depth, double lo, double hi, int32_t H, int32_t W,
+                           int32_t x, int32_t y) {
+    const int64_t i = (int64_t)y * W + x;
+    const uint32_t c = color[i];
+    const double d = depth[i];
+    const float t = depth_t(depth, i, lo, hi);
+    const int32_t xl = x > 0 ? x - 1 : x, xr = x + 1 < W ? x + 1 : x;
+    const int32_t yu = y > 0 ? y - 1 : y, yd = y + 1 < H ? y + 1 : y;
+    const float gx = depth_t(depth, (int64_t)y * W + xr, lo, hi) - depth_t(depth, (int64_t)y * W + xl, lo, hi);
+    const float gy = depth_t(depth, (int64_t)yd * W + x, lo, hi) - depth_t(depth, (int64_t)yu * W + x, lo, hi);
+    const float sx = -64.0f * gx, sy = -64.0f * gy;
+    const float rn = 1.0f / sqrtf(sx * sx + sy * sy + 1.0f);
+    uint4 f;
+    f.x = f2bf((float)(c & 0xff) * (1.0f / 255.0f)) | ((uint32_t)f2bf((float)((c >> 8) & 0xff) * (1.0f / 255.0f)) << 16);
+    f.y = f2bf((float)((c >> 16) & 0xff) * (1.0f / 255.0f)) | ((uint32_t)f2bf(t) << 16);
+    f.z = f2bf(sx * rn) | ((uint32_t)f2bf(sy * rn) << 16);
+    f.w = f2bf(rn) | ((uint32_t)f2bf(d == kMaxF64 ? 0.0f : 1.0f) << 16);
+    return f;
+}
+
 __global__ __launch_bounds__(256) void k_nn_features(const uint32_t *color, const double *depth, const double *range,
                                                      int32_t H, int32_t W, uint16_t *feat) {
     const int64_t n = (int64_t)W * H;
     const double lo = range[0], hi = range[1];
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const int32_t x = (int32_t)(i % W), y = (int32_t)(i / W);
-        const uint32_t c = color[i];
-        const float t = depth_t(depth, i, lo, hi);
-        const int32_t xl = x > 0 ? x - 1 : x, xr = x + 1 < W ? x + 1 : x;
-        const int32_t yu = y > 0 ? y - 1 : y, yd = y + 1 < H ? y + 1 : y;
-        const float gx = depth_t(depth, (int64_t)y * W + xr, lo, hi) - depth_t(depth, (int64_t)y * W + xl, lo, hi);
-        const float gy = depth_t(depth, (int64_t)yd * W + x, lo, hi) - depth_t(depth, (int64_t)yu * W + x, lo, hi);
-        const float sx = -64.0f * gx, sy = -64.0f * gy;
-        const float rn = 1.0f / sqrtf(sx * sx + sy * sy + 1.0f);
-        float f[8];
-        f[0] = (float)(c & 0xff) * (1.0f / 255.0f);
-        f[1] = (float)((c >> 8) & 0xff) * (1.0f / 255.0f);
-        f[2] = (float)((c >> 16) & 0xff) * (1.0f / 255.0f);
-        f[3] = t;
-        f[4] = sx * rn;
-        f[5] = sy * rn;
-        f[6] = rn;
-        f[7] = depth[i] == kMaxF64 ? 0.0f : 1.0f;
-        uint4 lo4, hi4 = make_uint4(0, 0, 0, 0);
-        lo4.x = f2bf(f[0]) | ((uint32_t)f2bf(f[1]) << 16);
-        lo4.y = f2bf(f[2]) | ((uint32_t)f2bf(f[3]) << 16);
-        lo4.z = f2bf(f[4]) | ((uint32_t)f2bf(f[5]) << 16);
-        lo4.w = f2bf(f[6]) | ((uint32_t)f2bf(f[7]) << 16);
         uint4 *o = reinterpret_cast<uint4 *>(feat + i * NN_FEAT_C);
-        o[0] = lo4;
-        o[1] = hi4;
+        o[0] = pixel_features(color, depth, lo, hi, H, W, x, y);
+        o[1] = make_uint4(0, 0, 0, 0);
     }
 }
 
 // ---------------------------------------------------------------------------
-// 3x3 convolution, zero padding.  in: H x W x Cin (bf16), w: 9 x Cout x Cin
-// (bf16, tap = 3 * (dy + 1) + (dx + 1)), Cin and Cout multiples of 16,
-// Cout <= NN_C.  EPI 0: + bias, ReLU -> bf16; 1: + bias -> f32;
-// 2: times (mask > 0) -> bf16 (the ReLU derivative of the layer below).
+// 3x3 convolution, zero padding.  in: H x W x CIN (bf16), w: 9 x COUT x CIN
+// (bf16, tap = 3 * (dy + 1) + (dx + 1)).  EPI 0: + bias, ReLU -> bf16;
+// 1: + bias -> f32; 2: times (mask > 0) -> bf16 (the ReLU derivative of the
+// layer below); 3: + bias, added to the input color -> NRGBA8 (inference
+// output; COUT 16, channels 0..2 used).
+//
+// A workgroup (4 waves) computes CT_W x CT_H pixel tiles.  The input tile
+// with its one-pixel halo is staged in LDS (CT_S bytes per pixel: 64 of
+// channels, padded), 32 channels wide (a 16-channel input is
+// zero-extended).  Each wave keeps every weight fragment of the layer in
+// registers and runs v_mfma_f32_16x16x32_bf16 with A = weights (16 output
+// channels x 32 input channels of one tap) and B = input (32 channels x 16
+// pixels), so that the accumulator holds 4 consecutive output channels of
+// one pixel per lane.  For COUT 32 the rows of tile t are channel
+// 8 * (m / 4) + 4 * t + m % 4, which gives every lane 8 consecutive
+// channels: one 16-byte store.
+//
+// The workgroups are persistent (two per CU): the grid is 8 * g
+// workgroups, workgroup b runs on XCD b % 8 (round-robin dispatch) and
+// walks the tiles of band b % 8, a contiguous eighth of the tile rows, so
+// that the halo rows two tiles share go through one L2.  The next tile's
+// input is loaded into registers before the current tile is computed and
+// written to LDS after it, so that the loads' latency hides behind the
+// matrix work.  The weights and bias reach their registers through LDS, so
+// that no register the tile loop reads waits on a global load.
 
-template <int EPI>
-__global__ __launch_bounds__(256) void k_nn_conv(const uint16_t *in, int32_t Cin, const uint16_t *w,
-                                                 const float *bias, int32_t H, int32_t W, int32_t Cout, void *out,
-                                                 const uint16_t *mask) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int y = blockIdx.y * 4 + wave;
-    const int x0 = blockIdx.x * 16;
-    if (y >= H) return;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int CT_W = 64, CT_H = 8, CT_LW = CT_W + 2, CT_LH = CT_H + 2, CT_S = 80;
+constexpr int CT_PIX = CT_LW * CT_LH;
+constexpr int CT_WG_PER_XCD = 64;  // 2 resident workgroups (register-bound) x 32 CUs
+
+FGL_D uint32_t pack_bf2(float a, float b) { return (uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16); }
+FGL_D float relu(float v) { return v > 0.0f ? v : 0.0f; }
+FGL_D float keep_if(uint32_t m16, float v) { return ((m16 & 0x8000u) == 0 && (m16 & 0x7fffu) != 0) ? v : 0.0f; }
+
+template <int CIN, int COUT, int EPI, bool FEAT>
+__global__ __launch_bounds__(256) void k_nn_conv(const uint16_t *in, const uint16_t *w, const float *bias, int32_t H,
+                                                 int32_t W, void *out, const uint16_t *mask, const uint32_t *color,
+                                                 const double *depth, const double *range) {
+    static_assert(CIN == 16 || CIN == 32, "CIN");
+    static_assert(COUT == 16 || COUT == 32, "COUT");
+    static_assert(!FEAT || CIN == 16, "features are 16 channels");
+    static_assert(EPI != 3 || COUT == 16, "EPI 3 needs COUT 16");
+    constexpr int NT = COUT / 16, CH = CIN / 8;
+    constexpr int ITEMS = CT_PIX * CH, ITER = (ITEMS + 255) / 256;
+    __shared__ __attribute__((aligned(16))) uint8_t lds[CT_PIX * CT_S];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int pm = lane & 15, kg = lane >> 4;
-    const int nt = Cout >> 4;
-    f32x4 acc[NN_C / 16];
+
+    const int ntx = (W + CT_W - 1) / CT_W, nty = (H + CT_H - 1) / CT_H;
+    const int band = blockIdx.x & 7, g = gridDim.x >> 3;
+    const int by0 = (band * nty) >> 3, by1 = ((band + 1) * nty) >> 3;
+    const int nband = (by1 - by0) * ntx;
+    int j = blockIdx.x >> 3;
+    if (j >= nband) return;
+
+    // weights and bias through LDS into registers; lane (pm, kg) holds
+    // w[tap][row pm of tile t][8 kg .. 8 kg + 7]
+    constexpr int WBYTES = 9 * COUT * CIN * 2;
+    static_assert(WBYTES + COUT * 4 <= CT_PIX * CT_S, "weights fit the tile area");
+    for (int q = tid; q < WBYTES / 16; q += 256)
+        *reinterpret_cast<uint4 *>(lds + 16 * q) = reinterpret_cast<const uint4 *>(w)[q];
+    if (EPI != 2)
+        for (int q = tid; q < COUT; q += 256) *reinterpret_cast<float *>(lds + WBYTES + 4 * q) = bias[q];
+    __syncthreads();
+    bf16x8 A[9][NT];
 #pragma unroll
-    for (int t = 0; t < NN_C / 16; t++) acc[t] = f32x4{0, 0, 0, 0};
-    for (int tap = 0; tap < 9; tap++) {
-        const int sy = y + tap / 3 - 1, sx = x0 + pm + tap % 3 - 1;
-        const bool inb = sy >= 0 && sy < H && sx >= 0 && sx < W;
-        const uint16_t *ap = in + ((int64_t)(inb ? sy : 0) * W + (inb ? sx : 0)) * Cin + 4 * kg;
-        const uint16_t *bp = w + ((int64_t)tap * Cout + pm) * Cin + 4 * kg;
-        for (int c0 = 0; c0 < Cin; c0 += 16) {
-            uint2 av = inb ? *reinterpret_cast<const uint2 *>(ap + c0) : make_uint2(0, 0);
-            bf16x4 a;
-            a[0] = (short)(av.x & 0xffff);
-            a[1] = (short)(av.x >> 16);
-            a[2] = (short)(av.y & 0xffff);
-            a[3] = (short)(av.y >> 16);
+    for (int tap = 0; tap < 9; tap++)
 #pragma unroll
-            for (int t = 0; t < NN_C / 16; t++) {
-                if (t < nt) {
-                    const uint2 bv = *reinterpret_cast<const uint2 *>(bp + (int64_t)t * 16 * Cin + c0);
-                    bf16x4 b;
-                    b[0] = (short)(bv.x & 0xffff);
-                    b[1] = (short)(bv.x >> 16);
-                    b[2] = (short)(bv.y & 0xffff);
-                    b[3] = (short)(bv.y >> 16);
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+        for (int t = 0; t < NT; t++) {
+            const int co = NT == 2 ? 8 * (pm >> 2) + 4 * t + (pm & 3) : pm;
+            if (CIN == 32 || kg < 2) {
+                A[tap][t] = *reinterpret_cast<const bf16x8 *>(lds + 2 * ((tap * COUT + co) * CIN + 8 * kg));
+            } else {
+                A[tap][t] = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+            }
+        }
+    const int ch0 = NT == 2 ? 8 * kg : 4 * kg;  // first output channel this lane holds
+    float bb[4 * NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            bb[4 * t + q] = (EPI == 2) ? 0.0f : *reinterpret_cast<const float *>(lds + WBYTES + 4 * (ch0 + 4 * t + q));
+    __syncthreads();
+    // channels the input does not have stay zero in LDS for every tile
+    for (int p = tid; p < CT_PIX; p += 256) {
+        if (FEAT) *reinterpret_cast<uint4 *>(lds + p * CT_S + 16) = make_uint4(0, 0, 0, 0);
+        if (CIN == 16) {
+            *reinterpret_cast<uint4 *>(lds + p * CT_S + 32) = make_uint4(0, 0, 0, 0);
+            *reinterpret_cast<uint4 *>(lds + p * CT_S + 48) = make_uint4(0, 0, 0, 0);
+        }
+    }
+
+    // the input of tile jj: clamped unconditional 16-byte loads, zero
+    // outside the image
+    auto fetch = [&](int jj, uint4 *v) {
+        const int x0 = (jj % ntx) * CT_W, y0 = (by0 + jj / ntx) * CT_H;
+#pragma unroll
+        for (int it = 0; it < ITER; it++) {
+            const int idx = tid + it * 256;
+            const int p = (idx < ITEMS ? idx : ITEMS - 1) / CH, c = idx % CH;
+            const int ry = p / CT_LW, rx = p - ry * CT_LW;
+            const int gy = y0 - 1 + ry, gx = x0 - 1 + rx;
+            const int cy = gy < 0 ? 0 : (gy >= H ? H - 1 : gy), cx = gx < 0 ? 0 : (gx >= W ? W - 1 : gx);
+            v[it] = *reinterpret_cast<const uint4 *>(in + ((int64_t)cy * W + cx) * CIN + c * 8);
+            if (gy != cy || gx != cx) v[it] = make_uint4(0, 0, 0, 0);
+        }
+    };
+    auto put = [&](const uint4 *v) {
+#pragma unroll
+        for (int it = 0; it < ITER; it++) {
+            const int idx = tid + it * 256;
+            if (idx < ITEMS) *reinterpret_cast<uint4 *>(lds + (idx / CH) * CT_S + (idx % CH) * 16) = v[it];
+        }
+    };
+    // layer 0 of inference: the features of each tile pixel, computed here
+    // from color and depth (k_nn_features' values, bit for bit)
+    auto put_features = [&](int jj) {
+        const int x0 = (jj % ntx) * CT_W, y0 = (by0 + jj / ntx) * CT_H;
+        const double lo = range[0], hi = range[1];
+        for (int p = tid; p < CT_PIX; p += 256) {
+            const int ry = p / CT_LW, rx = p - ry * CT_LW;
+            const int gy = y0 - 1 + ry, gx = x0 - 1 + rx;
+            uint4 f = make_uint4(0, 0, 0, 0);
+            if (gy >= 0 && gy < H && gx >= 0 && gx < W) f = pixel_features(color, depth, lo, hi, H, W, gx, gy);
+            *reinterpret_cast<uint4 *>(lds + p * CT_S) = f;
+        }
+    };
+
+    if (FEAT) {
+        put_features(j);
+    } else {
+        uint4 v[ITER];
+        fetch(j, v);
+        put(v);
+    }
+    __syncthreads();
+    for (;;) {
+        const int x0 = (j % ntx) * CT_W, y0 = (by0 + j / ntx) * CT_H;
+        const int jn = j + g;
+        uint4 v[FEAT ? 1 : ITER];
+        if (!FEAT && jn < nband) fetch(jn, v);
+
+        for (int rr = 0; rr < 2; rr++) {
+            const int r = wave * 2 + rr, y = y0 + r;
+            if (y >= H) break;
+            for (int xb = 0; xb < CT_W; xb += 16) {
+                if (x0 + xb >= W) break;
+                f32x4 acc[NT];
+#pragma unroll
+                for (int t = 0; t < NT; t++) acc[t] = f32x4{0, 0, 0, 0};
+#pragma unroll
+                for (int tap = 0; tap < 9; tap++) {
+                    const bf16x8 b = *reinterpret_cast<const bf16x8 *>(
+                        lds + ((r + tap / 3) * CT_LW + xb + pm + tap % 3) * CT_S + 16 * kg);
+#pragma unroll
+                    for (int t = 0; t < NT; t++)
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[tap][t], b, acc[t], 0, 0, 0);
+                }
+                const int x = x0 + xb + pm;
+                if (x >= W) continue;
+                const int64_t px = (int64_t)y * W + x;
+                if (EPI == 0 || EPI == 2) {
+                    float o[4 * NT];
+#pragma unroll
+                    for (int t = 0; t < NT; t++)
+#pragma unroll
+                        for (int q = 0; q < 4; q++) o[4 * t + q] = acc[t][q] + bb[4 * t + q];
+                    uint16_t *dst = static_cast<uint16_t *>(out) + px * COUT + ch0;
+                    if (EPI == 0) {
+#pragma unroll
+                        for (int k = 0; k < 4 * NT; k++) o[k] = relu(o[k]);
+                    } else if (NT == 2) {
+                        const uint4 m = *reinterpret_cast<const uint4 *>(mask + px * COUT + ch0);
+                        const uint32_t mw[4] = {m.x, m.y, m.z, m.w};
+#pragma unroll
+                        for (int k = 0; k < 8; k++) o[k] = keep_if((mw[k >> 1] >> (16 * (k & 1))) & 0xffffu, o[k]);
+                    } else {
+                        const uint2 m = *reinterpret_cast<const uint2 *>(mask + px * COUT + ch0);
+                        const uint32_t mw[2] = {m.x, m.y};
+#pragma unroll
+                        for (int k = 0; k < 4; k++) o[k] = keep_if((mw[k >> 1] >> (16 * (k & 1))) & 0xffffu, o[k]);
+                    }
+                    if (NT == 2) {
+                        *reinterpret_cast<uint4 *>(dst) = make_uint4(pack_bf2(o[0], o[1]), pack_bf2(o[2], o[3]),
+                                                                     pack_bf2(o[4], o[5]), pack_bf2(o[6], o[7]));
+                    } else {
+                        *reinterpret_cast<uint2 *>(dst) = make_uint2(pack_bf2(o[0], o[1]), pack_bf2(o[2], o[3]));
+                    }
+                } else if (EPI == 1) {
+                    float *dst = static_cast<float *>(out) + px * COUT + ch0;
+#pragma unroll
+                    for (int t = 0; t < NT; t++)
+                        *reinterpret_cast<float4 *>(dst + 4 * t) =
+                            make_float4(acc[t][0] + bb[4 * t], acc[t][1] + bb[4 * t + 1], acc[t][2] + bb[4 * t + 2],
+                                        acc[t][3] + bb[4 * t + 3]);
+                } else if (kg == 0) {
+                    const uint32_t c = color[px];
+                    uint32_t o = c & 0xff000000u;
+#pragma unroll
+                    for (int ch = 0; ch < 3; ch++) {
+                        float f = (float)((c >> (8 * ch)) & 0xff) * (1.0f / 255.0f) + (acc[0][ch] + bb[ch]);
+                        f = f < 0.0f ? 0.0f : (f > 1.0f ? 1.0f : f);
+                        o |= (uint32_t)(f * 255.0f + 0.5f) << (8 * ch);
+                    }
+                    static_cast<uint32_t *>(out)[px] = o;
                 }
             }
         }
-    }
-    // lane holds rows (pixels) 4 * kg + j, column (channel) t * 16 + pm
-#pragma unroll
-    for (int t = 0; t < NN_C / 16; t++) {
-        if (t >= nt) continue;
-        const int ch = t * 16 + pm;
-        const float bb = (EPI == 2) ? 0.0f : bias[ch];
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int x = x0 + 4 * kg + j;
-            if (x >= W) continue;
-            const int64_t o = ((int64_t)y * W + x) * Cout + ch;
-            float v = acc[t][j] + bb;
-            if (EPI == 0) {
-                v = v > 0.0f ? v : 0.0f;
-                static_cast<uint16_t *>(out)[o] = f2bf(v);
-            } else if (EPI == 1) {
-                static_cast<float *>(out)[o] = v;
-            } else {
-                const float m = bf2f(mask[o]);
-                static_cast<uint16_t *>(out)[o] = f2bf(m > 0.0f ? v : 0.0f);
-            }
+        if (jn >= nband) break;
+        __syncthreads();  // every wave is done reading this tile
+        if (FEAT) {
+            put_features(jn);
+        } else {
+            put(v);
         }
+        __syncthreads();
+        j = jn;
     }
 }
 
@@ -219,13 +379,22 @@ __global__ __launch_bounds__(64) void k_nn_dw(const uint16_t *x, int32_t Cin, co
     }
 }
 
-// out[i] = sum over k < n of part[k * len + i]
+// out[k2][i] = sum over k = k2, k2 + ks, ... < n of part[k * len + i]
+// (grid: len / 256 x ks); a second pass with n = ks sums the ks rows
 __global__ void k_nn_reduce(const float *part, int64_t n, int64_t len, float *out) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x) {
-        float s = 0.0f;
-        for (int64_t k = 0; k < n; k++) s += part[k * len + i];
-        out[i] = s;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= len) return;
+    const int64_t ks = gridDim.y;
+    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+    int64_t k = blockIdx.y;
+    for (; k + 3 * ks < n; k += 4 * ks) {
+        s0 += part[k * len + i];
+        s1 += part[(k + ks) * len + i];
+        s2 += part[(k + 2 * ks) * len + i];
+        s3 += part[(k + 3 * ks) * len + i];
     }
+    for (; k < n; k += ks) s0 += part[k * len + i];
+    out[(int64_t)blockIdx.y * len + i] = (s0 + s1) + (s2 + s3);
 }
 
 // ---------------------------------------------------------------------------
@@ -255,21 +424,6 @@ __global__ __launch_bounds__(256) void k_nn_loss(const float *out, const uint32_
     if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = l;
     __syncthreads();
     if (threadIdx.x == 0) lpart[blockIdx.x] = (ws[0] + ws[1] + ws[2] + ws[3]) / (3.0f * (float)n);
-}
-
-// prediction -> NRGBA8 (alpha from the input)
-__global__ void k_nn_output(const float *out, const uint32_t *color, int32_t H, int32_t W, uint32_t *dst) {
-    const int64_t n = (int64_t)W * H;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const uint32_t c = color[i];
-        uint32_t o = c & 0xff000000u;
-        for (int ch = 0; ch < 3; ch++) {
-            float v = (float)((c >> (8 * ch)) & 0xff) * (1.0f / 255.0f) + out[i * NN_OUT_C + ch];
-            v = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
-            o |= (uint32_t)(v * 255.0f + 0.5f) << (8 * ch);
-        }
-        dst[i] = o;
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -317,12 +471,27 @@ hipError_t launch_nn_features(const uint32_t *color, const double *depth, const 
     return hipGetLastError();
 }
 hipError_t launch_nn_conv(int epi, const uint16_t *in, int32_t Cin, const uint16_t *w, const float *bias, int32_t H,
-                          int32_t W, int32_t Cout, void *out, const uint16_t *mask, hipStream_t s) {
-    dim3 grid((W + 15) / 16, (H + 3) / 4), block(256);
-    if (epi == 0) hipLaunchKernelGGL(k_nn_conv<0>, grid, block, 0, s, in, Cin, w, bias, H, W, Cout, out, mask);
-    else if (epi == 1) hipLaunchKernelGGL(k_nn_conv<1>, grid, block, 0, s, in, Cin, w, bias, H, W, Cout, out, mask);
-    else hipLaunchKernelGGL(k_nn_conv<2>, grid, block, 0, s, in, Cin, w, bias, H, W, Cout, out, mask);
-    return hipGetLastError();
+                          int32_t W, int32_t Cout, void *out, const uint16_t *mask, const uint32_t *color,
+                          const double *depth, const double *range, hipStream_t s) {
+    const int64_t ntiles = (int64_t)((W + CT_W - 1) / CT_W) * ((H + CT_H - 1) / CT_H);
+    const int64_t per_xcd = (ntiles + 7) / 8;
+    const unsigned grid = 8u * (unsigned)(per_xcd < CT_WG_PER_XCD ? per_xcd : CT_WG_PER_XCD);
+    const bool feat = in == nullptr;
+#define FGL_NN_CONV(CI, CO, E, F)                                                                               \
+    if (Cin == CI && Cout == CO && epi == E && feat == F) {                                                     \
+        hipLaunchKernelGGL((k_nn_conv<CI, CO, E, F>), dim3(grid), dim3(256), 0, s, in, w, bias, H, W, out, mask, \
+                           color, depth, range);                                                                \
+        return hipGetLastError();                                                                               \
+    }
+    FGL_NN_CONV(16, 32, 0, true)
+    FGL_NN_CONV(16, 32, 0, false)
+    FGL_NN_CONV(32, 32, 0, false)
+    FGL_NN_CONV(32, 16, 1, false)
+    FGL_NN_CONV(32, 16, 3, false)
+    FGL_NN_CONV(16, 32, 2, false)
+    FGL_NN_CONV(32, 32, 2, false)
+#undef FGL_NN_CONV
+    return hipErrorInvalidValue;
 }
 int64_t nn_dw_waves(int32_t H, int32_t W) { return (int64_t)((W + 15) / 16) * ((H + NN_DW_ROWS - 1) / NN_DW_ROWS); }
 hipError_t launch_nn_dw(const uint16_t *x, int32_t Cin, const uint16_t *dy, int32_t Cout, int32_t H, int32_t W,
@@ -331,18 +500,16 @@ hipError_t launch_nn_dw(const uint16_t *x, int32_t Cin, const uint16_t *dy, int3
     hipLaunchKernelGGL(k_nn_dw, grid, dim3(64), 0, s, x, Cin, dy, Cout, H, W, partial, bpartial);
     return hipGetLastError();
 }
-hipError_t launch_nn_reduce(const float *part, int64_t n, int64_t len, float *out, hipStream_t s) {
-    hipLaunchKernelGGL(k_nn_reduce, dim3(nn_grid(len)), dim3(256), 0, s, part, n, len, out);
+hipError_t launch_nn_reduce(const float *part, int64_t n, int64_t len, float *out, float *tmp, hipStream_t s) {
+    const int64_t ks = n < NN_REDUCE_SPLIT ? n : NN_REDUCE_SPLIT;
+    const unsigned gx = (unsigned)((len + 255) / 256);
+    hipLaunchKernelGGL(k_nn_reduce, dim3(gx, (unsigned)ks), dim3(256), 0, s, part, n, len, tmp);
+    hipLaunchKernelGGL(k_nn_reduce, dim3(gx, 1), dim3(256), 0, s, tmp, ks, len, out);
     return hipGetLastError();
 }
 hipError_t launch_nn_loss(const float *out, const uint32_t *color, const uint32_t *target, int32_t H, int32_t W,
                           uint16_t *gout, float *lpart, int32_t nblocks, hipStream_t s) {
     hipLaunchKernelGGL(k_nn_loss, dim3(nblocks), dim3(256), 0, s, out, color, target, H, W, gout, lpart);
-    return hipGetLastError();
-}
-hipError_t launch_nn_output(const float *out, const uint32_t *color, int32_t H, int32_t W, uint32_t *dst,
-                            hipStream_t s) {
-    hipLaunchKernelGGL(k_nn_output, dim3(nn_grid((int64_t)W * H)), dim3(256), 0, s, out, color, H, W, dst);
     return hipGetLastError();
 }
 hipError_t launch_nn_adam(float *p, const float *g, float *m, float *v, int64_t n, float lr, float b1, float b2,
