@@ -1132,8 +1132,27 @@ int nn_forward(fgl_nn *nn, fgl_ctx *src, fgl_ctx *dst, bool keep_feat) {
                                                     nn->range.as<double>(), H, W, nn->feat.as<uint16_t>(), c->stream);
                       })))
         return r;
-    const uint16_t *in = keep_feat ? nn->feat.as<uint16_t>() : nullptr;
     if (dst && (r = stream_after(c, dst))) return r;
+    if (!keep_feat) {
+        // inference: layers fused in pairs, the odd layers' inputs stay in LDS
+        static const char *const kPairName[NN_LAYERS / 2] = {"nn_conv01", "nn_conv23", "nn_conv45"};
+        const uint16_t *pin = nullptr;  // pair 0 computes the features itself
+        for (int l = 0; l < NN_LAYERS; l += 2) {
+            const bool last = l + 2 == NN_LAYERS;
+            void *o = last ? (dst ? dst->color.p : nn->out.p) : nn->act[l + 1].p;
+            const int epi = last ? (dst ? 3 : 1) : 0;
+            if ((r = launch(c, kPairName[l / 2], [&] {
+                     return launch_nn_conv2(nn->cin[l], nn->cout[l + 1], epi, pin, nn->wf[l].as<uint16_t>(),
+                                            nn->p.as<float>() + nn->boff[l], nn->wf[l + 1].as<uint16_t>(),
+                                            nn->p.as<float>() + nn->boff[l + 1], H, W, o, src->color.as<uint32_t>(),
+                                            src->depth.as<double>(), nn->range.as<double>(), c->stream);
+                 })))
+                return r;
+            pin = nn->act[l + 1].as<uint16_t>();
+        }
+        return FGL_OK;
+    }
+    const uint16_t *in = nn->feat.as<uint16_t>();
     for (int l = 0; l < NN_LAYERS; l++) {
         const bool last = l == NN_LAYERS - 1;
         void *o = last ? (dst ? dst->color.p : nn->out.p) : nn->act[l].p;

@@ -5,14 +5,15 @@
 // No reference counterpart: this replaces 16x SSAA + resolve after the draw.
 //
 // Tensors are pixel-major with channels contiguous (H x W x C), BF16 for
-// convolution operands, FP32 accumulation (MFMA 16x16x16 bf16).  A wave
-// computes 16 consecutive pixels of one row times every output channel:
-// the A operand is 16 pixels x 16 input channels of one tap, the B operand
-// 16 input x 16 output channels of that tap's weights.  Training runs the
-// backward pass with the same convolution kernel (input gradient: flipped,
-// transposed weights, ReLU mask in the epilogue), a weight-gradient kernel
-// (MFMA over pixels) with per-wave partial sums reduced without atomics,
-// and Adam on FP32 master weights.
+// convolution operands, FP32 accumulation on the matrix cores
+// (v_mfma_f32_16x16x32_bf16 for the convolutions, 16x16x16 for the weight
+// gradient).  Inference runs the layers fused in pairs (k_nn_conv2), the
+// layer between kept in LDS; training runs them one by one (k_nn_conv),
+// keeping every activation for the backward pass, which uses the same
+// convolution kernel for the input gradient (flipped, transposed weights,
+// ReLU mask in the epilogue), a weight-gradient kernel (MFMA over pixels)
+// with per-wave partial sums reduced without atomics, and Adam on FP32
+// master weights.
 #include "fgl_internal.h"
 
 namespace fgl {
@@ -20,12 +21,8 @@ namespace fgl {
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// round-to-nearest-even FP32 -> BF16 (finite inputs)
-FGL_D uint16_t f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
+// round-to-nearest-even FP32 -> BF16 (v_cvt_pk_bf16_f32)
+FGL_D uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 FGL_D float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
 constexpr double kMaxF64 = 1.79769313486231570814527423731704356798070e+308;
@@ -126,7 +123,13 @@ constexpr int CT_W = 64, CT_H = 8, CT_LW = CT_W + 2, CT_LH = CT_H + 2, CT_S = 80
 constexpr int CT_PIX = CT_LW * CT_LH;
 constexpr int CT_WG_PER_XCD = 64;  // 2 resident workgroups (register-bound) x 32 CUs
 
-FGL_D uint32_t pack_bf2(float a, float b) { return (uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16); }
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+FGL_D uint32_t pack_bf2(float a, float b) {
+    bf16x2 v;
+    v[0] = (__bf16)a;
+    v[1] = (__bf16)b;
+    return __builtin_bit_cast(uint32_t, v);
+}
 FGL_D float relu(float v) { return v > 0.0f ? v : 0.0f; }
 FGL_D float keep_if(uint32_t m16, float v) { return ((m16 & 0x8000u) == 0 && (m16 & 0x7fffu) != 0) ? v : 0.0f; }
 
@@ -246,7 +249,7 @@ __global__ __launch_bounds__(256) void k_nn_conv(const uint16_t *in, const uint1
                 if (x0 + xb >= W) break;
                 f32x4 acc[NT];
 #pragma unroll
-                for (int t = 0; t < NT; t++) acc[t] = f32x4{0, 0, 0, 0};
+                for (int t = 0; t < NT; t++) acc[t] = f32x4{bb[4 * t], bb[4 * t + 1], bb[4 * t + 2], bb[4 * t + 3]};
 #pragma unroll
                 for (int tap = 0; tap < 9; tap++) {
                     const bf16x8 b = *reinterpret_cast<const bf16x8 *>(
@@ -263,7 +266,7 @@ __global__ __launch_bounds__(256) void k_nn_conv(const uint16_t *in, const uint1
 #pragma unroll
                     for (int t = 0; t < NT; t++)
 #pragma unroll
-                        for (int q = 0; q < 4; q++) o[4 * t + q] = acc[t][q] + bb[4 * t + q];
+                        for (int q = 0; q < 4; q++) o[4 * t + q] = acc[t][q];
                     uint16_t *dst = static_cast<uint16_t *>(out) + px * COUT + ch0;
                     if (EPI == 0) {
 #pragma unroll
@@ -290,14 +293,13 @@ __global__ __launch_bounds__(256) void k_nn_conv(const uint16_t *in, const uint1
 #pragma unroll
                     for (int t = 0; t < NT; t++)
                         *reinterpret_cast<float4 *>(dst + 4 * t) =
-                            make_float4(acc[t][0] + bb[4 * t], acc[t][1] + bb[4 * t + 1], acc[t][2] + bb[4 * t + 2],
-                                        acc[t][3] + bb[4 * t + 3]);
+                            make_float4(acc[t][0], acc[t][1], acc[t][2], acc[t][3]);
                 } else if (kg == 0) {
                     const uint32_t c = color[px];
                     uint32_t o = c & 0xff000000u;
 #pragma unroll
                     for (int ch = 0; ch < 3; ch++) {
-                        float f = (float)((c >> (8 * ch)) & 0xff) * (1.0f / 255.0f) + (acc[0][ch] + bb[ch]);
+                        float f = (float)((c >> (8 * ch)) & 0xff) * (1.0f / 255.0f) + acc[0][ch];
                         f = f < 0.0f ? 0.0f : (f > 1.0f ? 1.0f : f);
                         o |= (uint32_t)(f * 255.0f + 0.5f) << (8 * ch);
                     }
@@ -313,6 +315,243 @@ __global__ __launch_bounds__(256) void k_nn_conv(const uint16_t *in, const uint1
             put(v);
         }
         __syncthreads();
+        j = jn;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Two layers in one kernel (inference): layer A (CIN_A -> 32, ReLU) and
+// layer B (32 -> COUT_B, epilogue EPI_B as in k_nn_conv) on an F2_W x F2_H
+// output tile.  The input tile (two-pixel halo) and layer A's output (one-
+// pixel halo, zero outside the image: the padding layer B sees) stay in
+// LDS, so that the intermediate activation never goes to HBM.  One
+// persistent workgroup of 8 waves per CU; both layers' weights sit in LDS
+// (rows padded to F2_WS bytes) and are read into registers per phase.
+// Each wave computes a strip of rows of one 16-pixel column block, reading
+// every input row's three shifted fragments once for the up to three
+// output rows that use it.  The multiply-accumulate order per output is
+// bias, then taps 0..8, as in k_nn_conv: both kernels give the same bits.
+
+constexpr int F2_W = 62, F2_H = 8;                  // output tile
+constexpr int F2_IW = F2_W + 4, F2_IH = F2_H + 4;   // input tile
+constexpr int F2_MW = F2_W + 4, F2_MH = F2_H + 2;   // intermediate: 64 computed columns + 2 padding
+constexpr int F2_S = 80, F2_WS = 80;
+constexpr int F2_WG_PER_XCD = 32;                   // one workgroup per CU
+
+template <int CIN_A, int COUT_B, int EPI_B, bool FEAT>
+__global__ __launch_bounds__(512) void k_nn_conv2(const uint16_t *in, const uint16_t *wa, const float *bias_a,
+                                                  const uint16_t *wb, const float *bias_b, int32_t H, int32_t W,
+                                                  void *out, const uint32_t *color, const double *depth,
+                                                  const double *range) {
+    static_assert(CIN_A == 16 || CIN_A == 32, "CIN_A");
+    static_assert((COUT_B == 32 && EPI_B == 0) || (COUT_B == 16 && (EPI_B == 1 || EPI_B == 3)), "layer B");
+    static_assert(!FEAT || CIN_A == 16, "features are 16 channels");
+    constexpr int CH = CIN_A / 8, NTB = COUT_B / 16;
+    constexpr int IN_BYTES = F2_IW * F2_IH * F2_S, MID_BYTES = F2_MW * F2_MH * F2_S;
+    constexpr int WA_S = CIN_A == 16 ? 48 : F2_WS;
+    constexpr int WA_BYTES = 9 * 32 * WA_S, WB_BYTES = 9 * COUT_B * F2_WS;
+    constexpr int ITEMS = F2_IW * F2_IH * CH, ITER = (ITEMS + 511) / 512;
+    __shared__ __attribute__((aligned(16))) uint8_t lds[IN_BYTES + MID_BYTES + WA_BYTES + WB_BYTES + 64 * 4];
+    uint8_t *const lin = lds, *const lmid = lds + IN_BYTES, *const lwa = lmid + MID_BYTES, *const lwb = lwa + WA_BYTES;
+    float *const lbias = reinterpret_cast<float *>(lwb + WB_BYTES);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int pm = lane & 15, kg = lane >> 4;
+    const int blk = wave & 3, half = wave >> 2;
+
+    const int ntx = (W + F2_W - 1) / F2_W, nty = (H + F2_H - 1) / F2_H;
+    const int band = blockIdx.x & 7, g = gridDim.x >> 3;
+    const int by0 = (band * nty) >> 3, by1 = ((band + 1) * nty) >> 3;
+    const int nband = (by1 - by0) * ntx;
+    int j = blockIdx.x >> 3;
+    if (j >= nband) return;
+
+    // both layers' weights and biases into LDS, once
+    for (int q = tid; q < 9 * 32 * CH; q += 512)
+        *reinterpret_cast<uint4 *>(lwa + (q / CH) * WA_S + (q % CH) * 16) = reinterpret_cast<const uint4 *>(wa)[q];
+    for (int q = tid; q < 9 * COUT_B * 4; q += 512)
+        *reinterpret_cast<uint4 *>(lwb + (q >> 2) * F2_WS + (q & 3) * 16) = reinterpret_cast<const uint4 *>(wb)[q];
+    if (tid < 32) lbias[tid] = bias_a[tid];
+    if (tid < COUT_B) lbias[32 + tid] = bias_b[tid];
+    // channels the input does not have stay zero for every tile
+    for (int p = tid; p < F2_IW * F2_IH; p += 512) {
+        if (FEAT) *reinterpret_cast<uint4 *>(lin + p * F2_S + 16) = make_uint4(0, 0, 0, 0);
+        if (CIN_A == 16) {
+            *reinterpret_cast<uint4 *>(lin + p * F2_S + 32) = make_uint4(0, 0, 0, 0);
+            *reinterpret_cast<uint4 *>(lin + p * F2_S + 48) = make_uint4(0, 0, 0, 0);
+        }
+    }
+
+    auto fetch = [&](int jj, uint4 *v) {
+        const int x0 = (jj % ntx) * F2_W, y0 = (by0 + jj / ntx) * F2_H;
+#pragma unroll
+        for (int it = 0; it < ITER; it++) {
+            const int idx = tid + it * 512;
+            const int p = (idx < ITEMS ? idx : ITEMS - 1) / CH, c = idx % CH;
+            const int ry = p / F2_IW, rx = p - ry * F2_IW;
+            const int gy = y0 - 2 + ry, gx = x0 - 2 + rx;
+            const int cy = gy < 0 ? 0 : (gy >= H ? H - 1 : gy), cx = gx < 0 ? 0 : (gx >= W ? W - 1 : gx);
+            v[it] = *reinterpret_cast<const uint4 *>(in + ((int64_t)cy * W + cx) * CIN_A + c * 8);
+            if (gy != cy || gx != cx) v[it] = make_uint4(0, 0, 0, 0);
+        }
+    };
+    auto put = [&](const uint4 *v) {
+#pragma unroll
+        for (int it = 0; it < ITER; it++) {
+            const int idx = tid + it * 512;
+            if (idx < ITEMS) *reinterpret_cast<uint4 *>(lin + (idx / CH) * F2_S + (idx % CH) * 16) = v[it];
+        }
+    };
+    auto put_features = [&](int jj) {
+        const int x0 = (jj % ntx) * F2_W, y0 = (by0 + jj / ntx) * F2_H;
+        const double lo = range[0], hi = range[1];
+        for (int p = tid; p < F2_IW * F2_IH; p += 512) {
+            const int ry = p / F2_IW, rx = p - ry * F2_IW;
+            const int gy = y0 - 2 + ry, gx = x0 - 2 + rx;
+            uint4 f = make_uint4(0, 0, 0, 0);
+            if (gy >= 0 && gy < H && gx >= 0 && gx < W) f = pixel_features(color, depth, lo, hi, H, W, gx, gy);
+            *reinterpret_cast<uint4 *>(lin + p * F2_S) = f;
+        }
+    };
+
+    __syncthreads();  // weights staged before the first tile goes in
+    if (FEAT) {
+        put_features(j);
+    } else {
+        uint4 v0[ITER];
+        fetch(j, v0);
+        put(v0);
+    }
+    __syncthreads();
+
+    for (;;) {
+        const int x0 = (j % ntx) * F2_W, y0 = (by0 + j / ntx) * F2_H;
+        const int jn = j + g;
+        uint4 v[FEAT ? 1 : ITER];
+        if (!FEAT && jn < nband) fetch(jn, v);
+
+        // ---- layer A: intermediate rows 5 * half .. + 4, columns 16 * blk .. + 15
+        {
+            bf16x8 A[9][2];
+#pragma unroll
+            for (int tap = 0; tap < 9; tap++)
+#pragma unroll
+                for (int t = 0; t < 2; t++) {
+                    const int co = 8 * (pm >> 2) + 4 * t + (pm & 3);
+                    if (CIN_A == 32 || kg < 2)
+                        A[tap][t] = *reinterpret_cast<const bf16x8 *>(lwa + (tap * 32 + co) * WA_S + 16 * kg);
+                    else
+                        A[tap][t] = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+                }
+            const int r0 = 5 * half;
+            f32x4 acc[5][2];
+#pragma unroll
+            for (int q = 0; q < 5; q++)
+#pragma unroll
+                for (int t = 0; t < 2; t++)
+                    acc[q][t] = f32x4{lbias[8 * kg + 4 * t], lbias[8 * kg + 4 * t + 1], lbias[8 * kg + 4 * t + 2],
+                                      lbias[8 * kg + 4 * t + 3]};
+            // input row r0 + k feeds output row r0 + k - dy with tap row dy;
+            // for each output row the taps arrive in order 0..8
+#pragma unroll
+            for (int k = 0; k < 7; k++)
+#pragma unroll
+                for (int dx = 0; dx < 3; dx++) {
+                    const bf16x8 f = *reinterpret_cast<const bf16x8 *>(
+                        lin + ((r0 + k) * F2_IW + 16 * blk + pm + dx) * F2_S + 16 * kg);
+#pragma unroll
+                    for (int dy = 0; dy < 3; dy++) {
+                        const int q = k - dy;
+                        if (q < 0 || q > 4) continue;
+#pragma unroll
+                        for (int t = 0; t < 2; t++)
+                            acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[3 * dy + dx][t], f, acc[q][t], 0, 0, 0);
+                    }
+                }
+#pragma unroll
+            for (int q = 0; q < 5; q++) {
+                const int iy = r0 + q, ix = 16 * blk + pm;
+                const int gy = y0 - 1 + iy, gx = x0 - 1 + ix;
+                uint4 o = make_uint4(0, 0, 0, 0);
+                if (gy >= 0 && gy < H && gx >= 0 && gx < W)
+                    o = make_uint4(pack_bf2(relu(acc[q][0][0]), relu(acc[q][0][1])),
+                                   pack_bf2(relu(acc[q][0][2]), relu(acc[q][0][3])),
+                                   pack_bf2(relu(acc[q][1][0]), relu(acc[q][1][1])),
+                                   pack_bf2(relu(acc[q][1][2]), relu(acc[q][1][3])));
+                *reinterpret_cast<uint4 *>(lmid + (iy * F2_MW + ix) * F2_S + 16 * kg) = o;
+            }
+        }
+        __syncthreads();
+
+        // ---- layer B: output rows 4 * half .. + 3, columns 16 * blk .. + 15
+        {
+            bf16x8 B[9][NTB];
+#pragma unroll
+            for (int tap = 0; tap < 9; tap++)
+#pragma unroll
+                for (int t = 0; t < NTB; t++) {
+                    const int co = NTB == 2 ? 8 * (pm >> 2) + 4 * t + (pm & 3) : pm;
+                    B[tap][t] = *reinterpret_cast<const bf16x8 *>(lwb + (tap * COUT_B + co) * F2_WS + 16 * kg);
+                }
+            const int ch0 = NTB == 2 ? 8 * kg : 4 * kg;
+            const int r0 = 4 * half;
+            f32x4 acc[4][NTB];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+#pragma unroll
+                for (int t = 0; t < NTB; t++)
+                    acc[q][t] = f32x4{lbias[32 + ch0 + 4 * t], lbias[32 + ch0 + 4 * t + 1], lbias[32 + ch0 + 4 * t + 2],
+                                      lbias[32 + ch0 + 4 * t + 3]};
+#pragma unroll
+            for (int k = 0; k < 6; k++)
+#pragma unroll
+                for (int dx = 0; dx < 3; dx++) {
+                    const bf16x8 f = *reinterpret_cast<const bf16x8 *>(
+                        lmid + ((r0 + k) * F2_MW + 16 * blk + pm + dx) * F2_S + 16 * kg);
+#pragma unroll
+                    for (int dy = 0; dy < 3; dy++) {
+                        const int q = k - dy;
+                        if (q < 0 || q > 3) continue;
+#pragma unroll
+                        for (int t = 0; t < NTB; t++)
+                            acc[q][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[3 * dy + dx][t], f, acc[q][t], 0, 0, 0);
+                    }
+                }
+            const int ox = 16 * blk + pm, x = x0 + ox;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int y = y0 + r0 + q;
+                if (ox >= F2_W || x >= W || y >= H) continue;
+                const int64_t px = (int64_t)y * W + x;
+                if (EPI_B == 0) {
+                    *reinterpret_cast<uint4 *>(static_cast<uint16_t *>(out) + px * COUT_B + ch0) =
+                        make_uint4(pack_bf2(relu(acc[q][0][0]), relu(acc[q][0][1])),
+                                   pack_bf2(relu(acc[q][0][2]), relu(acc[q][0][3])),
+                                   pack_bf2(relu(acc[q][NTB - 1][0]), relu(acc[q][NTB - 1][1])),
+                                   pack_bf2(relu(acc[q][NTB - 1][2]), relu(acc[q][NTB - 1][3])));
+                } else if (EPI_B == 1) {
+                    *reinterpret_cast<float4 *>(static_cast<float *>(out) + px * COUT_B + ch0) =
+                        make_float4(acc[q][0][0], acc[q][0][1], acc[q][0][2], acc[q][0][3]);
+                } else if (kg == 0) {
+                    const uint32_t c = color[px];
+                    uint32_t o = c & 0xff000000u;
+#pragma unroll
+                    for (int ch = 0; ch < 3; ch++) {
+                        float f = (float)((c >> (8 * ch)) & 0xff) * (1.0f / 255.0f) + acc[q][0][ch];
+                        f = f < 0.0f ? 0.0f : (f > 1.0f ? 1.0f : f);
+                        o |= (uint32_t)(f * 255.0f + 0.5f) << (8 * ch);
+                    }
+                    static_cast<uint32_t *>(out)[px] = o;
+                }
+            }
+        }
+        if (jn >= nband) break;
+        // every wave is past layer A, so the input buffer is free
+        if (FEAT) {
+            put_features(jn);
+        } else {
+            put(v);
+        }
+        __syncthreads();  // next input in place; every wave is done with the intermediate
         j = jn;
     }
 }
@@ -491,6 +730,26 @@ hipError_t launch_nn_conv(int epi, const uint16_t *in, int32_t Cin, const uint16
     FGL_NN_CONV(16, 32, 2, false)
     FGL_NN_CONV(32, 32, 2, false)
 #undef FGL_NN_CONV
+    return hipErrorInvalidValue;
+}
+hipError_t launch_nn_conv2(int cin_a, int cout_b, int epi_b, const uint16_t *in, const uint16_t *wa, const float *ba,
+                           const uint16_t *wb, const float *bb, int32_t H, int32_t W, void *out,
+                           const uint32_t *color, const double *depth, const double *range, hipStream_t s) {
+    const int64_t ntiles = (int64_t)((W + F2_W - 1) / F2_W) * ((H + F2_H - 1) / F2_H);
+    const int64_t per_xcd = (ntiles + 7) / 8;
+    const unsigned grid = 8u * (unsigned)(per_xcd < F2_WG_PER_XCD ? per_xcd : F2_WG_PER_XCD);
+    const bool feat = in == nullptr;
+#define FGL_NN_CONV2(CA, CB, E, F)                                                                              \
+    if (cin_a == CA && cout_b == CB && epi_b == E && feat == F) {                                               \
+        hipLaunchKernelGGL((k_nn_conv2<CA, CB, E, F>), dim3(grid), dim3(512), 0, s, in, wa, ba, wb, bb, H, W,    \
+                           out, color, depth, range);                                                           \
+        return hipGetLastError();                                                                               \
+    }
+    FGL_NN_CONV2(16, 32, 0, true)
+    FGL_NN_CONV2(32, 32, 0, false)
+    FGL_NN_CONV2(32, 16, 3, false)
+    FGL_NN_CONV2(32, 16, 1, false)
+#undef FGL_NN_CONV2
     return hipErrorInvalidValue;
 }
 int64_t nn_dw_waves(int32_t H, int32_t W) { return (int64_t)((W + 15) / 16) * ((H + NN_DW_ROWS - 1) / NN_DW_ROWS); }

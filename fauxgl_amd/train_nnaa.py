@@ -3,8 +3,10 @@
 Each step renders one view at 1x (the network's input) and the same view at
 4x4 the size followed by the tent resolve (the target), then takes one Adam
 step on the GPU.  Views orbit the scene at random azimuth, elevation and
-distance; the evaluation views (the headline camera among them) are never
-trained on.
+distance, with a random mesh (the headline noise sphere, two other noise
+spheres, the sphere-and-cube scene), light direction, object color and
+background; the evaluation views (the headline camera among them, with the
+headline light and colors) are never trained on.
 
     python -m fauxgl_amd.train_nnaa --steps 1500 --out fauxgl_amd/nnaa_weights.npy
 """
@@ -38,28 +40,60 @@ HEADLINE_EYE = M.EYE
 
 
 class PairRenderer:
-    """Renders (1x, 16x-resolved) pairs of one scene at W x H."""
+    """Renders (1x, 16x-resolved) pairs of W x H frames of one or more
+    meshes resident on the GPU."""
 
-    def __init__(self, width, height, mesh):
+    def __init__(self, width, height, meshes):
         self.W, self.H = width, height
         self.lo = A.NewContext(width, height)
         self.big = A.NewContext(4 * width, 4 * height)
         self.hi = A.NewContext(width, height)
         self.out = A.NewContext(width, height)
-        self.dm = self.big.Upload(mesh)
+        if not isinstance(meshes, (list, tuple)):
+            meshes = [meshes]
+        self.dms = [self.big.Upload(m) for m in meshes]
+        self.dm = self.dms[0]
 
-    def render(self, eye):
+    def render(self, eye, mesh=0, light=None, color=None, background=None):
+        """The frame of mesh index `mesh` seen from eye (headline light,
+        object color and background unless given)."""
         for ctx, w, h in ((self.lo, self.W, self.H), (self.big, 4 * self.W, 4 * self.H)):
-            ctx.ClearColorBufferWith(M.BACKGROUND)
+            ctx.ClearColorBufferWith(background if background is not None else M.BACKGROUND)
             ctx.ClearDepthBuffer()
-            ctx.Shader = M.phong_shader(w, h, eye=eye)
-            ctx.DrawMesh(self.dm)
+            sh = M.phong_shader(w, h, eye=eye)
+            if light is not None:
+                sh.LightDirection = A.Vector(*light)
+            if color is not None:
+                sh.ObjectColor = A.Color(*color)
+            ctx.Shader = sh
+            ctx.DrawMesh(self.dms[mesh])
         self.big.Resolve(self.hi, "tent")
 
     def close(self):
-        self.dm.close()
+        for dm in self.dms:
+            dm.close()
         for c in (self.lo, self.big, self.hi, self.out):
             c.close()
+
+
+def training_meshes():
+    """The headline noise sphere, two smaller noise spheres of other seeds
+    and amplitudes, and the sphere-and-cube scene."""
+    return [M.noise_sphere_mesh(),
+            M.noise_sphere_mesh(ntri=300000, detail=7, amplitude=0.25, seed=7),
+            M.noise_sphere_mesh(ntri=80000, detail=6, amplitude=0.08, seed=11),
+            M.sphere_and_cube_mesh()]
+
+
+def random_scene(rng, nmesh):
+    """(mesh index, light direction, object color, background) drawn at random."""
+    mesh = int(rng.integers(nmesh))
+    v = rng.normal(size=3)
+    v[1] = abs(v[1]) + 0.2
+    light = tuple(v / np.linalg.norm(v))
+    color = tuple(rng.uniform(0.1, 0.9, 3)) + (1.0,)
+    bg = tuple(rng.uniform(0.0, 1.0, 3)) + (1.0,) if rng.uniform() < 0.5 else M.BACKGROUND
+    return mesh, light, color, bg
 
 
 def random_eye(rng):
@@ -85,14 +119,15 @@ def evaluate(net, pr, eyes):
     return res
 
 
-def train(steps=1500, width=1920, height=1080, lr=1e-3, seed=0, init=None, log=None, mesh=None):
+def train(steps=1500, width=1920, height=1080, lr=1e-3, seed=0, init=None, log=None, meshes=None):
     rng = np.random.Generator(np.random.PCG64(seed))
-    pr = PairRenderer(width, height, mesh if mesh is not None else M.noise_sphere_mesh())
+    pr = PairRenderer(width, height, meshes if meshes is not None else training_meshes())
     net = nnaa.Network(pr.lo, seed=seed, weights=init)
     t0 = time.time()
     losses = []
     for s in range(steps):
-        pr.render(random_eye(rng))
+        mesh, light, color, bg = random_scene(rng, len(pr.dms))
+        pr.render(random_eye(rng), mesh, light, color, bg)
         frac = s / max(steps - 1, 1)
         cur = lr * (0.5 * (1 + math.cos(math.pi * frac)) * 0.95 + 0.05)
         losses.append(net.TrainStep(pr.lo, pr.hi, cur))
