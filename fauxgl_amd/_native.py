@@ -32,6 +32,8 @@ EXPORTED = [
     "fgl_profile_read", "fgl_profile_reset", "fgl_ctx_sync", "fgl_draw_stats", "fgl_profile_only",
     "fgl_nn_create", "fgl_nn_destroy", "fgl_nn_param_count", "fgl_nn_get_params", "fgl_nn_set_params",
     "fgl_nn_infer", "fgl_nn_residual", "fgl_nn_grad", "fgl_nn_train_step",
+    "fgl_mesh_counts", "fgl_mesh_read", "fgl_mesh_bounding_box", "fgl_mesh_transform", "fgl_mesh_fit_inside",
+    "fgl_mesh_smooth_normals", "fgl_mesh_smooth_normals_threshold",
 ]
 
 _d = ctypes.c_double
@@ -114,6 +116,13 @@ def load():
         "fgl_ctx_sync": (i32, [vp]),
         "fgl_profile_only": (i32, [vp, ctypes.c_char_p]),
         "fgl_draw_stats": (i32, [vp, ctypes.POINTER(FglDrawStat)]),
+        "fgl_mesh_counts": (i32, [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
+        "fgl_mesh_read": (i32, [vp, i32, i32, ctypes.POINTER(_d), ctypes.c_int64]),
+        "fgl_mesh_bounding_box": (i32, [vp, ctypes.POINTER(_d)]),
+        "fgl_mesh_transform": (i32, [vp, ctypes.POINTER(_d)]),
+        "fgl_mesh_fit_inside": (i32, [vp, ctypes.POINTER(_d), ctypes.POINTER(_d), ctypes.POINTER(_d)]),
+        "fgl_mesh_smooth_normals": (i32, [vp]),
+        "fgl_mesh_smooth_normals_threshold": (i32, [vp, _d]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)

@@ -453,6 +453,83 @@ class DeviceMesh:
         except Exception:
             pass
 
+    def _ptr(self):
+        if not self.h:
+            raise RuntimeError("mesh is closed")
+        return self.h
+
+    # mesh preparation on the device (mesh.go), in place
+    def BoundingBox(self):
+        """Mesh.BoundingBox (mesh.go:153-165) -> (min Vector, max Vector)."""
+        b = (ctypes.c_double * 6)()
+        N.check(N.load().fgl_mesh_bounding_box(self._ptr(), b), self.ctx._ptr())
+        return Vector(b[0], b[1], b[2]), Vector(b[3], b[4], b[5])
+
+    def Transform(self, matrix):
+        """Mesh.Transform (mesh.go:167-175)."""
+        m = (ctypes.c_double * 16)(*Matrix(matrix))
+        N.check(N.load().fgl_mesh_transform(self._ptr(), m), self.ctx._ptr())
+
+    def FitInside(self, box_min, box_max, anchor):
+        """Mesh.FitInside (mesh.go:142-151); returns the matrix applied."""
+        b = (ctypes.c_double * 6)(*box_min, *box_max)
+        a = (ctypes.c_double * 3)(*anchor)
+        m = (ctypes.c_double * 16)()
+        N.check(N.load().fgl_mesh_fit_inside(self._ptr(), b, a, m), self.ctx._ptr())
+        return Matrix(list(m))
+
+    def BiUnitCube(self):
+        """Mesh.BiUnitCube (mesh.go:132-135)."""
+        return self.FitInside(V(-1, -1, -1), V(1, 1, 1), V(0.5, 0.5, 0.5))
+
+    def UnitCube(self):
+        """Mesh.UnitCube (mesh.go:127-130)."""
+        return self.FitInside(V(-0.5, -0.5, -0.5), V(0.5, 0.5, 0.5), V(0.5, 0.5, 0.5))
+
+    def MoveTo(self, position, anchor):
+        """Mesh.MoveTo (mesh.go:137-141): Translate(position - box.Anchor(anchor))."""
+        lo, hi = self.BoundingBox()
+        a = lo.Add(hi.Sub(lo).Mul(Vector(*anchor)))
+        m = Translate(Vector(*position).Sub(a))
+        self.Transform(m)
+        return m
+
+    def Center(self):
+        """Mesh.Center (mesh.go:143-145)."""
+        return self.MoveTo(V(0, 0, 0), V(0.5, 0.5, 0.5))
+
+    def SmoothNormals(self):
+        """Mesh.SmoothNormals (mesh.go:105-125)."""
+        N.check(N.load().fgl_mesh_smooth_normals(self._ptr()), self.ctx._ptr())
+
+    def SmoothNormalsThreshold(self, radians):
+        """Mesh.SmoothNormalsThreshold (mesh.go:91-103).  The threshold is
+        cos(radians) from Python's math.cos (the C library's); a Go caller
+        passes math.Cos(radians) to fgl_mesh_smooth_normals_threshold."""
+        self.SmoothNormalsThresholdCos(math.cos(radians))
+
+    def SmoothNormalsThresholdCos(self, threshold):
+        N.check(N.load().fgl_mesh_smooth_normals_threshold(self._ptr(), float(threshold)), self.ctx._ptr())
+
+    def Download(self):
+        """The mesh's current vertex data as a host Mesh (fgl_prims layout)."""
+        nt, nl = ctypes.c_int64(), ctypes.c_int64()
+        L = N.load()
+        N.check(L.fgl_mesh_counts(self._ptr(), ctypes.byref(nt), ctypes.byref(nl)), self.ctx._ptr())
+
+        def attrs(lines, n, nv):
+            out = []
+            for attr, w in enumerate((3, 3, 2, 4)):
+                a = np.empty((n, nv, w), dtype=np.float64)
+                N.check(L.fgl_mesh_read(self._ptr(), lines, attr, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                        a.size), self.ctx._ptr())
+                out.append(a)
+            return Prims(*out, nv=nv)
+
+        tris = attrs(0, nt.value, 3) if nt.value else None
+        lines = attrs(1, nl.value, 2) if nl.value else None
+        return Mesh(tris, lines)
+
 
 # ---------------------------------------------------------------------------
 # Context (context.go)

@@ -1419,3 +1419,260 @@ int fgl_nn_train_step(fgl_nn *nn, fgl_ctx *src, fgl_ctx *target, float lr, float
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Device-side mesh preparation (fgl_mesh.hip): BoundingBox, Transform,
+// FitInside, SmoothNormals, SmoothNormalsThreshold on a resident mesh.
+
+namespace {
+
+double hgo_min(double x, double y) {
+    if ((std::isinf(x) && x < 0) || (std::isinf(y) && y < 0)) return -INFINITY;
+    if (std::isnan(x) || std::isnan(y)) return NAN;
+    if (x == 0 && x == y) return std::signbit(x) ? x : y;
+    return x < y ? x : y;
+}
+double hgo_max(double x, double y) {
+    if ((std::isinf(x) && x > 0) || (std::isinf(y) && y > 0)) return INFINITY;
+    if (std::isnan(x) || std::isnan(y)) return NAN;
+    if (x == 0 && x == y) return std::signbit(x) ? y : x;
+    return x > y ? x : y;
+}
+
+// Matrix.Mul (matrix.go:188-207), row-major X00..X33
+void mat_mul(const double a[16], const double b[16], double m[16]) {
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++)
+            m[4 * i + j] = a[4 * i] * b[j] + a[4 * i + 1] * b[4 + j] + a[4 * i + 2] * b[8 + j] + a[4 * i + 3] * b[12 + j];
+}
+
+int check_mesh(fgl_mesh *m) {
+    if (!m || !m->ctx) {
+        g_err = "null mesh";
+        return FGL_E_INVALID;
+    }
+    return FGL_OK;
+}
+
+constexpr int32_t kMeshRedBlocks = 1024;
+
+// index of the first primitive with a coordinate != 0, or -1
+int mesh_first_nonzero(fgl_ctx *c, const double *pos, int64_t nprim, int32_t nv, int64_t *first) {
+    *first = -1;
+    if (nprim == 0) return FGL_OK;
+    const int32_t nb = (int32_t)std::min<int64_t>(kMeshRedBlocks, (nprim + 255) / 256);
+    DevBuf part;
+    HIPCHK(c, part.ensure((size_t)nb * 8));
+    int r = launch(c, "mesh_box", [&] {
+        return launch_mesh_first_nonzero(pos, nprim, nv, part.as<int64_t>(), nb, c->stream);
+    });
+    if (r) return r;
+    std::vector<int64_t> h(nb);
+    HIPCHK(c, hipMemcpyAsync(h.data(), part.p, (size_t)nb * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int64_t f = INT64_MAX;
+    for (int64_t x : h) f = std::min(f, x);
+    if (f != INT64_MAX) *first = f;
+    return FGL_OK;
+}
+
+// math.Min / Max of every coordinate of primitives p >= from into box
+int mesh_box_from(fgl_ctx *c, const double *pos, int64_t nprim, int32_t nv, int64_t from, double box[6]) {
+    const int32_t nb = (int32_t)std::min<int64_t>(kMeshRedBlocks, ((nprim - from) * nv + 255) / 256);
+    DevBuf part;
+    HIPCHK(c, part.ensure((size_t)nb * 6 * 8));
+    int r = launch(c, "mesh_box", [&] {
+        return launch_mesh_box(pos, nprim, nv, from, part.as<double>(), nb, c->stream);
+    });
+    if (r) return r;
+    std::vector<double> h((size_t)nb * 6);
+    HIPCHK(c, hipMemcpyAsync(h.data(), part.p, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int b = 0; b < nb; b++)
+        for (int k = 0; k < 3; k++) {
+            box[k] = hgo_min(box[k], h[6 * b + k]);
+            box[3 + k] = hgo_max(box[3 + k], h[6 * b + 3 + k]);
+        }
+    return FGL_OK;
+}
+
+// Mesh.BoundingBox (mesh.go:153-165): triangles, then lines, each folded
+// with Box.Extend (box.go:48-53)
+int mesh_bbox(fgl_mesh *m, double box[6]) {
+    fgl_ctx *c = m->ctx;
+    for (int k = 0; k < 3; k++) {
+        box[k] = INFINITY;
+        box[3 + k] = -INFINITY;
+    }
+    int64_t ft = -1, fl = -1;
+    int r;
+    if (m->t_has[0] && (r = mesh_first_nonzero(c, m->tpos.as<double>(), m->ntri, 3, &ft))) return r;
+    if (ft >= 0) {
+        if ((r = mesh_box_from(c, m->tpos.as<double>(), m->ntri, 3, ft, box))) return r;
+        if (m->l_has[0] && m->nline > 0 && (r = mesh_box_from(c, m->lpos.as<double>(), m->nline, 2, 0, box)))
+            return r;
+        return FGL_OK;
+    }
+    if (m->l_has[0] && (r = mesh_first_nonzero(c, m->lpos.as<double>(), m->nline, 2, &fl))) return r;
+    if (fl >= 0) return mesh_box_from(c, m->lpos.as<double>(), m->nline, 2, fl, box);
+    for (int k = 0; k < 6; k++) box[k] = 0.0;  // EmptyBox
+    return FGL_OK;
+}
+
+// a zero-filled array for an attribute the mesh was created without
+int mesh_materialize(fgl_ctx *c, DevBuf &b, bool &has, int64_t ndoubles) {
+    if (has || ndoubles == 0) return FGL_OK;
+    HIPCHK(c, b.ensure((size_t)ndoubles * 8));
+    HIPCHK(c, hipMemsetAsync(b.p, 0, (size_t)ndoubles * 8, c->stream));
+    has = true;
+    return FGL_OK;
+}
+
+int mesh_transform(fgl_mesh *m, const double M[16]) {
+    fgl_ctx *c = m->ctx;
+    int r;
+    const int64_t ntv = m->ntri * 3, nlv = m->nline * 2;
+    if ((r = mesh_materialize(c, m->tpos, m->t_has[0], ntv * 3))) return r;
+    if ((r = mesh_materialize(c, m->tnrm, m->t_has[1], ntv * 3))) return r;
+    if ((r = mesh_materialize(c, m->lpos, m->l_has[0], nlv * 3))) return r;
+    if ((r = mesh_materialize(c, m->lnrm, m->l_has[1], nlv * 3))) return r;
+    if (ntv > 0 && (r = launch(c, "mesh_transform", [&] {
+                        return launch_mesh_transform(m->tpos.as<double>(), m->tnrm.as<double>(), ntv, M, c->stream);
+                    })))
+        return r;
+    if (nlv > 0 && (r = launch(c, "mesh_transform", [&] {
+                        return launch_mesh_transform(m->lpos.as<double>(), m->lnrm.as<double>(), nlv, M, c->stream);
+                    })))
+        return r;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FGL_OK;
+}
+
+int mesh_smooth(fgl_mesh *m, bool threshold_mode, double threshold) {
+    fgl_ctx *c = m->ctx;
+    const int64_t nv = m->ntri * 3;
+    if (nv == 0) return FGL_OK;
+    if (nv > (int64_t)UINT32_MAX) {
+        set_err(c->err, "mesh smoothing: %lld vertices exceed 2^32", (long long)nv);
+        return FGL_E_INVALID;
+    }
+    int r;
+    if ((r = mesh_materialize(c, m->tpos, m->t_has[0], nv * 3))) return r;
+    if ((r = mesh_materialize(c, m->tnrm, m->t_has[1], nv * 3))) return r;
+    const int32_t nb = mesh_sort_blocks(nv);
+    DevBuf k0, k1, v0, v1, hist, dtot, where, out;
+    HIPCHK(c, k0.ensure((size_t)nv * 4));
+    HIPCHK(c, k1.ensure((size_t)nv * 4));
+    HIPCHK(c, v0.ensure((size_t)nv * 4));
+    HIPCHK(c, v1.ensure((size_t)nv * 4));
+    HIPCHK(c, hist.ensure((size_t)256 * nb * 4));
+    HIPCHK(c, dtot.ensure(256 * 4));
+    HIPCHK(c, where.ensure((size_t)nv * 4));
+    HIPCHK(c, out.ensure((size_t)nv * 3 * 8));
+    uint32_t *keys[2] = {k0.as<uint32_t>(), k1.as<uint32_t>()};
+    uint32_t *vals[2] = {v0.as<uint32_t>(), v1.as<uint32_t>()};
+    if ((r = launch(c, "mesh_group", [&] {
+             return launch_mesh_group(m->tpos.as<double>(), nv, keys, vals, hist.as<uint32_t>(), dtot.as<uint32_t>(),
+                                      where.as<uint32_t>(), c->stream);
+         })))
+        return r;
+    if ((r = launch(c, "mesh_smooth", [&] {
+             return launch_mesh_smooth(threshold_mode, m->tpos.as<double>(), m->tnrm.as<double>(), nv, keys[0],
+                                       vals[0], where.as<uint32_t>(), threshold, out.as<double>(), c->stream);
+         })))
+        return r;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::swap(m->tnrm.p, out.p);
+    std::swap(m->tnrm.bytes, out.bytes);
+    return FGL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fgl_mesh_counts(const fgl_mesh *mesh, int64_t *ntri, int64_t *nline) {
+    if (!mesh || !ntri || !nline) return FGL_E_INVALID;
+    *ntri = mesh->ntri;
+    *nline = mesh->nline;
+    return FGL_OK;
+}
+
+int fgl_mesh_read(const fgl_mesh *mesh, int32_t lines, int32_t attr, double *out, int64_t n) {
+    if (!mesh || !mesh->ctx || !out || attr < 0 || attr > 3) return FGL_E_INVALID;
+    fgl_ctx *c = mesh->ctx;
+    static const int kWidth[4] = {3, 3, 2, 4};
+    const int64_t nvtx = lines ? mesh->nline * 2 : mesh->ntri * 3;
+    const int64_t need = nvtx * kWidth[attr];
+    if (n != need) {
+        set_err(c->err, "fgl_mesh_read: needs %lld doubles", (long long)need);
+        return FGL_E_INVALID;
+    }
+    if (need == 0) return FGL_OK;
+    const DevBuf *bufs[2][4] = {{&mesh->tpos, &mesh->tnrm, &mesh->tuv, &mesh->tcol},
+                                {&mesh->lpos, &mesh->lnrm, &mesh->luv, &mesh->lcol}};
+    const bool has = lines ? mesh->l_has[attr] : mesh->t_has[attr];
+    HIPCHK(c, hipSetDevice(c->device));
+    if (!has) {
+        for (int64_t i = 0; i < need; i++) out[i] = 0.0;
+        return FGL_OK;
+    }
+    HIPCHK(c, hipMemcpyAsync(out, bufs[lines ? 1 : 0][attr]->p, (size_t)need * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return FGL_OK;
+}
+
+int fgl_mesh_bounding_box(fgl_mesh *mesh, double box[6]) {
+    if (check_mesh(mesh) || !box) return FGL_E_INVALID;
+    HIPCHK(mesh->ctx, hipSetDevice(mesh->ctx->device));
+    return mesh_bbox(mesh, box);
+}
+
+int fgl_mesh_transform(fgl_mesh *mesh, const double matrix[16]) {
+    if (check_mesh(mesh) || !matrix) return FGL_E_INVALID;
+    HIPCHK(mesh->ctx, hipSetDevice(mesh->ctx->device));
+    return mesh_transform(mesh, matrix);
+}
+
+int fgl_mesh_fit_inside(fgl_mesh *mesh, const double box[6], const double anchor[3], double matrix_out[16]) {
+    if (check_mesh(mesh) || !box || !anchor) return FGL_E_INVALID;
+    HIPCHK(mesh->ctx, hipSetDevice(mesh->ctx->device));
+    double bb[6];
+    int r = mesh_bbox(mesh, bb);
+    if (r) return r;
+    double size[3], bsize[3], extra[3], q[3];
+    for (int k = 0; k < 3; k++) {
+        size[k] = bb[3 + k] - bb[k];
+        bsize[k] = box[3 + k] - box[k];
+        q[k] = bsize[k] / size[k];
+    }
+    const double scale = hgo_min(hgo_min(q[0], q[1]), q[2]);
+    for (int k = 0; k < 3; k++) extra[k] = bsize[k] - size[k] * scale;
+    const double I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    double T1[16] = {1, 0, 0, -bb[0], 0, 1, 0, -bb[1], 0, 0, 1, -bb[2], 0, 0, 0, 1};
+    double S[16] = {scale, 0, 0, 0, 0, scale, 0, 0, 0, 0, scale, 0, 0, 0, 0, 1};
+    double T2[16] = {1, 0, 0, box[0] + extra[0] * anchor[0], 0, 1, 0, box[1] + extra[1] * anchor[1],
+                     0, 0, 1, box[2] + extra[2] * anchor[2], 0, 0, 0, 1};
+    double m1[16], m2[16], m3[16];
+    mat_mul(T1, I, m1);
+    mat_mul(S, m1, m2);
+    mat_mul(T2, m2, m3);
+    if ((r = mesh_transform(mesh, m3))) return r;
+    if (matrix_out)
+        for (int k = 0; k < 16; k++) matrix_out[k] = m3[k];
+    return FGL_OK;
+}
+
+int fgl_mesh_smooth_normals(fgl_mesh *mesh) {
+    if (check_mesh(mesh)) return FGL_E_INVALID;
+    HIPCHK(mesh->ctx, hipSetDevice(mesh->ctx->device));
+    return mesh_smooth(mesh, false, 0.0);
+}
+
+int fgl_mesh_smooth_normals_threshold(fgl_mesh *mesh, double cos_threshold) {
+    if (check_mesh(mesh)) return FGL_E_INVALID;
+    HIPCHK(mesh->ctx, hipSetDevice(mesh->ctx->device));
+    return mesh_smooth(mesh, true, cos_threshold);
+}
+
+}  // extern "C"

@@ -132,6 +132,19 @@ hipError_t launch_nn_features(const uint32_t *color, const double *depth, const 
 hipError_t launch_nn_conv(int epi, const uint16_t *in, int32_t Cin, const uint16_t *w, const float *bias, int32_t H,
                           int32_t W, int32_t Cout, void *out, const uint16_t *mask, const uint32_t *color,
                           const double *depth, const double *range, hipStream_t s);
+// mesh preparation (fgl_mesh.hip)
+hipError_t launch_mesh_first_nonzero(const double *pos, int64_t nprim, int32_t nv, int64_t *block_first,
+                                     int32_t nblocks, hipStream_t s);
+hipError_t launch_mesh_box(const double *pos, int64_t nprim, int32_t nv, int64_t from, double *block_box,
+                           int32_t nblocks, hipStream_t s);
+hipError_t launch_mesh_transform(double *pos, double *nrm, int64_t nvert, const double m[16], hipStream_t s);
+int32_t mesh_sort_blocks(int64_t n);
+hipError_t launch_mesh_group(const double *pos, int64_t nvert, uint32_t *keys[2], uint32_t *vals[2], uint32_t *hist,
+                             uint32_t *dtot, uint32_t *where, hipStream_t s);
+hipError_t launch_mesh_smooth(bool threshold_mode, const double *pos, const double *nrm, int64_t nvert,
+                              const uint32_t *skeys, const uint32_t *svals, const uint32_t *where, double threshold,
+                              double *out, hipStream_t s);
+
 // two fused layers (inference): cin_a -> 32 (ReLU) -> cout_b with epilogue
 // epi_b (0: ReLU -> bf16, 1: f32, 3: final color); in == nullptr: layer A
 // reads the features computed from color, depth and range

@@ -272,17 +272,23 @@ def sphere_and_cube_mesh(detail=5):
     return Mesh(concat(sphere, cube))
 
 
-def noise_sphere_mesh(ntri=871306, detail=8, amplitude=0.15, seed=1, smooth_degrees=30.0):
+def noise_sphere_prims(ntri=871306, detail=8, amplitude=0.15, seed=1):
     """Icosphere NewSphere(detail) truncated to ntri triangles in generation
     order, each vertex radius displaced by 1 + amplitude * fbm6(p, seed)
-    (a function of position, so shared vertices stay shared), face normals,
-    BiUnitCube, SmoothNormalsThreshold(Radians(smooth_degrees))."""
+    (a function of position, so shared vertices stay shared), face normals:
+    the noise sphere before its mesh preparation."""
     pos = sphere_positions(detail)[:ntri]
     flat = pos.reshape(-1, 3) + 0.0
     uniq, inv = np.unique(flat, axis=0, return_inverse=True)
     disp = 1.0 + amplitude * fbm(uniq, 6, seed)
     moved = (uniq * disp[:, None])[inv.reshape(-1)].reshape(pos.shape)
-    p = Prims(moved, with_face_normals(moved))
+    return Prims(moved, with_face_normals(moved))
+
+
+def noise_sphere_mesh(ntri=871306, detail=8, amplitude=0.15, seed=1, smooth_degrees=30.0):
+    """noise_sphere_prims, then BiUnitCube and
+    SmoothNormalsThreshold(Radians(smooth_degrees))."""
+    p = noise_sphere_prims(ntri, detail, amplitude, seed)
     p = bi_unit_cube(p)
     p = smooth_normals_threshold(p, Radians(smooth_degrees))
     return Mesh(p)

@@ -163,6 +163,32 @@ void fgl_mesh_destroy(fgl_mesh *mesh);
 int fgl_draw_mesh(fgl_ctx *ctx, const fgl_state *state, const fgl_shader *shader, const fgl_mesh *mesh,
                   uint64_t info[2]);
 
+/* ---- device-side mesh preparation on a resident mesh (in place) ---- */
+
+/* Primitive counts of a resident mesh. */
+int fgl_mesh_counts(const fgl_mesh *mesh, int64_t *ntri, int64_t *nline);
+/* Copies one attribute out (lines = 0: triangles, 1: lines; attr 0..3 =
+ * position, normal, texture, color in the fgl_prims layout; n = the number
+ * of doubles).  An attribute the mesh was created without reads as zeros. */
+int fgl_mesh_read(const fgl_mesh *mesh, int32_t lines, int32_t attr, double *out, int64_t n);
+/* Mesh.BoundingBox (mesh.go:153-165): box = {min x, y, z, max x, y, z},
+ * Box.Extend semantics (box.go:48-53: a running box equal to EmptyBox is
+ * replaced) and math.Min / Max. */
+int fgl_mesh_bounding_box(fgl_mesh *mesh, double box[6]);
+/* Mesh.Transform (mesh.go:167-175): positions by MulPosition, normals by
+ * MulDirection (matrix.go:209-229); matrix row-major X00..X33. */
+int fgl_mesh_transform(fgl_mesh *mesh, const double matrix[16]);
+/* Mesh.FitInside (mesh.go:142-151) into box {min, max} at anchor; writes the
+ * matrix it applied to matrix_out when not NULL.  BiUnitCube / UnitCube
+ * (mesh.go:127-135) are box +-1 / +-0.5 with anchor 0.5. */
+int fgl_mesh_fit_inside(fgl_mesh *mesh, const double box[6], const double anchor[3], double matrix_out[16]);
+/* Mesh.SmoothNormals (mesh.go:105-125). */
+int fgl_mesh_smooth_normals(fgl_mesh *mesh);
+/* Mesh.SmoothNormalsThreshold (mesh.go:91-103) given the threshold the
+ * reference computes first, math.Cos(radians) (the caller evaluates it, so
+ * that the value is Go's bit for bit). */
+int fgl_mesh_smooth_normals_threshold(fgl_mesh *mesh, double cos_threshold);
+
 /* ---- readback and host writes (the public ColorBuffer/DepthBuffer fields) ---- */
 
 /* Image() (context.go:83-85): copies the 4*W*H NRGBA bytes out. */

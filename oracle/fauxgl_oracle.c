@@ -1234,3 +1234,181 @@ void orc_resolve(const uint8_t *src, int32_t sw, int32_t sh, uint8_t *dst, int32
     free(oy);
     free(mid);
 }
+
+/* ========================================================================
+ * Mesh preparation (mesh.go)
+ * ======================================================================== */
+
+/* Box.Extend (box.go:48-53) with the triangle / line box (triangle.go:60-64,
+ * line.go:17-21), folded in primitive order (mesh.go:153-165) */
+static void box_extend_prims(const double *pos, int64_t nprim, int nv, double b[6]) {
+    for (int64_t p = 0; p < nprim; p++) {
+        const double *q = pos + 3 * nv * p;
+        double mn[3] = {q[0], q[1], q[2]}, mx[3] = {q[0], q[1], q[2]};
+        for (int k = 1; k < nv; k++)
+            for (int c = 0; c < 3; c++) {
+                mn[c] = go_min(mn[c], q[3 * k + c]);
+                mx[c] = go_max(mx[c], q[3 * k + c]);
+            }
+        int empty = 1; /* a == EmptyBox: every field == 0 */
+        for (int c = 0; c < 6; c++) empty &= b[c] == 0.0;
+        if (empty) {
+            for (int c = 0; c < 3; c++) {
+                b[c] = mn[c];
+                b[3 + c] = mx[c];
+            }
+        } else {
+            for (int c = 0; c < 3; c++) {
+                b[c] = go_min(b[c], mn[c]);
+                b[3 + c] = go_max(b[3 + c], mx[c]);
+            }
+        }
+    }
+}
+
+void orc_mesh_bounding_box(const double *tpos, int64_t ntri, const double *lpos, int64_t nline, double box[6]) {
+    for (int c = 0; c < 6; c++) box[c] = 0.0; /* EmptyBox */
+    if (tpos) box_extend_prims(tpos, ntri, 3, box);
+    if (lpos) box_extend_prims(lpos, nline, 2, box);
+}
+
+/* MulPosition / MulDirection (matrix.go:209-229), Normalize (vector.go:83-86) */
+void orc_mesh_transform(double *pos, double *nrm, int64_t nvert, const double a[16]) {
+    for (int64_t v = 0; v < nvert; v++) {
+        double *P = pos + 3 * v, *N = nrm + 3 * v;
+        const double x = P[0], y = P[1], z = P[2];
+        P[0] = a[0] * x + a[1] * y + a[2] * z + a[3];
+        P[1] = a[4] * x + a[5] * y + a[6] * z + a[7];
+        P[2] = a[8] * x + a[9] * y + a[10] * z + a[11];
+        const double nx = N[0], ny = N[1], nz = N[2];
+        const double dx = a[0] * nx + a[1] * ny + a[2] * nz;
+        const double dy = a[4] * nx + a[5] * ny + a[6] * nz;
+        const double dz = a[8] * nx + a[9] * ny + a[10] * nz;
+        const double r = 1 / sqrt(dx * dx + dy * dy + dz * dz);
+        N[0] = dx * r;
+        N[1] = dy * r;
+        N[2] = dz * r;
+    }
+}
+
+static void mat_mul16(const double a[16], const double b[16], double m[16]) { /* matrix.go:188-207 */
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++)
+            m[4 * i + j] = a[4 * i] * b[j] + a[4 * i + 1] * b[4 + j] + a[4 * i + 2] * b[8 + j] + a[4 * i + 3] * b[12 + j];
+}
+
+void orc_mesh_fit_inside(double *tpos, double *tnrm, int64_t ntri, double *lpos, double *lnrm, int64_t nline,
+                         const double box[6], const double anchor[3], double m_out[16]) {
+    double bb[6];
+    orc_mesh_bounding_box(tpos, ntri, lpos, nline, bb);
+    Vec size = {bb[3] - bb[0], bb[4] - bb[1], bb[5] - bb[2]};          /* Box.Size, box.go:44-46 */
+    Vec bsize = {box[3] - box[0], box[4] - box[1], box[5] - box[2]};
+    Vec q = {bsize.X / size.X, bsize.Y / size.Y, bsize.Z / size.Z};  /* Vector.Div */
+    const double scale = go_min(go_min(q.X, q.Y), q.Z);               /* MinComponent */
+    Vec extra = {bsize.X - size.X * scale, bsize.Y - size.Y * scale, bsize.Z - size.Z * scale};
+    const double I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    const double T1[16] = {1, 0, 0, -bb[0], 0, 1, 0, -bb[1], 0, 0, 1, -bb[2], 0, 0, 0, 1};
+    const double S[16] = {scale, 0, 0, 0, 0, scale, 0, 0, 0, 0, scale, 0, 0, 0, 0, 1};
+    const double T2[16] = {1, 0, 0, box[0] + extra.X * anchor[0], 0, 1, 0, box[1] + extra.Y * anchor[1],
+                           0, 0, 1, box[2] + extra.Z * anchor[2], 0, 0, 0, 1};
+    double m1[16], m2[16], m3[16];
+    mat_mul16(T1, I, m1); /* Matrix.Translate(v) = Translate(v).Mul(m), matrix.go:143-145 */
+    mat_mul16(S, m1, m2);
+    mat_mul16(T2, m2, m3);
+    if (tpos) orc_mesh_transform(tpos, tnrm, 3 * ntri, m3);
+    if (lpos) orc_mesh_transform(lpos, lnrm, 2 * nline, m3);
+    if (m_out) memcpy(m_out, m3, sizeof m3);
+}
+
+/* The map[Vector][]Vector of mesh.go:92-97 as groups of equal positions:
+ * vertex indices sorted by position (-0 == +0; a position with a NaN is
+ * its own key and is never found again), ties by index, so each group
+ * lists its vertices in the order the reference appends them. */
+static const double *g_sort_pos;
+static int has_nan3(const double *p) { return p[0] != p[0] || p[1] != p[1] || p[2] != p[2]; }
+static int cmp_vertex(const void *pa, const void *pb) {
+    const int64_t a = *(const int64_t *)pa, b = *(const int64_t *)pb;
+    const double *A = g_sort_pos + 3 * a, *B = g_sort_pos + 3 * b;
+    const int na = has_nan3(A), nb = has_nan3(B);
+    if (na != nb) return na - nb;
+    if (!na)
+        for (int c = 0; c < 3; c++) {
+            if (A[c] < B[c]) return -1;
+            if (A[c] > B[c]) return 1;
+        }
+    return a < b ? -1 : (a > b ? 1 : 0);
+}
+static int same_pos(const double *A, const double *B) { return A[0] == B[0] && A[1] == B[1] && A[2] == B[2]; }
+
+static void smooth(const double *pos, double *nrm, int64_t ntri, int threshold_mode, double threshold) {
+    const int64_t n = 3 * ntri;
+    if (n == 0) return;
+    int64_t *ord = malloc((size_t)n * sizeof *ord);
+    double *out = malloc((size_t)n * 3 * sizeof *out);
+    for (int64_t i = 0; i < n; i++) ord[i] = i;
+    g_sort_pos = pos;
+    qsort(ord, (size_t)n, sizeof *ord, cmp_vertex);
+    int64_t g0 = 0;
+    while (g0 < n) {
+        int64_t g1 = g0 + 1;
+        while (g1 < n && same_pos(pos + 3 * ord[g0], pos + 3 * ord[g1])) g1++;
+        if (!same_pos(pos + 3 * ord[g0], pos + 3 * ord[g0])) {
+            /* NaN key: its list is never found.  SmoothNormals reads the zero
+             * Vector (mesh.go:121-123); the threshold version normalizes the
+             * empty sum (mesh.go:81-88). */
+            for (int64_t k = g0; k < g1; k++) {
+                double *o = out + 3 * ord[k];
+                if (threshold_mode) {
+                    const double r = 1 / sqrt(0.0);
+                    o[0] = 0.0 * r;
+                    o[1] = 0.0 * r;
+                    o[2] = 0.0 * r;
+                } else {
+                    o[0] = o[1] = o[2] = 0.0;
+                }
+            }
+            g0 = g1;
+            continue;
+        }
+        if (!threshold_mode) {
+            /* lookup[p] = lookup[p].Add(n) in order, then Normalize (mesh.go:107-117) */
+            Vec sum = {0, 0, 0};
+            for (int64_t k = g0; k < g1; k++) {
+                const double *x = nrm + 3 * ord[k];
+                sum = (Vec){sum.X + x[0], sum.Y + x[1], sum.Z + x[2]};
+            }
+            const double r = 1 / sqrt(sum.X * sum.X + sum.Y * sum.Y + sum.Z * sum.Z);
+            for (int64_t k = g0; k < g1; k++) {
+                double *o = out + 3 * ord[k];
+                o[0] = sum.X * r;
+                o[1] = sum.Y * r;
+                o[2] = sum.Z * r;
+            }
+        } else {
+            /* smoothNormalsThreshold (mesh.go:80-89) per vertex of the group */
+            for (int64_t k = g0; k < g1; k++) {
+                const double *own = nrm + 3 * ord[k];
+                Vec sum = {0, 0, 0};
+                for (int64_t j = g0; j < g1; j++) {
+                    const double *x = nrm + 3 * ord[j];
+                    if (x[0] * own[0] + x[1] * own[1] + x[2] * own[2] >= threshold)
+                        sum = (Vec){sum.X + x[0], sum.Y + x[1], sum.Z + x[2]};
+                }
+                const double r = 1 / sqrt(sum.X * sum.X + sum.Y * sum.Y + sum.Z * sum.Z);
+                double *o = out + 3 * ord[k];
+                o[0] = sum.X * r;
+                o[1] = sum.Y * r;
+                o[2] = sum.Z * r;
+            }
+        }
+        g0 = g1;
+    }
+    memcpy(nrm, out, (size_t)n * 3 * sizeof *out);
+    free(out);
+    free(ord);
+}
+
+void orc_mesh_smooth_normals(const double *tpos, double *tnrm, int64_t ntri) { smooth(tpos, tnrm, ntri, 0, 0.0); }
+void orc_mesh_smooth_normals_threshold(const double *tpos, double *tnrm, int64_t ntri, double threshold) {
+    smooth(tpos, tnrm, ntri, 1, threshold);
+}

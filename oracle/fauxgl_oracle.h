@@ -160,6 +160,22 @@ int32_t orc_tent_weights(int32_t src, int32_t dst, int16_t *w, int32_t *start);
 void orc_rasterize_screen(orc_ctx *c, const double s0[3], const double s1[3],
                           const double s2[3], const double w[3], uint64_t info[2]);
 
+/* Mesh preparation (mesh.go), on SoA vertex arrays (vertex v at [3v, 3v+3);
+ * triangles have 3 vertices, lines 2).  Parity pins: tests/test_oracle_mesh.py
+ * (a dict-based brute force of mesh.go's map code, the numpy implementation
+ * in fauxgl_amd/meshes.py, closed forms and special cases). */
+/* Mesh.BoundingBox (mesh.go:153-165; Box.Extend box.go:48-53). */
+void orc_mesh_bounding_box(const double *tpos, int64_t ntri, const double *lpos, int64_t nline, double box[6]);
+/* Triangle/Line.Transform (triangle.go:66-73, line.go:23-28) of nvert vertices. */
+void orc_mesh_transform(double *pos, double *nrm, int64_t nvert, const double m[16]);
+/* Mesh.FitInside (mesh.go:142-151): the matrix it builds, then Transform. */
+void orc_mesh_fit_inside(double *tpos, double *tnrm, int64_t ntri, double *lpos, double *lnrm, int64_t nline,
+                         const double box[6], const double anchor[3], double m_out[16]);
+/* Mesh.SmoothNormals (mesh.go:105-125) and SmoothNormalsThreshold
+ * (mesh.go:80-103, threshold = math.Cos(radians)) on triangle vertices. */
+void orc_mesh_smooth_normals(const double *tpos, double *tnrm, int64_t ntri);
+void orc_mesh_smooth_normals_threshold(const double *tpos, double *tnrm, int64_t ntri, double threshold);
+
 #ifdef __cplusplus
 }
 #endif

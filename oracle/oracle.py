@@ -103,6 +103,11 @@ def lib():
         "orc_bilinear_sample": (None, [_P(OrcTexture), _d, _d, dp]),
         "orc_phong_fragment": (None, [_P(OrcShader), dp, dp, dp, dp, dp]),
         "orc_rasterize_screen": (None, [vp, dp, dp, dp, dp, u64p]),
+        "orc_mesh_bounding_box": (None, [dp, ctypes.c_int64, dp, ctypes.c_int64, dp]),
+        "orc_mesh_transform": (None, [dp, dp, ctypes.c_int64, dp]),
+        "orc_mesh_fit_inside": (None, [dp, dp, ctypes.c_int64, dp, dp, ctypes.c_int64, dp, dp, dp]),
+        "orc_mesh_smooth_normals": (None, [dp, dp, ctypes.c_int64]),
+        "orc_mesh_smooth_normals_threshold": (None, [dp, dp, ctypes.c_int64, _d]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -396,3 +401,53 @@ def tent_weights(src, dst):
     st = np.zeros(dst, dtype=np.int32)
     lib().orc_tent_weights(src, dst, w.ctypes.data_as(_P(ctypes.c_int16)), st.ctypes.data_as(_P(ctypes.c_int32)))
     return w, st
+
+
+# ---------------------------------------------------------------------------
+# mesh preparation (mesh.go); arrays are (n, nv, 3) float64, returned as copies
+
+
+def _c(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64).copy()
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_P(_d))
+
+
+def mesh_bounding_box(tpos, lpos=None):
+    """Mesh.BoundingBox -> (min (3,), max (3,))."""
+    t, l = _c(tpos), _c(lpos)
+    box = np.zeros(6)
+    lib().orc_mesh_bounding_box(_p(t), 0 if t is None else t.shape[0], _p(l), 0 if l is None else l.shape[0], _p(box))
+    return box[:3], box[3:]
+
+
+def mesh_transform(pos, nrm, m):
+    """Triangle/Line.Transform of every vertex -> (pos, nrm)."""
+    P, Nn = _c(pos), _c(nrm)
+    M = np.ascontiguousarray(m, dtype=np.float64)
+    lib().orc_mesh_transform(_p(P), _p(Nn), P.size // 3, _p(M))
+    return P, Nn
+
+
+def mesh_fit_inside(tpos, tnrm, box_min, box_max, anchor, lpos=None, lnrm=None):
+    """Mesh.FitInside -> (tpos, tnrm, lpos, lnrm, matrix (16,))."""
+    tp, tn, lp, ln = _c(tpos), _c(tnrm), _c(lpos), _c(lnrm)
+    box = np.array(list(box_min) + list(box_max), dtype=np.float64)
+    a = np.array(anchor, dtype=np.float64)
+    m = np.zeros(16)
+    lib().orc_mesh_fit_inside(_p(tp), _p(tn), 0 if tp is None else tp.shape[0], _p(lp), _p(ln),
+                              0 if lp is None else lp.shape[0], _p(box), _p(a), _p(m))
+    return tp, tn, lp, ln, m
+
+
+def mesh_smooth_normals(tpos, tnrm, threshold=None):
+    """Mesh.SmoothNormals (threshold None) or SmoothNormalsThreshold with
+    threshold = cos(radians) -> new normals."""
+    tp, tn = _c(tpos), _c(tnrm)
+    if threshold is None:
+        lib().orc_mesh_smooth_normals(_p(tp), _p(tn), tp.shape[0])
+    else:
+        lib().orc_mesh_smooth_normals_threshold(_p(tp), _p(tn), tp.shape[0], float(threshold))
+    return tn
