@@ -433,8 +433,11 @@ def NewLineMesh(lines):
 class DeviceMesh:
     """A Mesh uploaded once to a context's device (fgl_mesh)."""
 
-    def __init__(self, ctx, mesh):
+    def __init__(self, ctx, mesh, handle=None):
         self.ctx = ctx
+        if handle is not None:
+            self.h = handle
+            return
         t = mesh.Triangles.struct() if mesh.Triangles is not None else None
         l = mesh.Lines.struct() if mesh.Lines is not None else None
         h = ctypes.c_void_p()
@@ -529,6 +532,32 @@ class DeviceMesh:
         tris = attrs(0, nt.value, 3) if nt.value else None
         lines = attrs(1, nl.value, 2) if nl.value else None
         return Mesh(tris, lines)
+
+
+def _file_bytes(src):
+    if isinstance(src, (bytes, bytearray, memoryview)):
+        return bytes(src)
+    with open(src, "rb") as f:
+        return f.read()
+
+
+def LoadSTL(ctx, src):
+    """LoadSTL (stl.go:23-57) into a mesh resident on ctx's device; src is a
+    path or the file's bytes."""
+    data = _file_bytes(src)
+    buf = (ctypes.c_uint8 * max(len(data), 1)).from_buffer_copy(data or b"\0")
+    h = ctypes.c_void_p()
+    N.check(N.load().fgl_mesh_load_stl(ctx._ptr(), buf, len(data), ctypes.byref(h)), ctx._ptr())
+    return DeviceMesh(ctx, None, handle=h)
+
+
+def LoadOBJ(ctx, src):
+    """LoadOBJ (obj.go:19-79) into a mesh resident on ctx's device; src is a
+    path or the file's bytes / text."""
+    data = src.encode() if isinstance(src, str) and "\n" in src else _file_bytes(src)
+    h = ctypes.c_void_p()
+    N.check(N.load().fgl_mesh_load_obj(ctx._ptr(), data, len(data), ctypes.byref(h)), ctx._ptr())
+    return DeviceMesh(ctx, None, handle=h)
 
 
 # ---------------------------------------------------------------------------

@@ -1676,3 +1676,181 @@ int fgl_mesh_smooth_normals_threshold(fgl_mesh *mesh, double cos_threshold) {
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Mesh ingest (fgl_ingest.hip): LoadSTL, LoadOBJ into a resident mesh.
+
+namespace {
+
+// a triangle mesh with device positions and normals (and uv when given)
+// from host arrays, then FixNormals on the device
+int mesh_from_host(fgl_ctx *c, const std::vector<double> &pos, const std::vector<double> &nrm,
+                   const std::vector<double> *uv, fgl_mesh **out) {
+    const int64_t nt = (int64_t)(pos.size() / 9);
+    fgl_prims t{};
+    t.count = nt;
+    t.position = pos.data();
+    t.normal = nrm.data();
+    t.texture = uv ? uv->data() : nullptr;
+    fgl_mesh *m = nullptr;
+    int r = fgl_mesh_create(c, nt ? &t : nullptr, nullptr, &m);
+    if (r) return r;
+    if (nt > 0 && (r = launch(c, "fix_normals", [&] {
+                       return launch_fix_normals(m->tpos.as<double>(), m->tnrm.as<double>(), nt, c->stream);
+                   }))) {
+        fgl_mesh_destroy(m);
+        return r;
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess) {
+        fgl_mesh_destroy(m);
+        set_err(c->err, "mesh ingest: synchronize failed");
+        return FGL_E_DEVICE;
+    }
+    *out = m;
+    return FGL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fgl_mesh_load_stl(fgl_ctx *ctx, const uint8_t *data, int64_t size, fgl_mesh **out) {
+    if (check_ctx(ctx) || !out || (!data && size > 0) || size < 0) return FGL_E_INVALID;
+    *out = nullptr;
+    if (size < 84) {
+        ctx->err = "fgl_mesh_load_stl: shorter than the 84-byte header";
+        return FGL_E_INVALID;
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const uint32_t count = (uint32_t)data[80] | ((uint32_t)data[81] << 8) | ((uint32_t)data[82] << 16) |
+                           ((uint32_t)data[83] << 24);
+    if ((int64_t)count * 50 + 84 == size) {
+        // binary: the bytes go to the device, records decode there
+        if ((int64_t)count > kMaxPrims) {
+            set_err(ctx->err, "fgl_mesh_load_stl: %u triangles exceed the limit", count);
+            return FGL_E_INVALID;
+        }
+        fgl_mesh *m = new (std::nothrow) fgl_mesh;
+        if (!m) return FGL_E_NOMEM;
+        m->ctx = ctx;
+        m->ntri = count;
+        DevBuf file;
+        auto fail = [&](int code) {
+            delete m;
+            return code;
+        };
+        if (count == 0) {
+            *out = m;
+            return FGL_OK;
+        }
+        if (file.ensure((size_t)size) != hipSuccess || m->tpos.ensure((size_t)count * 72) != hipSuccess ||
+            m->tnrm.ensure((size_t)count * 72) != hipSuccess) {
+            set_err(ctx->err, "fgl_mesh_load_stl: device allocation failed");
+            return fail(FGL_E_NOMEM);
+        }
+        if (hipMemcpy(file.p, data, (size_t)size, hipMemcpyHostToDevice) != hipSuccess) {
+            set_err(ctx->err, "fgl_mesh_load_stl: upload failed");
+            return fail(FGL_E_DEVICE);
+        }
+        int r = launch(ctx, "stl_decode", [&] {
+            return launch_stl_decode(file.as<uint8_t>(), count, m->tpos.as<double>(), m->tnrm.as<double>(),
+                                     ctx->stream);
+        });
+        if (r) return fail(r);
+        if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+            set_err(ctx->err, "fgl_mesh_load_stl: synchronize failed");
+            return fail(FGL_E_DEVICE);
+        }
+        m->t_has[0] = m->t_has[1] = true;
+        *out = m;
+        return FGL_OK;
+    }
+    // ASCII (stl.go:59-84): lines of exactly "vertex x y z"
+    std::vector<double> pos, nrm;
+    std::vector<std::pair<const char *, size_t>> f;
+    const char *t = reinterpret_cast<const char *>(data), *end = t + size;
+    for (const char *a = t; a < end;) {
+        const char *b = static_cast<const char *>(memchr(a, '\n', (size_t)(end - a)));
+        if (!b) b = end;
+        ingest_fields(a, b, f);
+        if (f.size() == 4 && f[0].second == 6 && !memcmp(f[0].first, "vertex", 6))
+            for (int k = 1; k < 4; k++) pos.push_back(ingest_parse_float(f[k].first, f[k].second));
+        a = b + 1;
+    }
+    if ((pos.size() / 3) % 3) {
+        ctx->err = "fgl_mesh_load_stl: ASCII vertex count is not a multiple of 3";
+        return FGL_E_INVALID;
+    }
+    nrm.assign(pos.size(), 0.0);
+    return mesh_from_host(ctx, pos, nrm, nullptr, out);
+}
+
+int fgl_mesh_load_obj(fgl_ctx *ctx, const char *text, int64_t size, fgl_mesh **out) {
+    if (check_ctx(ctx) || !out || (!text && size > 0) || size < 0) return FGL_E_INVALID;
+    *out = nullptr;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    // 1-based lists with a zero Vector at index 0 (obj.go:25-27)
+    std::vector<double> vs(3, 0.0), vts(3, 0.0), vns(3, 0.0);
+    std::vector<double> pos, nrm, uv;
+    std::vector<std::pair<const char *, size_t>> f;
+    std::vector<int64_t> idx;
+    const char *end = text + size;
+    auto bad = [&](const char *msg) {
+        set_err(ctx->err, "fgl_mesh_load_obj: %s", msg);
+        return FGL_E_INVALID;
+    };
+    for (const char *a = text; a < end;) {
+        const char *b = static_cast<const char *>(memchr(a, '\n', (size_t)(end - a)));
+        if (!b) b = end;
+        ingest_fields(a, b, f);
+        a = b + 1;
+        if (f.empty()) continue;
+        const char *kw = f[0].first;
+        const size_t kl = f[0].second;
+        auto num = [&](size_t k) { return ingest_parse_float(f[k].first, f[k].second); };
+        if (kl == 1 && kw[0] == 'v') {
+            if (f.size() < 4) return bad("a v line with fewer than 3 numbers");
+            vs.insert(vs.end(), {num(1), num(2), num(3)});
+        } else if (kl == 2 && kw[0] == 'v' && kw[1] == 't') {
+            if (f.size() < 3) return bad("a vt line with fewer than 2 numbers");
+            vts.insert(vts.end(), {num(1), num(2), 0.0});
+        } else if (kl == 2 && kw[0] == 'v' && kw[1] == 'n') {
+            if (f.size() < 4) return bad("a vn line with fewer than 3 numbers");
+            vns.insert(vns.end(), {num(1), num(2), num(3)});
+        } else if (kl == 1 && kw[0] == 'f') {
+            const size_t na = f.size() - 1;
+            idx.assign(3 * na, 0);
+            const int64_t len[3] = {(int64_t)vs.size() / 3, (int64_t)vts.size() / 3, (int64_t)vns.size() / 3};
+            for (size_t k = 0; k < na; k++) {
+                // strings.Split(arg + "//", "/")[0..2]
+                const char *s = f[1 + k].first, *se = s + f[1 + k].second;
+                const char *piece = s;
+                for (int q = 0; q < 3; q++) {
+                    const char *pe = piece;
+                    while (pe < se && *pe != '/') ++pe;
+                    int64_t v = piece <= se ? ingest_parse_int(piece, (size_t)(pe - piece)) : 0;
+                    if (v < 0) v += len[q];
+                    idx[3 * k + q] = v;
+                    piece = pe < se ? pe + 1 : se + 1;
+                }
+            }
+            for (size_t i = 1; i + 1 < na; i++) {
+                const size_t c[3] = {0, i, i + 1};
+                for (int q = 0; q < 3; q++)
+                    for (int a3 = 0; a3 < 3; a3++) {
+                        const int64_t v = idx[3 * c[q] + a3];
+                        if (v < 0 || v >= len[a3]) return bad("index out of range");
+                    }
+                for (int q = 0; q < 3; q++) {
+                    const int64_t iv = idx[3 * c[q]], it = idx[3 * c[q] + 1], in = idx[3 * c[q] + 2];
+                    pos.insert(pos.end(), {vs[3 * iv], vs[3 * iv + 1], vs[3 * iv + 2]});
+                    nrm.insert(nrm.end(), {vns[3 * in], vns[3 * in + 1], vns[3 * in + 2]});
+                    uv.insert(uv.end(), {vts[3 * it], vts[3 * it + 1]});
+                }
+            }
+        }
+    }
+    return mesh_from_host(ctx, pos, nrm, &uv, out);
+}
+
+}  // extern "C"

@@ -5,6 +5,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+#include <vector>
+
 #include "fgl_device.h"
 
 namespace fgl {
@@ -132,6 +135,13 @@ hipError_t launch_nn_features(const uint32_t *color, const double *depth, const 
 hipError_t launch_nn_conv(int epi, const uint16_t *in, int32_t Cin, const uint16_t *w, const float *bias, int32_t H,
                           int32_t W, int32_t Cout, void *out, const uint16_t *mask, const uint32_t *color,
                           const double *depth, const double *range, hipStream_t s);
+// mesh ingest (fgl_ingest.hip)
+hipError_t launch_stl_decode(const uint8_t *file, int64_t count, double *pos, double *nrm, hipStream_t s);
+hipError_t launch_fix_normals(const double *pos, double *nrm, int64_t ntri, hipStream_t s);
+double ingest_parse_float(const char *s, size_t n);
+int64_t ingest_parse_int(const char *s, size_t n);
+void ingest_fields(const char *a, const char *b, std::vector<std::pair<const char *, size_t>> &out);
+
 // mesh preparation (fgl_mesh.hip)
 hipError_t launch_mesh_first_nonzero(const double *pos, int64_t nprim, int32_t nv, int64_t *block_first,
                                      int32_t nblocks, hipStream_t s);

@@ -189,6 +189,20 @@ int fgl_mesh_smooth_normals(fgl_mesh *mesh);
  * that the value is Go's bit for bit). */
 int fgl_mesh_smooth_normals_threshold(fgl_mesh *mesh, double cos_threshold);
 
+/* ---- mesh ingest into a resident mesh ---- */
+
+/* LoadSTL (stl.go:23-154) from the file's bytes: binary when size is
+ * 84 + 50 * (little-endian uint32 at byte 80), decoded on the device
+ * (float32 widened, face normals); otherwise ASCII "vertex x y z" lines.
+ * Errors where the reference errors or panics (size < 84, an ASCII vertex
+ * count that is not a multiple of 3). */
+int fgl_mesh_load_stl(fgl_ctx *ctx, const uint8_t *data, int64_t size, fgl_mesh **out);
+/* LoadOBJ (obj.go:19-79) from the file's text: v / vt / vn / f, fan
+ * triangulation, 1-based and negative indices, FixNormals on the device.
+ * Errors where the reference panics (an index out of range, a v / vt / vn
+ * line with too few numbers). */
+int fgl_mesh_load_obj(fgl_ctx *ctx, const char *text, int64_t size, fgl_mesh **out);
+
 /* ---- readback and host writes (the public ColorBuffer/DepthBuffer fields) ---- */
 
 /* Image() (context.go:83-85): copies the 4*W*H NRGBA bytes out. */

@@ -9,6 +9,7 @@
  */
 #include "fauxgl_oracle.h"
 
+#include <ctype.h>
 #include <float.h>
 #include <math.h>
 #include <pthread.h>
@@ -1411,4 +1412,309 @@ static void smooth(const double *pos, double *nrm, int64_t ntri, int threshold_m
 void orc_mesh_smooth_normals(const double *tpos, double *tnrm, int64_t ntri) { smooth(tpos, tnrm, ntri, 0, 0.0); }
 void orc_mesh_smooth_normals_threshold(const double *tpos, double *tnrm, int64_t ntri, double threshold) {
     smooth(tpos, tnrm, ntri, 1, threshold);
+}
+
+/* ========================================================================
+ * Mesh loaders (stl.go, obj.go)
+ * ======================================================================== */
+
+/* Triangle.Normal (triangle.go:33-37): Cross (vector.go:76-81), Normalize */
+static Vec tri_normal(Vec a, Vec b, Vec c) {
+    const Vec e1 = {b.X - a.X, b.Y - a.Y, b.Z - a.Z}, e2 = {c.X - a.X, c.Y - a.Y, c.Z - a.Z};
+    const Vec n = {e1.Y * e2.Z - e1.Z * e2.Y, e1.Z * e2.X - e1.X * e2.Z, e1.X * e2.Y - e1.Y * e2.X};
+    const double r = 1 / sqrt(n.X * n.X + n.Y * n.Y + n.Z * n.Z);
+    return (Vec){n.X * r, n.Y * r, n.Z * r};
+}
+
+static double le_f32(const uint8_t *b) {
+    uint32_t u = (uint32_t)b[0] | ((uint32_t)b[1] << 8) | ((uint32_t)b[2] << 16) | ((uint32_t)b[3] << 24);
+    float f;
+    memcpy(&f, &u, 4);
+    return (double)f;
+}
+
+static int is_space(char c) { return c == ' ' || c == '\t' || c == '\n' || c == '\v' || c == '\f' || c == '\r'; }
+
+/* strconv.ParseFloat(s, 64) with the error dropped (util.go:65-72) */
+static double parse_float_z(char *buf, size_t n);
+static double parse_float(const char *s, size_t n) {
+    if (n == 0) return 0.0;
+    char *buf = malloc(n + 1);
+    memcpy(buf, s, n);
+    buf[n] = 0;
+    const double v = parse_float_z(buf, n);
+    free(buf);
+    return v;
+}
+static double parse_float_z(char *buf, size_t n) {
+    (void)n;
+    size_t i = 0;
+    int neg = 0;
+    if (buf[i] == '+' || buf[i] == '-') neg = buf[i++] == '-';
+    if (!strcasecmp(buf + i, "inf") || !strcasecmp(buf + i, "infinity")) return neg ? -INFINITY : INFINITY;
+    if (!strcasecmp(buf + i, "nan")) return NAN;
+    if (buf[i] == '0' && (buf[i + 1] == 'x' || buf[i + 1] == 'X')) {
+        /* hexadecimal mantissa with a mandatory binary exponent */
+        size_t j = i + 2, nd = 0;
+        while (isxdigit((unsigned char)buf[j])) j++, nd++;
+        if (buf[j] == '.') {
+            j++;
+            while (isxdigit((unsigned char)buf[j])) j++, nd++;
+        }
+        if (nd == 0 || (buf[j] != 'p' && buf[j] != 'P')) return 0.0;
+        size_t k = j + 1, ne = 0;
+        if (buf[k] == '+' || buf[k] == '-') k++;
+        while (buf[k] >= '0' && buf[k] <= '9') k++, ne++;
+        if (ne == 0 || buf[k] != 0) return 0.0;
+        return strtod(buf, NULL);
+    }
+    /* decimal: digits [. digits] or . digits, then [eE][+-]digits */
+    size_t j = i, nd = 0;
+    while (buf[j] >= '0' && buf[j] <= '9') j++, nd++;
+    if (buf[j] == '.') {
+        j++;
+        while (buf[j] >= '0' && buf[j] <= '9') j++, nd++;
+    }
+    if (nd == 0) return 0.0;
+    if (buf[j] == 'e' || buf[j] == 'E') {
+        size_t k = j + 1, ne = 0;
+        if (buf[k] == '+' || buf[k] == '-') k++;
+        while (buf[k] >= '0' && buf[k] <= '9') k++, ne++;
+        if (ne == 0) return 0.0;
+        j = k;
+    }
+    if (buf[j] != 0) return 0.0;
+    return strtod(buf, NULL);
+}
+
+/* strconv.ParseInt(s, 0, 0) with the error dropped (obj.go:10-17): sign,
+ * 0x / 0o / 0b / leading-0 octal prefixes, underscores between digits */
+static int64_t parse_int0(const char *s, size_t n) {
+    size_t i = 0;
+    int neg = 0;
+    if (i < n && (s[i] == '+' || s[i] == '-')) neg = s[i++] == '-';
+    if (i >= n) return 0;
+    int base = 10;
+    int prefixed = 0;
+    if (s[i] == '0' && i + 1 < n) {
+        const char c = s[i + 1];
+        if (c == 'x' || c == 'X') base = 16, i += 2, prefixed = 1;
+        else if (c == 'o' || c == 'O') base = 8, i += 2, prefixed = 1;
+        else if (c == 'b' || c == 'B') base = 2, i += 2, prefixed = 1;
+        else base = 8, i += 1, prefixed = 1;
+    }
+    /* underscores: each must sit between two digits, or right after a prefix */
+    uint64_t v = 0;
+    int nd = 0, prev_digit = prefixed;
+    for (; i < n; i++) {
+        const char c = s[i];
+        if (c == '_') {
+            if (!prev_digit) return 0;
+            prev_digit = 0;
+            continue;
+        }
+        int d;
+        if (c >= '0' && c <= '9') d = c - '0';
+        else if (c >= 'a' && c <= 'f') d = c - 'a' + 10;
+        else if (c >= 'A' && c <= 'F') d = c - 'A' + 10;
+        else return 0;
+        if (d >= base) return 0;
+        if (v > (UINT64_MAX - (uint64_t)d) / (uint64_t)base) v = UINT64_MAX; /* out of range */
+        else v = v * (uint64_t)base + (uint64_t)d;
+        nd++;
+        prev_digit = 1;
+    }
+    if (!prev_digit) return 0;                /* trailing underscore or bare prefix */
+    if (nd == 0 && !(base == 8 && prefixed)) return 0;
+    if (!neg) return v > (uint64_t)INT64_MAX ? INT64_MAX : (int64_t)v;
+    return v > (uint64_t)INT64_MAX + 1 ? INT64_MIN : (int64_t)(0 - v);
+}
+
+int orc_load_stl(const uint8_t *data, int64_t size, int64_t *ntri, double *pos, double *nrm) {
+    if (size < 84) return -1; /* binary.Read of the header fails */
+    const uint32_t count = (uint32_t)data[80] | ((uint32_t)data[81] << 8) | ((uint32_t)data[82] << 16) |
+                           ((uint32_t)data[83] << 24);
+    if ((int64_t)count * 50 + 84 == size) { /* loadSTLB (stl.go:86-154) */
+        *ntri = count;
+        if (!pos) return 0;
+        for (int64_t i = 0; i < count; i++) {
+            const uint8_t *r = data + 84 + 50 * i;
+            Vec v[3];
+            for (int k = 0; k < 3; k++) v[k] = (Vec){le_f32(r + 12 + 12 * k), le_f32(r + 16 + 12 * k), le_f32(r + 20 + 12 * k)};
+            const Vec n = tri_normal(v[0], v[1], v[2]);
+            for (int k = 0; k < 3; k++) {
+                double *P = pos + 9 * i + 3 * k, *N = nrm + 9 * i + 3 * k;
+                P[0] = v[k].X, P[1] = v[k].Y, P[2] = v[k].Z;
+                N[0] = n.X, N[1] = n.Y, N[2] = n.Z;
+            }
+        }
+        return 0;
+    }
+    /* loadSTLA (stl.go:59-84): lines with exactly 4 fields, "vertex x y z" */
+    int64_t nv = 0;
+    const char *t = (const char *)data;
+    int64_t a = 0;
+    while (a < size) {
+        int64_t b = a;
+        while (b < size && t[b] != '\n') b++;
+        /* fields of line [a, b) */
+        const char *f[5];
+        size_t fl[5];
+        int nf = 0;
+        int64_t x = a;
+        while (x < b) {
+            while (x < b && is_space(t[x])) x++;
+            if (x >= b) break;
+            int64_t y = x;
+            while (y < b && !is_space(t[y])) y++;
+            if (nf < 5) f[nf] = t + x, fl[nf] = (size_t)(y - x);
+            nf++;
+            x = y;
+        }
+        if (nf == 4 && fl[0] == 6 && !memcmp(f[0], "vertex", 6)) {
+            if (pos)
+                for (int k = 0; k < 3; k++) pos[3 * nv + k] = parse_float(f[1 + k], fl[1 + k]);
+            nv++;
+        }
+        a = b + 1;
+    }
+    if (nv % 3) return -1; /* the triangle loop indexes past the end */
+    *ntri = nv / 3;
+    if (!pos) return 0;
+    for (int64_t i = 0; i < nv / 3; i++) {
+        double *P = pos + 9 * i;
+        const Vec n = tri_normal((Vec){P[0], P[1], P[2]}, (Vec){P[3], P[4], P[5]}, (Vec){P[6], P[7], P[8]});
+        for (int k = 0; k < 3; k++) nrm[9 * i + 3 * k] = n.X, nrm[9 * i + 3 * k + 1] = n.Y, nrm[9 * i + 3 * k + 2] = n.Z;
+    }
+    return 0;
+}
+
+typedef struct {
+    double *v;
+    int64_t n, cap;
+} VecList; /* n 3-vectors */
+
+static void vl_push(VecList *l, double x, double y, double z) {
+    if (l->n == l->cap) {
+        l->cap = l->cap ? 2 * l->cap : 1024;
+        l->v = realloc(l->v, (size_t)l->cap * 3 * sizeof(double));
+    }
+    l->v[3 * l->n] = x, l->v[3 * l->n + 1] = y, l->v[3 * l->n + 2] = z;
+    l->n++;
+}
+
+int orc_load_obj(const char *t, int64_t size, int64_t *ntri, double *pos, double *nrm, double *uv) {
+    VecList vs = {0}, vts = {0}, vns = {0};
+    vl_push(&vs, 0, 0, 0); /* 1-based indexing (obj.go:25-27) */
+    vl_push(&vts, 0, 0, 0);
+    vl_push(&vns, 0, 0, 0);
+    int64_t nt = 0;
+    int rc = 0;
+    int64_t a = 0;
+    while (a < size && rc == 0) {
+        int64_t b = a;
+        while (b < size && t[b] != '\n') b++;
+        /* fields: counted, then collected */
+        int nf = 0;
+        for (int64_t x = a; x < b;) {
+            while (x < b && is_space(t[x])) x++;
+            if (x >= b) break;
+            while (x < b && !is_space(t[x])) x++;
+            nf++;
+        }
+        const char **f = malloc((size_t)(nf + 1) * sizeof *f);
+        size_t *fl = malloc((size_t)(nf + 1) * sizeof *fl);
+        int64_t *iv = malloc((size_t)(nf + 1) * 3 * sizeof *iv), *it = iv + nf + 1, *in = it + nf + 1;
+        nf = 0;
+        for (int64_t x = a; x < b;) {
+            while (x < b && is_space(t[x])) x++;
+            if (x >= b) break;
+            int64_t y = x;
+            while (y < b && !is_space(t[y])) y++;
+            f[nf] = t + x, fl[nf] = (size_t)(y - x);
+            nf++;
+            x = y;
+        }
+        a = b + 1;
+        if (nf == 0) {
+            free(f), free(fl), free(iv);
+            continue;
+        }
+        const size_t kl = fl[0];
+        if (kl == 1 && f[0][0] == 'v') {
+            if (nf < 4) rc = -1;
+            else
+            vl_push(&vs, parse_float(f[1], fl[1]), parse_float(f[2], fl[2]), parse_float(f[3], fl[3]));
+        } else if (kl == 2 && f[0][0] == 'v' && f[0][1] == 't') {
+            if (nf < 3) rc = -1;
+            else
+            vl_push(&vts, parse_float(f[1], fl[1]), parse_float(f[2], fl[2]), 0.0);
+        } else if (kl == 2 && f[0][0] == 'v' && f[0][1] == 'n') {
+            if (nf < 4) rc = -1;
+            else
+            vl_push(&vns, parse_float(f[1], fl[1]), parse_float(f[2], fl[2]), parse_float(f[3], fl[3]));
+        } else if (kl == 1 && f[0][0] == 'f') {
+            const int na = nf - 1;
+            for (int k = 0; k < na; k++) {
+                /* strings.Split(arg + "//", "/"): the first three pieces */
+                const char *s = f[1 + k];
+                const size_t n = fl[1 + k];
+                const char *piece[3] = {s, "", ""};
+                size_t pl[3] = {n, 0, 0};
+                size_t p0 = 0;
+                while (p0 < n && s[p0] != '/') p0++;
+                pl[0] = p0;
+                if (p0 < n) {
+                    size_t p1 = p0 + 1;
+                    while (p1 < n && s[p1] != '/') p1++;
+                    piece[1] = s + p0 + 1, pl[1] = p1 - p0 - 1;
+                    if (p1 < n) {
+                        size_t p2 = p1 + 1;
+                        while (p2 < n && s[p2] != '/') p2++;
+                        piece[2] = s + p1 + 1, pl[2] = p2 - p1 - 1;
+                    }
+                }
+                const int64_t L[3] = {vs.n, vts.n, vns.n};
+                int64_t *dst[3] = {&iv[k], &it[k], &in[k]};
+                for (int q = 0; q < 3; q++) {
+                    int64_t v = parse_int0(piece[q], pl[q]);
+                    if (v < 0) v += L[q];
+                    *dst[q] = v;
+                }
+            }
+            for (int i = 1; i < na - 1; i++) {
+                const int c[3] = {0, i, i + 1};
+                for (int q = 0; q < 3; q++)
+                    if (iv[c[q]] < 0 || iv[c[q]] >= vs.n || it[c[q]] < 0 || it[c[q]] >= vts.n || in[c[q]] < 0 ||
+                        in[c[q]] >= vns.n)
+                        rc = -1;
+                if (rc) break;
+                if (pos) {
+                    Vec P[3], Nn[3];
+                    for (int q = 0; q < 3; q++) {
+                        const double *pv = vs.v + 3 * iv[c[q]], *nv = vns.v + 3 * in[c[q]], *tv = vts.v + 3 * it[c[q]];
+                        P[q] = (Vec){pv[0], pv[1], pv[2]};
+                        Nn[q] = (Vec){nv[0], nv[1], nv[2]};
+                        uv[6 * nt + 2 * q] = tv[0];
+                        uv[6 * nt + 2 * q + 1] = tv[1];
+                    }
+                    const Vec fn = tri_normal(P[0], P[1], P[2]); /* FixNormals (triangle.go:46-58) */
+                    for (int q = 0; q < 3; q++) {
+                        if (Nn[q].X == 0 && Nn[q].Y == 0 && Nn[q].Z == 0) Nn[q] = fn;
+                        double *po = pos + 9 * nt + 3 * q, *no = nrm + 9 * nt + 3 * q;
+                        po[0] = P[q].X, po[1] = P[q].Y, po[2] = P[q].Z;
+                        no[0] = Nn[q].X, no[1] = Nn[q].Y, no[2] = Nn[q].Z;
+                    }
+                }
+                nt++;
+            }
+        }
+        free(f), free(fl), free(iv);
+    }
+    free(vs.v);
+    free(vts.v);
+    free(vns.v);
+    if (rc) return rc;
+    *ntri = nt;
+    return 0;
 }

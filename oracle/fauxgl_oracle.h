@@ -176,6 +176,19 @@ void orc_mesh_fit_inside(double *tpos, double *tnrm, int64_t ntri, double *lpos,
 void orc_mesh_smooth_normals(const double *tpos, double *tnrm, int64_t ntri);
 void orc_mesh_smooth_normals_threshold(const double *tpos, double *tnrm, int64_t ntri, double threshold);
 
+/* Mesh loaders.  LoadSTL (stl.go:23-154): binary when the file size is
+ * 84 + 50 * (uint32 at byte 80), else ASCII "vertex x y z" lines; face
+ * normals (Triangle.Normal, triangle.go:33-37).  LoadOBJ (obj.go:19-79):
+ * v / vt / vn / f with fan triangulation, 1-based and negative indices
+ * (strconv.ParseInt base 0), FixNormals.  Positions / normals are ntri*9
+ * doubles, uv ntri*6 (Texture.X, Y).  Call with NULL outputs to get ntri.
+ * Returns 0, or -1 where the reference returns an error or panics (short
+ * header, a vertex count not a multiple of 3, an index out of range, too
+ * few numbers on a v / vt / vn line).  Floats follow strconv.ParseFloat
+ * (decimal, inf / infinity / nan; a malformed token reads as 0). */
+int orc_load_stl(const uint8_t *data, int64_t size, int64_t *ntri, double *pos, double *nrm);
+int orc_load_obj(const char *text, int64_t size, int64_t *ntri, double *pos, double *nrm, double *uv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -108,6 +108,8 @@ def lib():
         "orc_mesh_fit_inside": (None, [dp, dp, ctypes.c_int64, dp, dp, ctypes.c_int64, dp, dp, dp]),
         "orc_mesh_smooth_normals": (None, [dp, dp, ctypes.c_int64]),
         "orc_mesh_smooth_normals_threshold": (None, [dp, dp, ctypes.c_int64, _d]),
+        "orc_load_stl": (ctypes.c_int, [_P(ctypes.c_uint8), ctypes.c_int64, _P(ctypes.c_int64), dp, dp]),
+        "orc_load_obj": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64, _P(ctypes.c_int64), dp, dp, dp]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -451,3 +453,30 @@ def mesh_smooth_normals(tpos, tnrm, threshold=None):
     else:
         lib().orc_mesh_smooth_normals_threshold(_p(tp), _p(tn), tp.shape[0], float(threshold))
     return tn
+
+
+def load_stl(data):
+    """LoadSTL from the file's bytes -> (pos (n,3,3), nrm (n,3,3)); raises on
+    the inputs the reference rejects or panics on."""
+    buf = np.frombuffer(bytes(data), dtype=np.uint8)
+    n = ctypes.c_int64(0)
+    bp = buf.ctypes.data_as(_P(ctypes.c_uint8))
+    if lib().orc_load_stl(bp, buf.size, ctypes.byref(n), None, None):
+        raise ValueError("LoadSTL fails on this input")
+    pos = np.zeros((n.value, 3, 3))
+    nrm = np.zeros((n.value, 3, 3))
+    lib().orc_load_stl(bp, buf.size, ctypes.byref(n), _p(pos), _p(nrm))
+    return pos, nrm
+
+
+def load_obj(text):
+    """LoadOBJ from the file's text -> (pos, nrm (n,3,3), uv (n,3,2))."""
+    raw = text.encode() if isinstance(text, str) else bytes(text)
+    n = ctypes.c_int64(0)
+    if lib().orc_load_obj(raw, len(raw), ctypes.byref(n), None, None, None):
+        raise ValueError("LoadOBJ fails on this input")
+    pos = np.zeros((n.value, 3, 3))
+    nrm = np.zeros((n.value, 3, 3))
+    uv = np.zeros((n.value, 3, 2))
+    lib().orc_load_obj(raw, len(raw), ctypes.byref(n), _p(pos), _p(nrm), _p(uv))
+    return pos, nrm, uv
