@@ -34,6 +34,7 @@ EXPORTED = [
     "fgl_nn_infer", "fgl_nn_residual", "fgl_nn_grad", "fgl_nn_train_step",
     "fgl_mesh_counts", "fgl_mesh_read", "fgl_mesh_bounding_box", "fgl_mesh_transform", "fgl_mesh_fit_inside",
     "fgl_mesh_smooth_normals", "fgl_mesh_smooth_normals_threshold", "fgl_mesh_load_stl", "fgl_mesh_load_obj",
+    "fgl_encode_png_bound", "fgl_encode_png",
 ]
 
 _d = ctypes.c_double
@@ -125,6 +126,8 @@ def load():
         "fgl_mesh_smooth_normals_threshold": (i32, [vp, _d]),
         "fgl_mesh_load_stl": (i32, [vp, ctypes.POINTER(ctypes.c_uint8), ctypes.c_int64, vpp]),
         "fgl_mesh_load_obj": (i32, [vp, ctypes.c_char_p, ctypes.c_int64, vpp]),
+        "fgl_encode_png_bound": (i32, [vp, ctypes.POINTER(ctypes.c_int64)]),
+        "fgl_encode_png": (i32, [vp, ctypes.POINTER(ctypes.c_uint8), ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)

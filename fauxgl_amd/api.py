@@ -693,6 +693,22 @@ class Context:
                 self._ptr())
         return out
 
+    def EncodePNG(self):
+        """The color buffer as PNG file bytes (SavePNG's encoding, util.go:56-63)."""
+        L = N.load()
+        bound = ctypes.c_int64()
+        N.check(L.fgl_encode_png_bound(self._ptr(), ctypes.byref(bound)), self._ptr())
+        buf = (ctypes.c_uint8 * bound.value)()
+        n = ctypes.c_int64()
+        N.check(L.fgl_encode_png(self._ptr(), buf, bound.value, ctypes.byref(n)), self._ptr())
+        return bytes(buf[:n.value])
+
+    def SavePNG(self, path):
+        """SavePNG(path, Image()) (util.go:56-63)."""
+        data = self.EncodePNG()
+        with open(path, "wb") as f:
+            f.write(data)
+
     def Resolve(self, dst, filter="tent"):
         """Downsamples this context's color into dst (an SSAA resolve)."""
         f = {"tent": N.FGL_RESOLVE_TENT, "box": N.FGL_RESOLVE_BOX}[filter]

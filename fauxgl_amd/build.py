@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfauxgl_amd.so")
 
-SOURCES = ["fgl_vs.hip", "fgl_bin.hip", "fgl_spans.hip", "fgl_tile.hip", "fgl_misc.hip", "fgl_nn.hip", "fgl_mesh.hip", "fgl_ingest.hip", "fgl_host.cpp"]
+SOURCES = ["fgl_vs.hip", "fgl_bin.hip", "fgl_spans.hip", "fgl_tile.hip", "fgl_misc.hip", "fgl_nn.hip", "fgl_mesh.hip", "fgl_ingest.hip", "fgl_png.hip", "fgl_host.cpp"]
 
 # -ffp-contract=off: Go evaluates a*b+c as two roundings; the FP64 coverage
 # and depth arithmetic must match it bit for bit.  Packed FP32 ops are

@@ -1854,3 +1854,146 @@ int fgl_mesh_load_obj(fgl_ctx *ctx, const char *text, int64_t size, fgl_mesh **o
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// SavePNG (util.go:56-63): the color buffer encoded on the device (fgl_png.hip)
+
+namespace {
+
+constexpr int32_t kPngChunk = 4096;                              // filtered bytes per deflate piece
+constexpr int32_t kPngCapWords = (9 * kPngChunk + 64) / 32 + 2;  // worst case: all 9-bit literals
+
+uint32_t crc_table_entry(uint32_t n) {
+    uint32_t c = n;
+    for (int k = 0; k < 8; k++) c = (c & 1) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
+    return c;
+}
+uint32_t crc32_update(uint32_t crc, const uint8_t *p, size_t n) {
+    static uint32_t table[256];
+    static bool ready = false;
+    if (!ready) {
+        for (uint32_t i = 0; i < 256; i++) table[i] = crc_table_entry(i);
+        ready = true;
+    }
+    uint32_t c = crc ^ 0xFFFFFFFFu;
+    for (size_t i = 0; i < n; i++) c = table[(c ^ p[i]) & 255u] ^ (c >> 8);
+    return c ^ 0xFFFFFFFFu;
+}
+void put_be32(uint8_t *p, uint32_t v) {
+    p[0] = (uint8_t)(v >> 24);
+    p[1] = (uint8_t)(v >> 16);
+    p[2] = (uint8_t)(v >> 8);
+    p[3] = (uint8_t)v;
+}
+
+int64_t png_bound(const fgl_ctx *c) {
+    const int64_t n = (int64_t)c->H * ((int64_t)c->W * 4 + 1);
+    const int64_t pieces = (n + kPngChunk - 1) / kPngChunk;
+    return 8 + 25 + 8 + 2 + pieces * kPngCapWords * 4 + 2 + 4 + 4 + 12;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fgl_encode_png_bound(fgl_ctx *ctx, int64_t *bound) {
+    if (check_ctx(ctx) || !bound) return FGL_E_INVALID;
+    *bound = png_bound(ctx);
+    return FGL_OK;
+}
+
+int fgl_encode_png(fgl_ctx *ctx, uint8_t *out, int64_t cap, int64_t *len) {
+    if (check_ctx(ctx) || !out || !len) return FGL_E_INVALID;
+    if (cap < png_bound(ctx)) {
+        set_err(ctx->err, "fgl_encode_png: the buffer needs fgl_encode_png_bound = %lld bytes",
+                (long long)png_bound(ctx));
+        return FGL_E_INVALID;
+    }
+    fgl_ctx *c = ctx;
+    HIPCHK(c, hipSetDevice(c->device));
+    int r = materialize(c);
+    if (r) return r;
+    const int32_t W = c->W, H = c->H;
+    DevBuf flag, stream, pieces, sizes, adl, offs, packed;
+    HIPCHK(c, flag.ensure(4));
+    HIPCHK(c, hipMemsetAsync(flag.p, 0, 4, c->stream));
+    if ((r = launch(c, "png_opaque", [&] {
+             return launch_png_opaque(c->color.as<uint32_t>(), (int64_t)W * H, flag.as<uint32_t>(), c->stream);
+         })))
+        return r;
+    uint32_t translucent = 0;
+    HIPCHK(c, hipMemcpyAsync(&translucent, flag.p, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int32_t bpp = translucent ? 4 : 3;  // image/png: RGBA unless NRGBA.Opaque()
+    const int64_t n = (int64_t)H * ((int64_t)W * bpp + 1);
+    const int64_t np = (n + kPngChunk - 1) / kPngChunk;
+    HIPCHK(c, stream.ensure((size_t)n));
+    HIPCHK(c, pieces.ensure((size_t)np * kPngCapWords * 4));
+    HIPCHK(c, sizes.ensure((size_t)np * 4));
+    HIPCHK(c, adl.ensure((size_t)np * 8));
+    HIPCHK(c, offs.ensure((size_t)np * 8));
+    if ((r = launch(c, "png_filter", [&] {
+             return launch_png_filter(c->color.as<uint32_t>(), W, H, bpp, stream.as<uint8_t>(), c->stream);
+         })))
+        return r;
+    if ((r = launch(c, "png_deflate", [&] {
+             return launch_png_deflate(stream.as<uint8_t>(), n, kPngChunk, kPngCapWords, np, pieces.as<uint32_t>(),
+                                       sizes.as<uint32_t>(), adl.as<uint32_t>(), c->stream);
+         })))
+        return r;
+    std::vector<uint32_t> hs((size_t)np), ha((size_t)np * 2);
+    HIPCHK(c, hipMemcpyAsync(hs.data(), sizes.p, (size_t)np * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(ha.data(), adl.p, (size_t)np * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<uint64_t> ho((size_t)np);
+    uint64_t total = 0;
+    uint32_t a = 1, b = 0;  // Adler-32 of the whole stream from the pieces'
+    for (int64_t k = 0; k < np; k++) {
+        ho[k] = total;
+        total += hs[k];
+        const uint64_t plen = (uint64_t)std::min<int64_t>(kPngChunk, n - k * kPngChunk);
+        b = (uint32_t)((b + (plen % 65521u) * a + ha[2 * k + 1]) % 65521u);
+        a = (uint32_t)((a + ha[2 * k]) % 65521u);
+    }
+    HIPCHK(c, packed.ensure((size_t)total));
+    HIPCHK(c, hipMemcpyAsync(offs.p, ho.data(), (size_t)np * 8, hipMemcpyHostToDevice, c->stream));
+    if ((r = launch(c, "png_pack", [&] {
+             return launch_png_pack(pieces.as<uint32_t>(), sizes.as<uint32_t>(), offs.as<uint64_t>(), kPngCapWords, np,
+                                    packed.as<uint8_t>(), c->stream);
+         })))
+        return r;
+    // signature, IHDR, then the IDAT chunk around the zlib stream, IEND
+    uint8_t *p = out;
+    static const uint8_t sig[8] = {0x89, 'P', 'N', 'G', 0x0D, 0x0A, 0x1A, 0x0A};
+    memcpy(p, sig, 8);
+    p += 8;
+    put_be32(p, 13);
+    memcpy(p + 4, "IHDR", 4);
+    put_be32(p + 8, (uint32_t)W);
+    put_be32(p + 12, (uint32_t)H);
+    p[16] = 8;                         // bit depth
+    p[17] = (uint8_t)(bpp == 4 ? 6 : 2);  // color type: truecolor with / without alpha
+    p[18] = p[19] = p[20] = 0;         // deflate, adaptive filters, no interlace
+    put_be32(p + 21, crc32_update(0, p + 4, 17));
+    p += 25;
+    const uint64_t zlen = 2 + total + 2 + 4;
+    put_be32(p, (uint32_t)zlen);
+    memcpy(p + 4, "IDAT", 4);
+    uint8_t *z = p + 8;
+    z[0] = 0x78;  // deflate, 32K window
+    z[1] = 0x9C;  // default level, header check
+    HIPCHK(c, hipMemcpyAsync(z + 2, packed.p, (size_t)total, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    z[2 + total] = 0x03;  // final block: fixed codes, end of block only
+    z[3 + total] = 0x00;
+    put_be32(z + 4 + total, (b << 16) | a);
+    put_be32(z + zlen, crc32_update(0, p + 4, (size_t)(4 + zlen)));
+    p = z + zlen + 4;
+    static const uint8_t iend[12] = {0, 0, 0, 0, 'I', 'E', 'N', 'D', 0xAE, 0x42, 0x60, 0x82};
+    memcpy(p, iend, 12);
+    p += 12;
+    *len = p - out;
+    return FGL_OK;
+}
+
+}  // extern "C"

@@ -142,6 +142,14 @@ double ingest_parse_float(const char *s, size_t n);
 int64_t ingest_parse_int(const char *s, size_t n);
 void ingest_fields(const char *a, const char *b, std::vector<std::pair<const char *, size_t>> &out);
 
+// PNG encode (fgl_png.hip)
+hipError_t launch_png_opaque(const uint32_t *color, int64_t n, uint32_t *flag, hipStream_t s);
+hipError_t launch_png_filter(const uint32_t *color, int32_t W, int32_t H, int32_t bpp, uint8_t *out, hipStream_t s);
+hipError_t launch_png_deflate(const uint8_t *stream, int64_t n, int32_t chunk, int32_t cap_words, int64_t npieces,
+                              uint32_t *out, uint32_t *sizes, uint32_t *adl, hipStream_t s);
+hipError_t launch_png_pack(const uint32_t *pieces, const uint32_t *sizes, const uint64_t *offs, int32_t cap_words,
+                           int64_t npieces, uint8_t *dst, hipStream_t s);
+
 // mesh preparation (fgl_mesh.hip)
 hipError_t launch_mesh_first_nonzero(const double *pos, int64_t nprim, int32_t nv, int64_t *block_first,
                                      int32_t nblocks, hipStream_t s);

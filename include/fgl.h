@@ -203,6 +203,17 @@ int fgl_mesh_load_stl(fgl_ctx *ctx, const uint8_t *data, int64_t size, fgl_mesh 
  * line with too few numbers). */
 int fgl_mesh_load_obj(fgl_ctx *ctx, const char *text, int64_t size, fgl_mesh **out);
 
+/* ---- encode ---- */
+
+/* SavePNG (util.go:56-63) to memory: the color buffer as the PNG Go's
+ * image/png writes for an *image.NRGBA (RGB when every alpha is 255, else
+ * RGBA; 8-bit; each row's filter chosen as image/png chooses it), the zlib
+ * stream deflated on the device (its bytes are this build's own; it
+ * inflates to the same filtered rows).  out must hold
+ * fgl_encode_png_bound bytes; *len receives the file size. */
+int fgl_encode_png_bound(fgl_ctx *ctx, int64_t *bound);
+int fgl_encode_png(fgl_ctx *ctx, uint8_t *out, int64_t cap, int64_t *len);
+
 /* ---- readback and host writes (the public ColorBuffer/DepthBuffer fields) ---- */
 
 /* Image() (context.go:83-85): copies the 4*W*H NRGBA bytes out. */

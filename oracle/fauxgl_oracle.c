@@ -1718,3 +1718,56 @@ int orc_load_obj(const char *t, int64_t size, int64_t *ntri, double *pos, double
     *ntri = nt;
     return 0;
 }
+
+/* ========================================================================
+ * PNG row filters (Go image/png writer: filter, paeth, abs8)
+ * ======================================================================== */
+
+static int png_abs8(uint8_t d) { return d < 128 ? d : 256 - d; }
+static uint8_t png_paeth(uint8_t a, uint8_t b, uint8_t c) {
+    int pc = c, pa = (int)b - pc, pb = (int)a - pc;
+    pc = abs(pa + pb);
+    pa = abs(pa);
+    pb = abs(pb);
+    if (pa <= pb && pa <= pc) return a;
+    if (pb <= pc) return b;
+    return c;
+}
+
+int32_t orc_png_filter(const uint8_t *nrgba, int32_t w, int32_t h, uint8_t *out) {
+    int opaque = 1;
+    for (int64_t i = 0; i < (int64_t)w * h; i++) opaque &= nrgba[4 * i + 3] == 255;
+    const int bpp = opaque ? 3 : 4;
+    if (!out) return bpp;
+    const int n = w * bpp;
+    uint8_t *cr[5], *prev = calloc((size_t)n, 1), *cur = malloc((size_t)n);
+    for (int f = 0; f < 5; f++) cr[f] = malloc((size_t)n);
+    for (int y = 0; y < h; y++) {
+        for (int x = 0; x < w; x++)
+            for (int k = 0; k < bpp; k++) cur[x * bpp + k] = nrgba[4 * ((int64_t)y * w + x) + k];
+        /* filtered candidates: 0 None, 1 Sub, 2 Up, 3 Average, 4 Paeth */
+        for (int i = 0; i < n; i++) {
+            const uint8_t a = i >= bpp ? cur[i - bpp] : 0, b = prev[i], c = i >= bpp ? prev[i - bpp] : 0;
+            cr[0][i] = cur[i];
+            cr[1][i] = (uint8_t)(cur[i] - a);
+            cr[2][i] = (uint8_t)(cur[i] - b);
+            cr[3][i] = (uint8_t)(cur[i] - (uint8_t)(((int)a + (int)b) / 2));
+            cr[4][i] = (uint8_t)(cur[i] - png_paeth(a, b, c));
+        }
+        static const int order[5] = {2, 4, 0, 1, 3};
+        int best = -1, pick = 2;
+        for (int k = 0; k < 5; k++) {
+            int sum = 0;
+            for (int i = 0; i < n; i++) sum += png_abs8(cr[order[k]][i]);
+            if (best < 0 || sum < best) best = sum, pick = order[k];
+        }
+        uint8_t *o = out + (int64_t)y * (n + 1);
+        o[0] = (uint8_t)pick;
+        memcpy(o + 1, cr[pick], (size_t)n);
+        memcpy(prev, cur, (size_t)n);
+    }
+    for (int f = 0; f < 5; f++) free(cr[f]);
+    free(prev);
+    free(cur);
+    return bpp;
+}

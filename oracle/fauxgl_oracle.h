@@ -189,6 +189,16 @@ void orc_mesh_smooth_normals_threshold(const double *tpos, double *tnrm, int64_t
 int orc_load_stl(const uint8_t *data, int64_t size, int64_t *ntri, double *pos, double *nrm);
 int orc_load_obj(const char *text, int64_t size, int64_t *ntri, double *pos, double *nrm, double *uv);
 
+/* SavePNG (util.go:56-63) goes through Go's image/png encoder (standard
+ * library, not in the reference tree): for *image.NRGBA it writes RGB when
+ * every alpha is 255, else RGBA, and filters each row with the filter that
+ * minimises the sum of |byte| read as int8, trying Up, Paeth, None, Sub,
+ * Average in that order and keeping the first minimum.  This writes the
+ * filtered stream (per row: filter type byte, then the row) for nrgba
+ * (w*h*4 bytes) into out (h * (w*bpp + 1) bytes) and returns bpp (3 or 4);
+ * out NULL: only bpp. */
+int32_t orc_png_filter(const uint8_t *nrgba, int32_t w, int32_t h, uint8_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -110,6 +110,7 @@ def lib():
         "orc_mesh_smooth_normals_threshold": (None, [dp, dp, ctypes.c_int64, _d]),
         "orc_load_stl": (ctypes.c_int, [_P(ctypes.c_uint8), ctypes.c_int64, _P(ctypes.c_int64), dp, dp]),
         "orc_load_obj": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int64, _P(ctypes.c_int64), dp, dp, dp]),
+        "orc_png_filter": (ctypes.c_int32, [_P(ctypes.c_uint8), ctypes.c_int32, ctypes.c_int32, _P(ctypes.c_uint8)]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -480,3 +481,15 @@ def load_obj(text):
     uv = np.zeros((n.value, 3, 2))
     lib().orc_load_obj(raw, len(raw), ctypes.byref(n), _p(pos), _p(nrm), _p(uv))
     return pos, nrm, uv
+
+
+def png_filter(nrgba):
+    """(bpp, filtered stream) of Go image/png's row filtering for an (h, w, 4)
+    NRGBA image."""
+    img = np.ascontiguousarray(nrgba, dtype=np.uint8)
+    h, w = img.shape[:2]
+    ip = img.ctypes.data_as(_P(ctypes.c_uint8))
+    bpp = lib().orc_png_filter(ip, w, h, None)
+    out = np.zeros(h * (w * bpp + 1), dtype=np.uint8)
+    lib().orc_png_filter(ip, w, h, out.ctypes.data_as(_P(ctypes.c_uint8)))
+    return bpp, out
