@@ -133,6 +133,72 @@ def nn_frames(mesh, W, H, steps, warmup):
     return ms, img, table
 
 
+def batch_views(steps, warmup):
+    """Config 4: 32 noise spheres (1k..1M triangles, log-uniform), 8 views
+    each at {256^2, 512^2, 1024^2, 1920x1080}; meshes prepared on the device
+    (BiUnitCube, SmoothNormalsThreshold 30 degrees).  Times one step (all 256
+    views) drawn view by view, and replayed as one recorded HIP graph."""
+    import math as _m
+    from fauxgl_amd import api as A
+    from fauxgl_amd import meshes as M
+    t = time.time()
+    meshes, views = M.batch_workload()
+    log(f"[bench] batch workload: {len(meshes)} meshes, {len(views)} views in {time.time() - t:.1f}s")
+    home = A.NewContext(8, 8)
+    dms = []
+    for p in meshes:
+        dm = home.Upload(A.Mesh(p))
+        dm.BiUnitCube()
+        dm.SmoothNormalsThreshold(_m.radians(30.0))
+        dms.append(dm)
+    ctxs = []
+    for m, eye, w, h in views:
+        c = A.NewContext(w, h)
+        c.Shader = M.batch_shader(w, h, eye)
+        ctxs.append(c)
+
+    def eager():
+        for (m, eye, w, h), c in zip(views, ctxs):
+            c.ClearColorBufferWith(M.BACKGROUND)
+            c.ClearDepthBuffer()
+            c.DrawMesh(dms[m])
+        for c in ctxs:
+            c.Sync()
+
+    for _ in range(warmup):
+        eager()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eager()
+    ms_eager = (time.perf_counter() - t0) * 1000.0 / steps
+    b = A.Batch()
+    for (m, eye, w, h), c in zip(views, ctxs):
+        b.Add(c, dms[m], clear_color=M.BACKGROUND)
+    t0 = time.perf_counter()
+    b.Run()  # eager pass + recording
+    ms_record = (time.perf_counter() - t0) * 1000.0
+    for _ in range(warmup):
+        b.Run()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        infos = b.Run()
+    ms_graph = (time.perf_counter() - t0) * 1000.0 / steps
+    ntri = sum(len(meshes[m].position) for m, _, _, _ in views)
+    px = sum(w * h for _, _, w, h in views)
+    b.close()
+    for dm in dms:
+        dm.close()
+    return {
+        "views": len(views), "meshes": len(meshes), "triangles_per_step": ntri, "pixels_per_step": px,
+        "ms_per_step_view_by_view": round(ms_eager, 3), "ms_per_step_graph": round(ms_graph, 3),
+        "views_per_s_graph": round(len(views) * 1000.0 / ms_graph, 1),
+        "views_per_s_view_by_view": round(len(views) * 1000.0 / ms_eager, 1),
+        "triangles_per_s_graph": round(ntri * 1000.0 / ms_graph, 1),
+        "ms_first_run_and_record": round(ms_record, 1),
+        "total_pixels_per_step": int(sum(i.TotalPixels for i in infos)),
+    }
+
+
 def psnr(a, b):
     d = a[..., :3].astype(np.float64) - b[..., :3].astype(np.float64)
     mse = float(np.mean(d * d))
@@ -238,6 +304,7 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=5, help="timed CPU frames per resolution (0 = skip)")
     ap.add_argument("--cpu-frames-1t", type=int, default=3, help="timed single-thread CPU frames at 1x (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = CPUs in this process's affinity mask")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-view line")
     ap.add_argument("--only", choices=["16x", "1x"], default=None,
                     help="time one resolution only and print its kernel table (for profiler runs)")
     args = ap.parse_args()
@@ -281,6 +348,12 @@ def main():
         "weights": "fauxgl_amd/nnaa_weights.npy (trained by fauxgl_amd/train_nnaa.py on engine renders; "
                    "this camera is not a training view)",
     }
+
+    batched = None
+    if not args.no_batch:
+        batched = batch_views(max(3, args.steps // 4), 1)
+        log(f"[bench] batched views: {batched['ms_per_step_graph']} ms per 256 views as a graph, "
+            f"{batched['ms_per_step_view_by_view']} ms view by view")
 
     # dominant kernel of the headline step and its roofline
     per_kernel = {k: {"launches": v[0], "avg_ms": v[1] / max(v[0], 1), "ms_per_frame": v[1] / PROFILE_FRAMES}
@@ -347,6 +420,7 @@ def main():
         "roofline_frame": roofline_frame,
         "cpu_baseline": cpu,
         "neural_aa": neural_aa,
+        "batched_views": batched,
     }
     if cpu is not None:
         line["speedup_vs_cpu_16x"] = round(fps16 / cpu["value"], 1)

@@ -35,6 +35,7 @@ EXPORTED = [
     "fgl_mesh_counts", "fgl_mesh_read", "fgl_mesh_bounding_box", "fgl_mesh_transform", "fgl_mesh_fit_inside",
     "fgl_mesh_smooth_normals", "fgl_mesh_smooth_normals_threshold", "fgl_mesh_load_stl", "fgl_mesh_load_obj",
     "fgl_encode_png_bound", "fgl_encode_png",
+    "fgl_batch_create", "fgl_batch_destroy", "fgl_batch_error", "fgl_batch_add", "fgl_batch_run",
 ]
 
 _d = ctypes.c_double
@@ -126,6 +127,12 @@ def load():
         "fgl_mesh_smooth_normals_threshold": (i32, [vp, _d]),
         "fgl_mesh_load_stl": (i32, [vp, ctypes.POINTER(ctypes.c_uint8), ctypes.c_int64, vpp]),
         "fgl_mesh_load_obj": (i32, [vp, ctypes.c_char_p, ctypes.c_int64, vpp]),
+        "fgl_batch_create": (i32, [vpp]),
+        "fgl_batch_destroy": (None, [vp]),
+        "fgl_batch_error": (ctypes.c_char_p, [vp]),
+        "fgl_batch_add": (i32, [vp, vp, ctypes.POINTER(ctypes.c_uint8), _d, ctypes.POINTER(FglState),
+                                ctypes.POINTER(FglShader), vp]),
+        "fgl_batch_run": (i32, [vp, u64p]),
         "fgl_encode_png_bound": (i32, [vp, ctypes.POINTER(ctypes.c_int64)]),
         "fgl_encode_png": (i32, [vp, ctypes.POINTER(ctypes.c_uint8), ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
     }

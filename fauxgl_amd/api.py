@@ -534,6 +534,53 @@ class DeviceMesh:
         return Mesh(tris, lines)
 
 
+class Batch:
+    """Many views rendered per step (fgl_batch): each view is a context, its
+    clear color, its current state and shader, and a resident mesh.  The
+    first Run draws them one by one and records them into one HIP graph;
+    later Runs replay it.  Run returns [RasterizeInfo] per view."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        N.check(N.load().fgl_batch_create(ctypes.byref(h)))
+        self.h = h
+        self.n = 0
+        self._keep = []
+
+    def _check(self, rc):
+        if rc != N.FGL_OK:
+            msg = N.load().fgl_batch_error(self.h)
+            raise N.FglError(rc, msg.decode() if msg else "")
+
+    def Add(self, ctx, mesh, clear_color=None, clear_depth=MaxFloat64):
+        """A view: ClearColorBufferWith(clear_color or ctx.ClearColor),
+        ClearDepthBufferWith(clear_depth), DrawMesh(mesh) with ctx's
+        current state and shader."""
+        c = (ctypes.c_uint8 * 4)(*Color(*(clear_color if clear_color is not None else ctx.ClearColor)).NRGBA())
+        st = ctx._state()
+        sh = shader_struct(ctx.Shader, ctx)
+        self._check(N.load().fgl_batch_add(self.h, ctx._ptr(), c, float(clear_depth), ctypes.byref(st),
+                                           ctypes.byref(sh), mesh._ptr()))
+        self._keep.append((ctx, mesh))
+        self.n += 1
+
+    def Run(self):
+        info = (ctypes.c_uint64 * (2 * max(self.n, 1)))()
+        self._check(N.load().fgl_batch_run(self.h, info))
+        return [RasterizeInfo(int(info[2 * i]), int(info[2 * i + 1])) for i in range(self.n)]
+
+    def close(self):
+        if getattr(self, "h", None):
+            N.load().fgl_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def _file_bytes(src):
     if isinstance(src, (bytes, bytearray, memoryview)):
         return bytes(src)

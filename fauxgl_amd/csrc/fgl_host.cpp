@@ -2,6 +2,7 @@
 // deferred clears, the per-draw kernel sequence and its capacity retries.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdarg>
@@ -315,7 +316,12 @@ constexpr int64_t kMaxPrims = (int64_t)1 << 26;
 // attribute offsets within 32 bits.
 constexpr uint64_t kMaxSideVtx = (uint64_t)1 << 28;
 
-int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, const DevPrims &in, uint64_t info[2]) {
+// capture_ctr: when not null, the launches are being recorded into a graph:
+// they are enqueued once with the context's current capacities (buffers
+// already sized by an earlier draw of the same inputs), the draw counters
+// are copied to capture_ctr by a kernel, and nothing is read back.
+int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, const DevPrims &in, uint64_t info[2],
+             uint32_t *capture_ctr = nullptr) {
     if (info) info[0] = info[1] = 0;
     if (in.n < 0 || in.n > kMaxPrims) {
         set_err(c->err, "primitive count %lld outside [0, %lld]", (long long)in.n, (long long)kMaxPrims);
@@ -474,6 +480,11 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
         if ((r = launch(c, "tile_frags", [&] { return launch_tile_frags(a, c->stream); }))) return r;
         if ((r = launch(c, "rop", [&] { return launch_tile(a, c->stream); }))) return r;
         if ((r = launch(c, "updated", [&] { return launch_updated(a, c->stream); }))) return r;
+        if (capture_ctr)
+            return launch(c, "counters", [&] {
+                return launch_copy_words(reinterpret_cast<const uint32_t *>(a.ctr), capture_ctr,
+                                         (int32_t)(sizeof(DrawCounters) / 4), c->stream);
+            });
 
         DrawCounters h;
         HIPCHK(c, hipMemcpyAsync(&h, a.ctr, sizeof h, hipMemcpyDeviceToHost, c->stream));
@@ -1993,6 +2004,256 @@ int fgl_encode_png(fgl_ctx *ctx, uint8_t *out, int64_t cap, int64_t *len) {
     memcpy(p, iend, 12);
     p += 12;
     *len = p - out;
+    return FGL_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Batched views: many (context, mesh, state, shader) frames rendered by one
+// HIP graph.  The first run draws every view eagerly, which sizes each
+// context's buffers (the capacity tuning: every capacity is the need of
+// that view's frame plus the usual margin), then records all views into one
+// graph: a fork from the batch stream to every context's stream, each
+// view's clear + triangle + line pipelines on its context's stream (views
+// of different contexts are independent branches; views of one context run
+// in order), a kernel copying each pipeline's counters into one array, a
+// join back.  Later runs replay the graph and read the counters once; a
+// view whose frame outgrew its buffers is redrawn eagerly and the graph is
+// recorded again.
+
+struct fgl_batch {
+    struct View {
+        fgl_ctx *ctx;
+        const fgl_mesh *mesh;
+        fgl_state st;
+        fgl_shader sh;
+        uint32_t clear_color;
+        double clear_depth;
+    };
+    std::vector<View> views;
+    int device = -1;
+    hipStream_t s0 = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> joins;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    DevBuf ctrs;  // 2 DrawCounters per view (triangles, lines)
+    std::vector<DrawCounters> host;
+    std::string err;
+    ~fgl_batch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (hipEvent_t e : joins) (void)hipEventDestroy(e);
+        if (fork) (void)hipEventDestroy(fork);
+        if (s0) (void)hipStreamDestroy(s0);
+    }
+};
+
+namespace {
+
+void mesh_prims(const fgl_mesh *m, bool lines, DevPrims &p) {
+    p.n = lines ? m->nline : m->ntri;
+    const DevBuf *b[4] = {lines ? &m->lpos : &m->tpos, lines ? &m->lnrm : &m->tnrm, lines ? &m->luv : &m->tuv,
+                          lines ? &m->lcol : &m->tcol};
+    const bool *has = lines ? m->l_has : m->t_has;
+    p.pos = has[0] ? b[0]->as<double>() : nullptr;
+    p.nrm = has[1] ? b[1]->as<double>() : nullptr;
+    p.uv = has[2] ? b[2]->as<double>() : nullptr;
+    p.col = has[3] ? b[3]->as<double>() : nullptr;
+}
+
+#define BCHK(b, call)                                                                               \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess) {                                                                     \
+            set_err((b)->err, "%s failed: %s", #call, hipGetErrorString(e_));                       \
+            return FGL_E_DEVICE;                                                                    \
+        }                                                                                           \
+    } while (0)
+
+int batch_eager(fgl_batch *b, size_t i, uint64_t info[2]) {
+    fgl_batch::View &v = b->views[i];
+    v.ctx->clear_color = v.clear_color;
+    v.ctx->pend_color = true;
+    v.ctx->clear_depth = v.clear_depth;
+    v.ctx->pend_depth = true;
+    DevPrims t, l;
+    mesh_prims(v.mesh, false, t);
+    mesh_prims(v.mesh, true, l);
+    uint64_t i1[2] = {0, 0}, i2[2] = {0, 0};
+    int r = run_draw(v.ctx, &v.st, &v.sh, PRIM_TRI, t, i1);
+    if (!r) r = run_draw(v.ctx, &v.st, &v.sh, PRIM_LINE, l, i2);
+    if (r) {
+        set_err(b->err, "view %zu: %s", i, v.ctx->err.c_str());
+        return r;
+    }
+    info[0] = i1[0] + i2[0];
+    info[1] = i1[1] + i2[1];
+    return FGL_OK;
+}
+
+int batch_capture(fgl_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    std::vector<fgl_ctx *> ctxs;
+    for (auto &v : b->views)
+        if (std::find(ctxs.begin(), ctxs.end(), v.ctx) == ctxs.end()) ctxs.push_back(v.ctx);
+    while (b->joins.size() < ctxs.size()) {
+        hipEvent_t e;
+        BCHK(b, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        b->joins.push_back(e);
+    }
+    std::vector<bool> prof(ctxs.size());
+    for (size_t k = 0; k < ctxs.size(); k++) {
+        prof[k] = ctxs[k]->prof;
+        ctxs[k]->prof = false;  // no timing events inside the graph
+    }
+    BCHK(b, hipStreamBeginCapture(b->s0, hipStreamCaptureModeGlobal));
+    int r = FGL_OK;
+    if (hipEventRecord(b->fork, b->s0) != hipSuccess) r = FGL_E_DEVICE;
+    for (size_t k = 0; k < ctxs.size() && !r; k++)
+        if (hipStreamWaitEvent(ctxs[k]->stream, b->fork, 0) != hipSuccess) r = FGL_E_DEVICE;
+    uint32_t *dst = b->ctrs.as<uint32_t>();
+    const int32_t words = (int32_t)(sizeof(DrawCounters) / 4);
+    for (size_t i = 0; i < b->views.size() && !r; i++) {
+        fgl_batch::View &v = b->views[i];
+        v.ctx->clear_color = v.clear_color;
+        v.ctx->pend_color = true;
+        v.ctx->clear_depth = v.clear_depth;
+        v.ctx->pend_depth = true;
+        DevPrims t, l;
+        mesh_prims(v.mesh, false, t);
+        mesh_prims(v.mesh, true, l);
+        r = run_draw(v.ctx, &v.st, &v.sh, PRIM_TRI, t, nullptr, dst + (2 * i) * words);
+        if (t.n > 0) v.ctx->pend_color = v.ctx->pend_depth = false;  // the triangle pass clears
+        if (!r) r = run_draw(v.ctx, &v.st, &v.sh, PRIM_LINE, l, nullptr, dst + (2 * i + 1) * words);
+        if (l.n > 0) v.ctx->pend_color = v.ctx->pend_depth = false;
+        if (r) set_err(b->err, "view %zu: %s", i, v.ctx->err.c_str());
+    }
+    for (size_t k = 0; k < ctxs.size() && !r; k++) {
+        if (hipEventRecord(b->joins[k], ctxs[k]->stream) != hipSuccess ||
+            hipStreamWaitEvent(b->s0, b->joins[k], 0) != hipSuccess)
+            r = FGL_E_DEVICE;
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(b->s0, &g);
+    for (size_t k = 0; k < ctxs.size(); k++) ctxs[k]->prof = prof[k];
+    if (r) {
+        if (g) (void)hipGraphDestroy(g);
+        if (b->err.empty()) b->err = "recording the batch failed";
+        return r;
+    }
+    if (ee != hipSuccess) {
+        set_err(b->err, "hipStreamEndCapture failed: %s", hipGetErrorString(ee));
+        return FGL_E_DEVICE;
+    }
+    b->graph = g;
+    BCHK(b, hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    return FGL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fgl_batch_create(fgl_batch **out) {
+    if (!out) return FGL_E_INVALID;
+    *out = new (std::nothrow) fgl_batch;
+    return *out ? FGL_OK : FGL_E_NOMEM;
+}
+
+void fgl_batch_destroy(fgl_batch *b) {
+    if (!b) return;
+    if (b->s0) (void)hipStreamSynchronize(b->s0);
+    delete b;
+}
+
+const char *fgl_batch_error(const fgl_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+int fgl_batch_add(fgl_batch *b, fgl_ctx *ctx, const uint8_t clear_nrgba[4], double clear_depth, const fgl_state *st,
+                  const fgl_shader *sh, const fgl_mesh *mesh) {
+    if (!b || check_ctx(ctx) || !clear_nrgba || !st || !sh || !mesh) return FGL_E_INVALID;
+    if (b->device >= 0 && ctx->device != b->device) {
+        b->err = "fgl_batch_add: every view's context must be on one device";
+        return FGL_E_INVALID;
+    }
+    if (mesh->ctx->device != ctx->device) {
+        b->err = "fgl_batch_add: mesh lives on another device";
+        return FGL_E_INVALID;
+    }
+    b->device = ctx->device;
+    fgl_batch::View v;
+    v.ctx = ctx;
+    v.mesh = mesh;
+    v.st = *st;
+    v.sh = *sh;
+    v.clear_color = (uint32_t)clear_nrgba[0] | ((uint32_t)clear_nrgba[1] << 8) | ((uint32_t)clear_nrgba[2] << 16) |
+                    ((uint32_t)clear_nrgba[3] << 24);
+    v.clear_depth = clear_depth;
+    b->views.push_back(v);
+    if (b->exec) {  // the recorded graph no longer covers every view
+        (void)hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+    }
+    return FGL_OK;
+}
+
+int fgl_batch_run(fgl_batch *b, uint64_t *info) {
+    if (!b) return FGL_E_INVALID;
+    const size_t n = b->views.size();
+    if (n == 0) return FGL_OK;
+    BCHK(b, hipSetDevice(b->device));
+    if (!b->s0) {
+        BCHK(b, hipStreamCreateWithFlags(&b->s0, hipStreamNonBlocking));
+        BCHK(b, hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    }
+    std::vector<uint64_t> tmp(2 * n);
+    uint64_t *out = info ? info : tmp.data();
+    if (!b->exec) {
+        // eager pass: every view drawn once, every context's buffers sized
+        for (size_t i = 0; i < n; i++) {
+            int r = batch_eager(b, i, out + 2 * i);
+            if (r) return r;
+        }
+        for (auto &v : b->views) BCHK(b, hipStreamSynchronize(v.ctx->stream));
+        if (b->ctrs.ensure(2 * n * sizeof(DrawCounters)) != hipSuccess) {
+            b->err = "fgl_batch_run: device allocation failed";
+            return FGL_E_NOMEM;
+        }
+        int r = batch_capture(b);
+        if (r) return r;
+        for (auto &v : b->views) {  // the eager pass left every frame drawn
+            v.ctx->pend_color = v.ctx->pend_depth = false;
+        }
+        return FGL_OK;
+    }
+    BCHK(b, hipGraphLaunch(b->exec, b->s0));
+    b->host.resize(2 * n);
+    BCHK(b, hipMemcpyAsync(b->host.data(), b->ctrs.p, 2 * n * sizeof(DrawCounters), hipMemcpyDeviceToHost, b->s0));
+    BCHK(b, hipStreamSynchronize(b->s0));
+    bool regrow = false;
+    for (size_t i = 0; i < n; i++) {
+        const DrawCounters &t = b->host[2 * i], &l = b->host[2 * i + 1];
+        const bool has_t = b->views[i].mesh->ntri > 0, has_l = b->views[i].mesh->nline > 0;
+        if ((has_t && t.overflow) || (has_l && l.overflow)) {
+            // this frame no longer fits the recorded capacities: draw it
+            // eagerly (growing its buffers) and record the graph again
+            int r = batch_eager(b, i, out + 2 * i);
+            if (r) return r;
+            regrow = true;
+            continue;
+        }
+        out[2 * i] = (has_t ? t.total : 0) + (has_l ? l.total : 0);
+        out[2 * i + 1] = (has_t ? t.updated : 0) + (has_l ? l.updated : 0);
+        b->views[i].ctx->last = has_l ? l : t;
+    }
+    if (regrow) {
+        for (auto &v : b->views) BCHK(b, hipStreamSynchronize(v.ctx->stream));
+        return batch_capture(b);
+    }
     return FGL_OK;
 }
 

@@ -110,6 +110,7 @@ hipError_t launch_tile_frags(const DrawArgs &a, hipStream_t s);
 hipError_t launch_tile(const DrawArgs &a, hipStream_t s);
 hipError_t launch_updated(const DrawArgs &a, hipStream_t s);
 
+hipError_t launch_copy_words(const uint32_t *src, uint32_t *dst, int32_t n, hipStream_t s);
 hipError_t launch_fill_color(uint32_t *color, int64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_fill_depth(double *depth, int64_t n, double v, hipStream_t s);
 hipError_t launch_resolve(const uint32_t *src, int32_t sw, int32_t sh, uint32_t *tmp, uint32_t *dst, int32_t dw,

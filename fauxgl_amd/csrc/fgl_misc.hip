@@ -21,6 +21,17 @@ static unsigned grid_for(int64_t n) {
     return (unsigned)b;
 }
 
+// word copy in a kernel (graph-capturable, unlike a memcpy node between
+// arbitrary device buffers on every runtime): the batch gathers each
+// view's draw counters with it
+__global__ void k_copy_words(const uint32_t *src, uint32_t *dst, int32_t n) {
+    for (int32_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+hipError_t launch_copy_words(const uint32_t *src, uint32_t *dst, int32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy_words, dim3(1), dim3(64), 0, s, src, dst, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_color(uint32_t *color, int64_t n, uint32_t v, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_color, dim3(grid_for(n)), dim3(256), 0, s, color, n, v);
     return hipGetLastError();

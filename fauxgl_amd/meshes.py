@@ -68,6 +68,18 @@ def new_icosahedron():
     return v[idx]
 
 
+_SPHERE_CACHE = {}
+
+
+def sphere_positions_cached(detail):
+    """sphere_positions(detail), computed once per detail (read-only)."""
+    if detail not in _SPHERE_CACHE:
+        a = sphere_positions(detail)
+        a.setflags(write=False)
+        _SPHERE_CACHE[detail] = a
+    return _SPHERE_CACHE[detail]
+
+
 def sphere_positions(detail):
     """NewSphere(detail) (shapes.go:107-133) triangle positions in the
     reference's depth-first order: each level replaces a triangle by
@@ -277,7 +289,7 @@ def noise_sphere_prims(ntri=871306, detail=8, amplitude=0.15, seed=1):
     order, each vertex radius displaced by 1 + amplitude * fbm6(p, seed)
     (a function of position, so shared vertices stay shared), face normals:
     the noise sphere before its mesh preparation."""
-    pos = sphere_positions(detail)[:ntri]
+    pos = sphere_positions_cached(detail)[:ntri]
     flat = pos.reshape(-1, 3) + 0.0
     uniq, inv = np.unique(flat, axis=0, return_inverse=True)
     disp = 1.0 + amplitude * fbm(uniq, 6, seed)
@@ -326,3 +338,38 @@ def soup_scene(width=1920, height=1080, n=200000, seed=2, tex_size=1024):
     tex = ImageTexture(noise_texture(tex_size, seed), "NRGBA")
     shader = TextureShader(camera(width, height), tex)
     return mesh, shader
+
+
+# ---------------------------------------------------------------------------
+# batched multi-view workload
+
+BATCH_RESOLUTIONS = [(256, 256), (512, 512), (1024, 1024), (1920, 1080)]
+
+
+def batch_workload(nmesh=32, views_per_mesh=8, seed=4, nmin=1e3, nmax=1e6, radius=3.5,
+                   resolutions=BATCH_RESOLUTIONS):
+    """Config-2-style noise spheres with triangle counts log-uniform in
+    [nmin, nmax] (raw: face normals, before BiUnitCube and smoothing), and
+    views_per_mesh views of each: eye on a sphere of the given radius at a
+    random azimuth and elevation, looking at the origin, resolution drawn
+    from resolutions.  Returns ([Prims], [(mesh index, eye, W, H)])."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    meshes, views = [], []
+    for m in range(nmesh):
+        n = int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
+        d = 0
+        while 20 * 4 ** d < n:
+            d += 1
+        meshes.append(noise_sphere_prims(ntri=n, detail=d, amplitude=0.15, seed=100 + m))
+        for _ in range(views_per_mesh):
+            az = rng.uniform(0, 2 * math.pi)
+            el = rng.uniform(-1.2, 1.2)
+            eye = V(radius * math.cos(el) * math.cos(az), radius * math.sin(el), radius * math.cos(el) * math.sin(az))
+            w, h = resolutions[int(rng.integers(len(resolutions)))]
+            views.append((m, eye, w, h))
+    return meshes, views
+
+
+def batch_shader(w, h, eye):
+    """Phong shader of a batch view: camera at eye looking at the origin."""
+    return phong_shader(w, h, eye=eye, center=V(0, 0, 0))

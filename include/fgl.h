@@ -163,6 +163,26 @@ void fgl_mesh_destroy(fgl_mesh *mesh);
 int fgl_draw_mesh(fgl_ctx *ctx, const fgl_state *state, const fgl_shader *shader, const fgl_mesh *mesh,
                   uint64_t info[2]);
 
+/* ---- batched views (no reference counterpart: callers that render many
+ * frames of resident meshes per step) ----
+ *
+ * A batch is a list of views: (context, clear color and depth, state,
+ * shader, resident mesh) = ClearColorBufferWith + ClearDepthBufferWith +
+ * DrawMesh on that context.  The first fgl_batch_run draws every view one
+ * by one, which sizes each context's buffers, and records the whole batch
+ * into one HIP graph (views on different contexts run as independent
+ * branches); later runs replay the graph.  A view whose frame no longer fits
+ * is redrawn directly and the graph recorded again.  info: 2 per view
+ * (TotalPixels, UpdatedPixels), may be NULL.  A view's context must not be
+ * used by other calls while a run is in flight. */
+typedef struct fgl_batch fgl_batch;
+int fgl_batch_create(fgl_batch **out);
+void fgl_batch_destroy(fgl_batch *batch);
+const char *fgl_batch_error(const fgl_batch *batch);
+int fgl_batch_add(fgl_batch *batch, fgl_ctx *ctx, const uint8_t clear_nrgba[4], double clear_depth,
+                  const fgl_state *state, const fgl_shader *shader, const fgl_mesh *mesh);
+int fgl_batch_run(fgl_batch *batch, uint64_t *info);
+
 /* ---- device-side mesh preparation on a resident mesh (in place) ---- */
 
 /* Primitive counts of a resident mesh. */
