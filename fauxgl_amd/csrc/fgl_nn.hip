@@ -326,7 +326,8 @@ __global__ __launch_bounds__(256) void k_nn_conv(const uint16_t *in, const uint1
 // pixel halo, zero outside the image: the padding layer B sees) stay in
 // LDS, so that the intermediate activation never goes to HBM.  One
 // persistent workgroup of 8 waves per CU; both layers' weights sit in LDS
-// (rows padded to F2_WS bytes) and are read into registers per phase.
+// (rows padded to F2_WS bytes) and reach registers from there, once per
+// kernel where the register budget allows, else once per phase.
 // Each wave computes a strip of rows of one 16-pixel column block, reading
 // every input row's three shifted fragments once for the up to three
 // output rows that use it.  The multiply-accumulate order per output is
@@ -423,6 +424,38 @@ __global__ __launch_bounds__(512) void k_nn_conv2(const uint16_t *in, const uint
     }
     __syncthreads();
 
+    // weight fragments: lane (pm, kg) holds w[tap][row pm of tile t][8 kg .. 8 kg + 7]
+    auto load_a = [&](bf16x8 (&A)[9][2]) {
+#pragma unroll
+        for (int tap = 0; tap < 9; tap++)
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+                const int co = 8 * (pm >> 2) + 4 * t + (pm & 3);
+                if (CIN_A == 32 || kg < 2)
+                    A[tap][t] = *reinterpret_cast<const bf16x8 *>(lwa + (tap * 32 + co) * WA_S + 16 * kg);
+                else
+                    A[tap][t] = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+            }
+    };
+    auto load_b = [&](bf16x8 (&B)[9][NTB]) {
+#pragma unroll
+        for (int tap = 0; tap < 9; tap++)
+#pragma unroll
+            for (int t = 0; t < NTB; t++) {
+                const int co = NTB == 2 ? 8 * (pm >> 2) + 4 * t + (pm & 3) : pm;
+                B[tap][t] = *reinterpret_cast<const bf16x8 *>(lwb + (tap * COUT_B + co) * F2_WS + 16 * kg);
+            }
+    };
+    // Both layers' fragments stay in registers across tiles where that fits
+    // the register budget (a 16-channel side makes one layer small);
+    // otherwise each phase reloads its layer's fragments from LDS.
+    constexpr bool HOIST = CIN_A == 16 || COUT_B == 16;
+    bf16x8 A[9][2], B[9][NTB];
+    if (HOIST) {
+        load_a(A);
+        load_b(B);
+    }
+
     for (;;) {
         const int x0 = (j % ntx) * F2_W, y0 = (by0 + j / ntx) * F2_H;
         const int jn = j + g;
@@ -431,17 +464,7 @@ __global__ __launch_bounds__(512) void k_nn_conv2(const uint16_t *in, const uint
 
         // ---- layer A: intermediate rows 5 * half .. + 4, columns 16 * blk .. + 15
         {
-            bf16x8 A[9][2];
-#pragma unroll
-            for (int tap = 0; tap < 9; tap++)
-#pragma unroll
-                for (int t = 0; t < 2; t++) {
-                    const int co = 8 * (pm >> 2) + 4 * t + (pm & 3);
-                    if (CIN_A == 32 || kg < 2)
-                        A[tap][t] = *reinterpret_cast<const bf16x8 *>(lwa + (tap * 32 + co) * WA_S + 16 * kg);
-                    else
-                        A[tap][t] = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
-                }
+            if (!HOIST) load_a(A);
             const int r0 = 5 * half;
             f32x4 acc[5][2];
 #pragma unroll
@@ -484,14 +507,7 @@ __global__ __launch_bounds__(512) void k_nn_conv2(const uint16_t *in, const uint
 
         // ---- layer B: output rows 4 * half .. + 3, columns 16 * blk .. + 15
         {
-            bf16x8 B[9][NTB];
-#pragma unroll
-            for (int tap = 0; tap < 9; tap++)
-#pragma unroll
-                for (int t = 0; t < NTB; t++) {
-                    const int co = NTB == 2 ? 8 * (pm >> 2) + 4 * t + (pm & 3) : pm;
-                    B[tap][t] = *reinterpret_cast<const bf16x8 *>(lwb + (tap * COUT_B + co) * F2_WS + 16 * kg);
-                }
+            if (!HOIST) load_b(B);
             const int ch0 = NTB == 2 ? 8 * kg : 4 * kg;
             const int r0 = 4 * half;
             f32x4 acc[4][NTB];
