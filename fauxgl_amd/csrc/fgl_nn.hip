@@ -51,31 +51,50 @@ __global__ __launch_bounds__(64) void k_nn_range(const double *partial, int32_t 
     }
 }
 
-FGL_D float depth_t(const double *depth, int64_t i, double lo, double hi) {
-    const double d = depth[i];
+FGL_D float depth_tv(double d, double lo, double hi) {
     if (d == kMaxF64 || !(hi > lo)) return 1.0f;
     return (float)((d - lo) / (hi - lo));
 }
 
-// the 8 feature channels of pixel (x, y) as bf16 pairs
-FGL_D uint4 pixel_features(const uint32_t *color, const double *depth, double lo, double hi, int32_t H, int32_t W,
-                           int32_t x, int32_t y) {
+// the depth and color samples a pixel's features read
+struct FeatIn {
+    uint32_t c;
+    double d, dl, dr, du, dd;  // centre, left, right, up, down (clamped at the border)
+};
+
+FGL_D FeatIn feat_load(const uint32_t *color, const double *depth, int32_t H, int32_t W, int32_t x, int32_t y) {
     const int64_t i = (int64_t)y * W + x;
-    const uint32_t c = color[i];
-    const double d = depth[i];
-    const float t = depth_t(depth, i, lo, hi);
     const int32_t xl = x > 0 ? x - 1 : x, xr = x + 1 < W ? x + 1 : x;
     const int32_t yu = y > 0 ? y - 1 : y, yd = y + 1 < H ? y + 1 : y;
-    const float gx = depth_t(depth, (int64_t)y * W + xr, lo, hi) - depth_t(depth, (int64_t)y * W + xl, lo, hi);
-    const float gy = depth_t(depth, (int64_t)yd * W + x, lo, hi) - depth_t(depth, (int64_t)yu * W + x, lo, hi);
+    FeatIn v;
+    v.c = color[i];
+    v.d = depth[i];
+    v.dl = depth[(int64_t)y * W + xl];
+    v.dr = depth[(int64_t)y * W + xr];
+    v.du = depth[(int64_t)yu * W + x];
+    v.dd = depth[(int64_t)yd * W + x];
+    return v;
+}
+
+// the 8 feature channels as bf16 pairs
+FGL_D uint4 feat_eval(const FeatIn &v, double lo, double hi) {
+    const float t = depth_tv(v.d, lo, hi);
+    const float gx = depth_tv(v.dr, lo, hi) - depth_tv(v.dl, lo, hi);
+    const float gy = depth_tv(v.dd, lo, hi) - depth_tv(v.du, lo, hi);
     const float sx = -64.0f * gx, sy = -64.0f * gy;
     const float rn = 1.0f / sqrtf(sx * sx + sy * sy + 1.0f);
+    const uint32_t c = v.c;
     uint4 f;
     f.x = f2bf((float)(c & 0xff) * (1.0f / 255.0f)) | ((uint32_t)f2bf((float)((c >> 8) & 0xff) * (1.0f / 255.0f)) << 16);
     f.y = f2bf((float)((c >> 16) & 0xff) * (1.0f / 255.0f)) | ((uint32_t)f2bf(t) << 16);
     f.z = f2bf(sx * rn) | ((uint32_t)f2bf(sy * rn) << 16);
-    f.w = f2bf(rn) | ((uint32_t)f2bf(d == kMaxF64 ? 0.0f : 1.0f) << 16);
+    f.w = f2bf(rn) | ((uint32_t)f2bf(v.d == kMaxF64 ? 0.0f : 1.0f) << 16);
     return f;
+}
+
+FGL_D uint4 pixel_features(const uint32_t *color, const double *depth, double lo, double hi, int32_t H, int32_t W,
+                           int32_t x, int32_t y) {
+    return feat_eval(feat_load(color, depth, H, W, x, y), lo, hi);
 }
 
 __global__ __launch_bounds__(256) void k_nn_features(const uint32_t *color, const double *depth, const double *range,
@@ -402,21 +421,41 @@ __global__ __launch_bounds__(512) void k_nn_conv2(const uint16_t *in, const uint
             if (idx < ITEMS) *reinterpret_cast<uint4 *>(lin + (idx / CH) * F2_S + (idx % CH) * 16) = v[it];
         }
     };
-    auto put_features = [&](int jj) {
+    // layer 0 of inference: each tile pixel's depth and color samples are
+    // loaded before the current tile is computed and turned into features
+    // after it (k_nn_features' values, bit for bit)
+    constexpr int FITER = (F2_IW * F2_IH + 511) / 512;
+    const double rlo = FEAT ? range[0] : 0.0, rhi = FEAT ? range[1] : 0.0;
+    auto fetch_features = [&](int jj, FeatIn *raw) {
         const int x0 = (jj % ntx) * F2_W, y0 = (by0 + jj / ntx) * F2_H;
-        const double lo = range[0], hi = range[1];
-        for (int p = tid; p < F2_IW * F2_IH; p += 512) {
+#pragma unroll
+        for (int it = 0; it < FITER; it++) {
+            const int p0 = tid + it * 512, p = p0 < F2_IW * F2_IH ? p0 : F2_IW * F2_IH - 1;
+            const int ry = p / F2_IW, rx = p - ry * F2_IW;
+            const int gy = y0 - 2 + ry, gx = x0 - 2 + rx;
+            const int cy = gy < 0 ? 0 : (gy >= H ? H - 1 : gy), cx = gx < 0 ? 0 : (gx >= W ? W - 1 : gx);
+            raw[it] = feat_load(color, depth, H, W, cx, cy);
+        }
+    };
+    auto put_features = [&](int jj, const FeatIn *raw) {
+        const int x0 = (jj % ntx) * F2_W, y0 = (by0 + jj / ntx) * F2_H;
+#pragma unroll
+        for (int it = 0; it < FITER; it++) {
+            const int p = tid + it * 512;
+            if (p >= F2_IW * F2_IH) continue;
             const int ry = p / F2_IW, rx = p - ry * F2_IW;
             const int gy = y0 - 2 + ry, gx = x0 - 2 + rx;
             uint4 f = make_uint4(0, 0, 0, 0);
-            if (gy >= 0 && gy < H && gx >= 0 && gx < W) f = pixel_features(color, depth, lo, hi, H, W, gx, gy);
+            if (gy >= 0 && gy < H && gx >= 0 && gx < W) f = feat_eval(raw[it], rlo, rhi);
             *reinterpret_cast<uint4 *>(lin + p * F2_S) = f;
         }
     };
 
     __syncthreads();  // weights staged before the first tile goes in
     if (FEAT) {
-        put_features(j);
+        FeatIn raw0[FITER];
+        fetch_features(j, raw0);
+        put_features(j, raw0);
     } else {
         uint4 v0[ITER];
         fetch(j, v0);
@@ -460,7 +499,11 @@ __global__ __launch_bounds__(512) void k_nn_conv2(const uint16_t *in, const uint
         const int x0 = (j % ntx) * F2_W, y0 = (by0 + j / ntx) * F2_H;
         const int jn = j + g;
         uint4 v[FEAT ? 1 : ITER];
-        if (!FEAT && jn < nband) fetch(jn, v);
+        FeatIn raw[FEAT ? FITER : 1];
+        if (jn < nband) {
+            if (FEAT) fetch_features(jn, raw);
+            else fetch(jn, v);
+        }
 
         // ---- layer A: intermediate rows 5 * half .. + 4, columns 16 * blk .. + 15
         {
@@ -563,7 +606,7 @@ __global__ __launch_bounds__(512) void k_nn_conv2(const uint16_t *in, const uint
         if (jn >= nband) break;
         // every wave is past layer A, so the input buffer is free
         if (FEAT) {
-            put_features(jn);
+            put_features(jn, raw);
         } else {
             put(v);
         }
