@@ -1066,6 +1066,8 @@ int fgl_ctx_sync(fgl_ctx *ctx) {
 // Learned anti-aliasing: parameters, forward, backward and Adam on the
 // context's device (fgl_nn.hip).
 
+constexpr int32_t kNnRangeBlocks = 2048;  // depth min / max partials (enough blocks to fill the GPU)
+
 struct fgl_nn {
     fgl_ctx *home = nullptr;  // device owner (parameters live with it)
     int32_t cin[NN_LAYERS] = {NN_FEAT_C, NN_C, NN_C, NN_C, NN_C, NN_C};
@@ -1112,7 +1114,7 @@ int nn_buffers(fgl_nn *nn, int32_t H, int32_t W) {
     for (int l = 0; l < NN_LAYERS - 1; l++) HIPCHK(c, nn->act[l].ensure(px * NN_C * 2));
     HIPCHK(c, nn->out.ensure(px * NN_OUT_C * 4));
     HIPCHK(c, nn->range.ensure(16));
-    HIPCHK(c, nn->dpart.ensure(512 * 16));
+    HIPCHK(c, nn->dpart.ensure(kNnRangeBlocks * 16));
     nn->H = H;
     nn->W = W;
     return FGL_OK;
@@ -1131,12 +1133,15 @@ int nn_forward(fgl_nn *nn, fgl_ctx *src, fgl_ctx *dst, bool keep_feat) {
     const int32_t H = src->H, W = src->W;
     const int64_t n = (int64_t)H * W;
     if ((r = launch(c, "nn_depth_minmax", [&] {
-             return launch_depth_minmax(src->depth.as<double>(), n, nn->dpart.as<double>(), 512, c->stream);
+             return launch_depth_minmax(src->depth.as<double>(), n, nn->dpart.as<double>(), kNnRangeBlocks,
+                                       c->stream);
          })))
         return r;
-    if ((r = launch(c, "nn_depth_range", [&] {
-             return launch_nn_range(nn->dpart.as<double>(), 512, nn->range.as<double>(), c->stream);
-         })))
+    // the fused first pair of inference reduces the partials itself
+    if (keep_feat && (r = launch(c, "nn_depth_range", [&] {
+                          return launch_nn_range(nn->dpart.as<double>(), kNnRangeBlocks, nn->range.as<double>(),
+                                                 c->stream);
+                      })))
         return r;
     if (keep_feat && (r = launch(c, "nn_features", [&] {
                           return launch_nn_features(src->color.as<uint32_t>(), src->depth.as<double>(),
@@ -1156,7 +1161,8 @@ int nn_forward(fgl_nn *nn, fgl_ctx *src, fgl_ctx *dst, bool keep_feat) {
                      return launch_nn_conv2(nn->cin[l], nn->cout[l + 1], epi, pin, nn->wf[l].as<uint16_t>(),
                                             nn->p.as<float>() + nn->boff[l], nn->wf[l + 1].as<uint16_t>(),
                                             nn->p.as<float>() + nn->boff[l + 1], H, W, o, src->color.as<uint32_t>(),
-                                            src->depth.as<double>(), nn->range.as<double>(), c->stream);
+                                            src->depth.as<double>(), nn->dpart.as<double>(), kNnRangeBlocks,
+                                            c->stream);
                  })))
                 return r;
             pin = nn->act[l + 1].as<uint16_t>();

@@ -166,10 +166,12 @@ hipError_t launch_mesh_smooth(bool threshold_mode, const double *pos, const doub
 
 // two fused layers (inference): cin_a -> 32 (ReLU) -> cout_b with epilogue
 // epi_b (0: ReLU -> bf16, 1: f32, 3: final color); in == nullptr: layer A
-// reads the features computed from color, depth and range
+// reads the features computed from color, depth and range ({lo, hi}, or
+// nrange > 0 per-block {lo, hi} partials)
 hipError_t launch_nn_conv2(int cin_a, int cout_b, int epi_b, const uint16_t *in, const uint16_t *wa, const float *ba,
                            const uint16_t *wb, const float *bb, int32_t H, int32_t W, void *out,
-                           const uint32_t *color, const double *depth, const double *range, hipStream_t s);
+                           const uint32_t *color, const double *depth, const double *range, int32_t nrange,
+                           hipStream_t s);
 int64_t nn_dw_waves(int32_t H, int32_t W);
 hipError_t launch_nn_dw(const uint16_t *x, int32_t Cin, const uint16_t *dy, int32_t Cout, int32_t H, int32_t W,
                         float *partial, float *bpartial, hipStream_t s);

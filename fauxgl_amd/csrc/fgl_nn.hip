@@ -362,7 +362,7 @@ template <int CIN_A, int COUT_B, int EPI_B, bool FEAT>
 __global__ __launch_bounds__(512) void k_nn_conv2(const uint16_t *in, const uint16_t *wa, const float *bias_a,
                                                   const uint16_t *wb, const float *bias_b, int32_t H, int32_t W,
                                                   void *out, const uint32_t *color, const double *depth,
-                                                  const double *range) {
+                                                  const double *range, int32_t nrange) {
     static_assert(CIN_A == 16 || CIN_A == 32, "CIN_A");
     static_assert((COUT_B == 32 && EPI_B == 0) || (COUT_B == 16 && (EPI_B == 1 || EPI_B == 3)), "layer B");
     static_assert(!FEAT || CIN_A == 16, "features are 16 channels");
@@ -425,7 +425,41 @@ __global__ __launch_bounds__(512) void k_nn_conv2(const uint16_t *in, const uint
     // loaded before the current tile is computed and turned into features
     // after it (k_nn_features' values, bit for bit)
     constexpr int FITER = (F2_IW * F2_IH + 511) / 512;
-    const double rlo = FEAT ? range[0] : 0.0, rhi = FEAT ? range[1] : 0.0;
+    // depth range: range = {lo, hi}, or nrange per-block {lo, hi} partials
+    // of k_depth_minmax reduced here (k_nn_range's result, exactly)
+    __shared__ double srange[2][8];
+    double rlo = 0.0, rhi = 0.0;
+    if (FEAT) {
+        if (nrange > 0) {
+            double lo = kMaxF64, hi = -kMaxF64;
+            for (int q = tid; q < nrange; q += 512) {
+                const double l = range[2 * q], h = range[2 * q + 1];
+                lo = l < lo ? l : lo;
+                hi = h > hi ? h : hi;
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                const double l = __shfl_xor(lo, off), h = __shfl_xor(hi, off);
+                lo = l < lo ? l : lo;
+                hi = h > hi ? h : hi;
+            }
+            if (lane == 0) {
+                srange[0][wave] = lo;
+                srange[1][wave] = hi;
+            }
+            __syncthreads();
+            lo = srange[0][0];
+            hi = srange[1][0];
+            for (int w = 1; w < 8; w++) {
+                lo = srange[0][w] < lo ? srange[0][w] : lo;
+                hi = srange[1][w] > hi ? srange[1][w] : hi;
+            }
+            rlo = lo;
+            rhi = hi;
+        } else {
+            rlo = range[0];
+            rhi = range[1];
+        }
+    }
     auto fetch_features = [&](int jj, FeatIn *raw) {
         const int x0 = (jj % ntx) * F2_W, y0 = (by0 + jj / ntx) * F2_H;
 #pragma unroll
@@ -793,7 +827,8 @@ hipError_t launch_nn_conv(int epi, const uint16_t *in, int32_t Cin, const uint16
 }
 hipError_t launch_nn_conv2(int cin_a, int cout_b, int epi_b, const uint16_t *in, const uint16_t *wa, const float *ba,
                            const uint16_t *wb, const float *bb, int32_t H, int32_t W, void *out,
-                           const uint32_t *color, const double *depth, const double *range, hipStream_t s) {
+                           const uint32_t *color, const double *depth, const double *range, int32_t nrange,
+                           hipStream_t s) {
     const int64_t ntiles = (int64_t)((W + F2_W - 1) / F2_W) * ((H + F2_H - 1) / F2_H);
     const int64_t per_xcd = (ntiles + 7) / 8;
     const unsigned grid = 8u * (unsigned)(per_xcd < F2_WG_PER_XCD ? per_xcd : F2_WG_PER_XCD);
@@ -801,7 +836,7 @@ hipError_t launch_nn_conv2(int cin_a, int cout_b, int epi_b, const uint16_t *in,
 #define FGL_NN_CONV2(CA, CB, E, F)                                                                              \
     if (cin_a == CA && cout_b == CB && epi_b == E && feat == F) {                                               \
         hipLaunchKernelGGL((k_nn_conv2<CA, CB, E, F>), dim3(grid), dim3(512), 0, s, in, wa, ba, wb, bb, H, W,    \
-                           out, color, depth, range);                                                           \
+                           out, color, depth, range, nrange);                                                   \
         return hipGetLastError();                                                                               \
     }
     FGL_NN_CONV2(16, 32, 0, true)
