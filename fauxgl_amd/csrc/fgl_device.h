@@ -83,6 +83,9 @@ struct DevShader {
     int32_t pw_frac, pw_big;
     int32_t pw_niter;  // bit length of pw_yi (squaring steps)
     int32_t pw_pad;
+    // For x in [pw_lo, pw_hi] every intermediate of the squaring loop, taken
+    // without the frexp scaling, stays within 2^-1000 .. 2^1000.
+    double pw_lo, pw_hi;
 };
 
 struct DevState {
@@ -150,6 +153,11 @@ FGL_D double go_max(double x, double y) {
     if (x == 0 && x == y) return __builtin_signbit(x) ? y : x;
     return x > y ? x : y;
 }
+// math.Max(x, 0) and math.Min(x, 1) as selects: with a constant second
+// operand the special cases reduce to one comparison (NaN passes through,
+// -0 gives +0 for Max, +-Inf fall on the right side of the comparison).
+FGL_D double go_max0(double x) { return !(x <= 0.0) ? x : 0.0; }
+FGL_D double go_min1(double x) { return !(x >= 1.0) ? x : 1.0; }
 // int(x) on amd64: truncation, NaN and out-of-range give INT64_MIN.
 FGL_D int64_t go_f2i(double x) {
     if (!(x >= -9223372036854775808.0 && x < 9223372036854775808.0)) return INT64_MIN;

@@ -291,6 +291,11 @@ int fill_shader(fgl_ctx *c, const fgl_shader *sh, DevShader &d) {
         d.pw_yi = (std::isfinite(yi) && yi < 9223372036854775808.0) ? (uint64_t)yi : 0;
         d.pw_niter = 0;
         for (uint64_t v = d.pw_yi; v; v >>= 1) d.pw_niter++;
+        // |log2 x| <= L keeps x^k within 2^+-1000 for every k <= yi + 1/2
+        // (the squares, the partial products, x^yf and the reciprocal)
+        const double L = std::min(1000.0 / ((double)d.pw_yi + 0.5), 1000.0);
+        d.pw_lo = std::exp2(-L);
+        d.pw_hi = std::exp2(L);
     }
     // ObjectColor != Discard compares all four channels with 0 (shader.go:76)
     d.obj_is_discard = (sh->object_color[0] == 0 && sh->object_color[1] == 0 && sh->object_color[2] == 0 &&

@@ -77,6 +77,19 @@ __device__ double go_pow(double x, const DevShader &sh) {
         return __builtin_inf();
     }
     double a1 = sh.pw_frac ? pow_frac(x, sh.pw_yf) : 1.0;
+    if (x >= sh.pw_lo && x <= sh.pw_hi) {
+        // Every value below stays normal (the host's bound on x), and there
+        // scaling by a power of two commutes with rounding: each product is
+        // the frexp form's product times its exact power of two, so the same
+        // double results without the exponent bookkeeping.
+        double x1 = x;
+        for (uint64_t i = sh.pw_yi; i != 0;) {
+            if (i & 1) a1 *= x1;
+            i >>= 1;
+            if (i != 0) x1 *= x1;
+        }
+        return y < 0 ? 1.0 / a1 : a1;
+    }
     int32_t ae = 0;
     int xe;
     double x1 = frexp(x, &xe);
@@ -180,11 +193,12 @@ __device__ __noinline__ C4 bilinear_sample(const DevShader &sh, double u, double
 // vertex attribute fetch for the shaders
 
 // Per-vertex attribute arrays.  A missing array (a mesh without normals,
-// texture coordinates or colors) is replaced by a run of zeros read with
-// stride 0, so every fetch is the same branch-free address computation.
+// texture coordinates or colors) is replaced by a run of zeros read at
+// offset 0 (its mask is 0), so every fetch is the same branch-free address
+// computation.
 struct AttrSrc {
     const double *pos, *nrm, *uv, *col;  // stride 3, 3, 2, 4 doubles
-    int32_t spos, snrm, suv, scol;       // strides (0 for a missing array)
+    uint32_t mpos, mnrm, muv, mcol;      // offset masks: ~0 present, 0 missing
     const double *side;                  // side vertices (stride SIDE_VTX_DOUBLES)
 };
 
@@ -194,43 +208,61 @@ FGL_D AttrSrc attr_src(const DrawArgs &a) {
     s.nrm = a.in.nrm ? a.in.nrm : a.zeros;
     s.uv = a.in.uv ? a.in.uv : a.zeros;
     s.col = a.in.col ? a.in.col : a.zeros;
-    s.spos = a.in.pos ? 3 : 0;
-    s.snrm = a.in.nrm ? 3 : 0;
-    s.suv = a.in.uv ? 2 : 0;
-    s.scol = a.in.col ? 4 : 0;
+    s.mpos = a.in.pos ? ~0u : 0u;
+    s.mnrm = a.in.nrm ? ~0u : 0u;
+    s.muv = a.in.uv ? ~0u : 0u;
+    s.mcol = a.in.col ? ~0u : 0u;
     s.side = a.side_vtx;
     return s;
 }
 
-// Address of vertex k's attribute: the input array (triangle 3i+sel, line
-// 2i+sel) or the side vertex (i+sel) at offset side_off.
+// The three vertices of a raster triangle: vertex ids into the input arrays
+// (triangle 3i+sel, line 2i+sel) or into the side vertices (i+sel).
 // Offsets fit 32 bits: at most 2^26 input primitives (3 * 2^26 vertices
 // times 4 doubles) and 2^28 side vertices (times SIDE_VTX_DOUBLES).
-FGL_D const double *attr_ptr(const AttrSrc &src, uint64_t attr, int k, const double *arr, int32_t stride,
-                             int side_off) {
+struct VtxRef {
+    uint32_t vid[3];
+    bool side;
+};
+
+FGL_D VtxRef vtx_ref(uint64_t attr) {
     const uint32_t kind = attr_kind(attr);
     const uint32_t idx = (uint32_t)attr_index(attr);
-    const uint32_t m = kind == ATTR_TRI ? 3u : kind == ATTR_LINE ? 2u : 1u;
-    const uint32_t vid = idx * m + attr_sel(attr, k);
-    return kind == ATTR_SIDE ? src.side + (vid * (uint32_t)SIDE_VTX_DOUBLES + (uint32_t)side_off)
-                             : arr + vid * (uint32_t)stride;
+    const uint32_t b = kind == ATTR_TRI ? idx + (idx << 1) : kind == ATTR_LINE ? idx << 1 : idx;
+    VtxRef r;
+#pragma unroll
+    for (int k = 0; k < 3; k++) r.vid[k] = b + attr_sel(attr, k);
+    r.side = kind == ATTR_SIDE;
+    return r;
 }
 
-FGL_D V3 fetch_pos(const AttrSrc &src, uint64_t attr, int k) {
-    const double *q = attr_ptr(src, attr, k, src.pos, src.spos, 0);
+// Address of vertex k's attribute: the input array (scale doubles per
+// vertex, offset mask m) or the side vertex at offset side_off; both the
+// base and the offset are selected, not branched on.
+FGL_D const double *attr_ptr(const AttrSrc &src, const VtxRef &r, int k, const double *arr, uint32_t scale,
+                             uint32_t m, uint32_t side_off) {
+    const uint32_t vid = r.vid[k];
+    const uint32_t in_off = (vid * scale) & m;
+    const uint32_t side_at = vid * (uint32_t)SIDE_VTX_DOUBLES + side_off;
+    const double *base = r.side ? src.side : arr;
+    return base + (r.side ? side_at : in_off);
+}
+
+FGL_D V3 fetch_pos(const AttrSrc &src, const VtxRef &r, int k) {
+    const double *q = attr_ptr(src, r, k, src.pos, 3u, src.mpos, 0u);
     return v3(q[0], q[1], q[2]);
 }
-FGL_D V3 fetch_nrm(const AttrSrc &src, uint64_t attr, int k) {
-    const double *q = attr_ptr(src, attr, k, src.nrm, src.snrm, 3);
+FGL_D V3 fetch_nrm(const AttrSrc &src, const VtxRef &r, int k) {
+    const double *q = attr_ptr(src, r, k, src.nrm, 3u, src.mnrm, 3u);
     return v3(q[0], q[1], q[2]);
 }
-FGL_D void fetch_uv(const AttrSrc &src, uint64_t attr, int k, double &u, double &v) {
-    const double *q = attr_ptr(src, attr, k, src.uv, src.suv, 6);
+FGL_D void fetch_uv(const AttrSrc &src, const VtxRef &r, int k, double &u, double &v) {
+    const double *q = attr_ptr(src, r, k, src.uv, 2u, src.muv, 6u);
     u = q[0];
     v = q[1];
 }
-FGL_D C4 fetch_col(const AttrSrc &src, uint64_t attr, int k) {
-    const double *q = attr_ptr(src, attr, k, src.col, src.scol, 8);
+FGL_D C4 fetch_col(const AttrSrc &src, const VtxRef &r, int k) {
+    const double *q = attr_ptr(src, r, k, src.col, 4u, src.mcol, 8u);
     return c4(q[0], q[1], q[2], q[3]);
 }
 
@@ -253,18 +285,20 @@ struct ShadeIn {
 // SH: 1 SolidColorShader, 2 TextureShader, 3 PhongShader, 4 PhongShader
 // with a Texture (its own instantiation keeps the sampler's registers out
 // of the untextured Phong kernel).
-template <int SH>
+// COLS: also the vertex colors Phong reads when ObjectColor is Discard.
+template <int SH, bool COLS>
 FGL_D void load_in(const DevShader &sh, const AttrSrc &src, uint64_t attr, ShadeIn &in) {
     const bool uvs = SH == 2 || SH == 4;
-    const bool cols = SH == 3 && sh.obj_is_discard;
+    const bool cols = COLS && SH == 3 && sh.obj_is_discard;
+    const VtxRef r = vtx_ref(attr);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         if (SH >= 3) {
-            in.p[k] = fetch_pos(src, attr, k);
-            in.n[k] = fetch_nrm(src, attr, k);
+            in.p[k] = fetch_pos(src, r, k);
+            in.n[k] = fetch_nrm(src, r, k);
         }
-        if (uvs) fetch_uv(src, attr, k, in.u[k], in.v[k]);
-        if (cols) in.c[k] = fetch_col(src, attr, k);
+        if (uvs) fetch_uv(src, r, k, in.u[k], in.v[k]);
+        if (cols) in.c[k] = fetch_col(src, r, k);
     }
 }
 
@@ -291,57 +325,30 @@ FGL_D C4 eval(const DevShader &sh, const ShadeIn &in, V4 b) {
                        interp(in.c[0].b, in.c[1].b, in.c[2].b, b), interp(in.c[0].a, in.c[1].a, in.c[2].a, b));
         }
         V3 L = v3(sh.light[0], sh.light[1], sh.light[2]);
-        double diffuse = go_max(vdot(N, L), 0);
+        double diffuse = go_max0(vdot(N, L));
         light = cadd_scaled(light, c4(sh.dif[0], sh.dif[1], sh.dif[2], sh.dif[3]), diffuse);
         if (diffuse > 0 && sh.power > 0) {
             V3 camera = vnormalize(vsub(v3(sh.cam[0], sh.cam[1], sh.cam[2]), P));
             V3 i = v3(-L.x, -L.y, -L.z);
             V3 reflected = vsub(i, vmuls(N, 2 * vdot(N, i)));
-            double specular = go_max(vdot(camera, reflected), 0);
+            double specular = go_max0(vdot(camera, reflected));
             if (specular > 0) {
                 specular = go_pow(specular, sh);
                 light = cadd_scaled(light, c4(sh.spc[0], sh.spc[1], sh.spc[2], sh.spc[3]), specular);
             }
         }
-        return c4(go_min(color.r * light.r, 1), go_min(color.g * light.g, 1), go_min(color.b * light.b, 1),
-                  color.a);
+        return c4(go_min1(color.r * light.r), go_min1(color.g * light.g), go_min1(color.b * light.b), color.a);
     }
 }
 
-// PhongShader.Fragment with the vertex attributes fetched where they are
-// first needed (positions, then normals, then colors), with scheduling
-// barriers between the steps so the three fetches are not hoisted together:
-// fewer values live at once leaves registers for more waves per SIMD.
-FGL_D C4 eval_phong_staged(const DevShader &sh, const AttrSrc &src, uint64_t attr, V4 b) {
-    V3 p0 = fetch_pos(src, attr, 0), p1 = fetch_pos(src, attr, 1), p2 = fetch_pos(src, attr, 2);
-    const V3 P = v3(interp(p0.x, p1.x, p2.x, b), interp(p0.y, p1.y, p2.y, b), interp(p0.z, p1.z, p2.z, b));
-    __builtin_amdgcn_sched_barrier(0);
-    V3 n0 = fetch_nrm(src, attr, 0), n1 = fetch_nrm(src, attr, 1), n2 = fetch_nrm(src, attr, 2);
-    const V3 N = vnormalize(v3(interp(n0.x, n1.x, n2.x, b), interp(n0.y, n1.y, n2.y, b), interp(n0.z, n1.z, n2.z, b)));
-    __builtin_amdgcn_sched_barrier(0);
-    C4 light = c4(sh.amb[0], sh.amb[1], sh.amb[2], sh.amb[3]);
-    C4 color;
-    if (!sh.obj_is_discard) {
-        color = c4(sh.obj[0], sh.obj[1], sh.obj[2], sh.obj[3]);
-    } else {
-        const C4 c0 = fetch_col(src, attr, 0), c1 = fetch_col(src, attr, 1), c2 = fetch_col(src, attr, 2);
-        color = c4(interp(c0.r, c1.r, c2.r, b), interp(c0.g, c1.g, c2.g, b), interp(c0.b, c1.b, c2.b, b),
-                   interp(c0.a, c1.a, c2.a, b));
-    }
-    const V3 L = v3(sh.light[0], sh.light[1], sh.light[2]);
-    const double diffuse = go_max(vdot(N, L), 0);
-    light = cadd_scaled(light, c4(sh.dif[0], sh.dif[1], sh.dif[2], sh.dif[3]), diffuse);
-    if (diffuse > 0 && sh.power > 0) {
-        const V3 camera = vnormalize(vsub(v3(sh.cam[0], sh.cam[1], sh.cam[2]), P));
-        const V3 i = v3(-L.x, -L.y, -L.z);
-        const V3 reflected = vsub(i, vmuls(N, 2 * vdot(N, i)));
-        double specular = go_max(vdot(camera, reflected), 0);
-        if (specular > 0) {
-            specular = go_pow(specular, sh);
-            light = cadd_scaled(light, c4(sh.spc[0], sh.spc[1], sh.spc[2], sh.spc[3]), specular);
-        }
-    }
-    return c4(go_min(color.r * light.r, 1), go_min(color.g * light.g, 1), go_min(color.b * light.b, 1), color.a);
+// The kernel's arguments read again from the kernarg segment.  The empty
+// asm makes the pointer opaque, so the loads through it stay at their uses
+// inside the shading loop (scalar loads each pass) instead of being hoisted
+// in front of it into scalar registers that the loop then spills.
+FGL_D const DrawArgs &args_reload() {
+    auto p = (const __attribute__((address_space(4))) DrawArgs *)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(p));
+    return *(const DrawArgs *)p;
 }
 
 FGL_D const PrimRec *prim_of(const DrawArgs &a, uint32_t ref) {
@@ -393,13 +400,11 @@ __global__ __launch_bounds__(FRAG_GROUP) void k_frag_scan(DrawArgs a) {
 //
 // Each wave takes 64 consecutive spans (one per lane) and shades their
 // fragments 64 at a time, one fragment per lane: a lane finds the span that
-// owns its fragment by a binary search over the wave's running fragment
-// counts, takes the span's edge values from the owning lane, replays the
-// reference's per-pixel additions up to its pixel and runs the shader.
+// owns its fragment (each span marks the window position of its first
+// fragment, a max-scan carries the mark over the span's pixels), takes the
+// span's edge values from the owning lane, replays the reference's
+// per-pixel additions up to its pixel and runs the shader.
 
-// Waves per SIMD the untextured Phong kernel is compiled for (its register
-// budget): four hide the attribute-fetch latency better than the three an
-// unconstrained build fits, at the cost of a few spilled registers.
 // Inclusive max-scan over the 64 lanes of a wave with DPP row shifts and
 // row broadcasts (values >= 0; lanes with no source contribute 0).
 FGL_D uint32_t wave_incl_max(uint32_t v) {
@@ -412,20 +417,137 @@ FGL_D uint32_t wave_incl_max(uint32_t v) {
     return v;
 }
 
+// Waves per SIMD the untextured Phong kernel is compiled for (its register
+// budget).
 #ifndef FGL_SHADE_WAVES
 #define FGL_SHADE_WAVES 4
 #endif
-#ifndef FGL_SHADE_STAGED
-#define FGL_SHADE_STAGED 0
+
+// Per-triangle shading inputs a wave shares: the first span of each run of
+// spans of one raster triangle (runs are consecutive in the span list: one
+// triangle's rows within one tile) loads the triangle's record fields and
+// vertex attributes once into an LDS slot, and the fragments of every span
+// of the run read them from there.  Fragment lanes of one triangle read the
+// same LDS address (a broadcast), so the per-fragment vector-memory return
+// traffic that bounded the one-load-per-fragment form is gone.  Triangles
+// past the wave's SHADE_SLOTS slots load from global memory per fragment.
+#ifndef FGL_SHADE_SLOTS
+#define FGL_SHADE_SLOTS 32
 #endif
-#ifndef FGL_SHADE_OWNER_DPP
-#define FGL_SHADE_OWNER_DPP 1
-#endif
+constexpr int SHADE_SLOTS = FGL_SHADE_SLOTS;
+
+// Slot layout in doubles: a12, a20, a01 (the per-pixel steps of the edge
+// values), sz[3], r[3], ra, the attr word, one pad, then the attributes the
+// shader reads (Phong: positions, normals; textures: u, v).
+template <int SH>
+struct SlotLayout {
+    static constexpr bool kPN = SH >= 3;
+    static constexpr bool kUV = SH == 2 || SH == 4;
+    static constexpr int kAttr = 12;
+    static constexpr int kUVat = kAttr + (kPN ? 18 : 0);
+    static constexpr int N = kUVat + (kUV ? 6 : 0);
+};
+
+// The per-fragment inputs taken from a raster triangle's record.
+struct TriIn {
+    double a12, a20, a01;
+    double sz[3], r[3], ra;
+    uint64_t attr;
+};
+
+FGL_D void tri_from_rec(const PrimRec *pr, TriIn &t) {
+    t.a12 = pr->sy[2] - pr->sy[1];
+    t.a20 = pr->sy[0] - pr->sy[2];
+    t.a01 = pr->sy[1] - pr->sy[0];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        t.sz[k] = pr->sz[k];
+        t.r[k] = pr->r[k];
+    }
+    t.ra = pr->ra;
+    t.attr = pr->attr;
+}
+
+template <int SH>
+FGL_D void slot_store(double *d, const TriIn &t, const ShadeIn &in) {
+    using L = SlotLayout<SH>;
+    d[0] = t.a12;
+    d[1] = t.a20;
+    d[2] = t.a01;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        d[3 + k] = t.sz[k];
+        d[6 + k] = t.r[k];
+    }
+    d[9] = t.ra;
+    d[10] = __builtin_bit_cast(double, t.attr);
+    d[11] = 0.0;
+    if (L::kPN) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            d[L::kAttr + 3 * k + 0] = in.p[k].x;
+            d[L::kAttr + 3 * k + 1] = in.p[k].y;
+            d[L::kAttr + 3 * k + 2] = in.p[k].z;
+            d[L::kAttr + 9 + 3 * k + 0] = in.n[k].x;
+            d[L::kAttr + 9 + 3 * k + 1] = in.n[k].y;
+            d[L::kAttr + 9 + 3 * k + 2] = in.n[k].z;
+        }
+    }
+    if (L::kUV) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            d[L::kUVat + k] = in.u[k];
+            d[L::kUVat + 3 + k] = in.v[k];
+        }
+    }
+}
+
+template <int SH>
+FGL_D void slot_load(const double *d, TriIn &t, ShadeIn &in) {
+    using L = SlotLayout<SH>;
+    t.a12 = d[0];
+    t.a20 = d[1];
+    t.a01 = d[2];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        t.sz[k] = d[3 + k];
+        t.r[k] = d[6 + k];
+    }
+    t.ra = d[9];
+    t.attr = __builtin_bit_cast(uint64_t, d[10]);
+    if (L::kPN) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            in.p[k] = v3(d[L::kAttr + 3 * k], d[L::kAttr + 3 * k + 1], d[L::kAttr + 3 * k + 2]);
+            in.n[k] = v3(d[L::kAttr + 9 + 3 * k], d[L::kAttr + 9 + 3 * k + 1], d[L::kAttr + 9 + 3 * k + 2]);
+        }
+    }
+    if (L::kUV) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            in.u[k] = d[L::kUVat + k];
+            in.v[k] = d[L::kUVat + 3 + k];
+        }
+    }
+}
+
+// Vertex attributes that the slot does not hold (Phong's vertex colors,
+// read when ObjectColor is Discard), fetched per fragment.
+template <int SH>
+FGL_D void load_extra(const DevShader &sh, const AttrSrc &src, uint64_t attr, ShadeIn &in) {
+    if (SH == 3 && sh.obj_is_discard) {
+        const VtxRef r = vtx_ref(attr);
+#pragma unroll
+        for (int k = 0; k < 3; k++) in.c[k] = fetch_col(src, r, k);
+    }
+}
 
 template <int SH>
 __global__ __launch_bounds__(SPAN_THREADS, SH == 3 ? FGL_SHADE_WAVES : 1) void k_shade(DrawArgs a) {
+    using L = SlotLayout<SH>;
     __shared__ uint32_t wsum[SPAN_THREADS / 64];
     __shared__ uint32_t ostart[SPAN_THREADS / 64][64];
+    __shared__ double slots[SPAN_THREADS / 64][SHADE_SLOTS][L::N];
     if (a.ctr->overflow) return;
     const uint32_t NS = a.ctr->nspans;
     const uint32_t nb = (NS + SPAN_THREADS - 1) / SPAN_THREADS;
@@ -444,7 +566,6 @@ __global__ __launch_bounds__(SPAN_THREADS, SH == 3 ? FGL_SHADE_WAVES : 1) void k
     }
     if (total > a.frag_cap) return;  // the host grows the fragment buffer and reruns
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const AttrSrc src = attr_src(a);
     for (uint32_t blk = blockIdx.x; blk < nb; blk += gridDim.x) {
         const uint32_t i = blk * SPAN_THREADS + threadIdx.x;
         const Span sp = i < NS ? a.spans[i] : Span{{0, 0, 0}, 0, 0, 0};
@@ -465,12 +586,31 @@ __global__ __launch_bounds__(SPAN_THREADS, SH == 3 ? FGL_SHADE_WAVES : 1) void k
         const uint32_t T = __shfl(incl, 63);
         const uint32_t pn = (uint32_t)sp.pix | ((uint32_t)sp.n << 16);
         Frag *out = a.frags + fo;
-#if FGL_SHADE_OWNER_DPP
+
+        // the wave's triangles: a span whose predecessor lane holds another
+        // triangle starts a run; slot = runs started up to this lane, - 1
+        const uint32_t prev_ref = __shfl_up(sp.ref, 1);
+        const bool lead = i < NS && (lane == 0 || prev_ref != sp.ref);
+        const uint64_t lm = __ballot(lead);
+        const uint32_t before = __builtin_amdgcn_mbcnt_hi((uint32_t)(lm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)lm, 0u));
+        // staging pays when the wave's triangles average two fragments or
+        // more; otherwise every fragment loads its own inputs
+        const bool staged = T >= 2u * (uint32_t)__popcll(lm);
+        const uint32_t myslot = staged ? before + (lead ? 1u : 0u) - 1u : ~0u;  // lanes >= NS own no fragments
+        if (staged && lead && before < (uint32_t)SHADE_SLOTS) {
+            const DrawArgs &ka = args_reload();
+            const PrimRec *pr = prim_of(ka, sp.ref);
+            TriIn t;
+            tri_from_rec(pr, t);
+            ShadeIn in;
+            if (L::kPN || L::kUV) load_in<SH, false>(ka.sh, attr_src(ka), t.attr, in);
+            slot_store<SH>(&slots[wave][before][0], t, in);
+        }
+        __builtin_amdgcn_wave_barrier();
+
         uint32_t carry = 0;  // owner lane + 1 of the last fragment of the previous window
-#endif
         for (uint32_t base = 0; base < T; base += 64) {
             const uint32_t p = base + (uint32_t)lane;
-#if FGL_SHADE_OWNER_DPP
             // owner: each span starting in this window marks its start with
             // its lane (+1); a max-scan carries it over the span's pixels
             ostart[wave][lane] = 0;
@@ -481,50 +621,41 @@ __global__ __launch_bounds__(SPAN_THREADS, SH == 3 ? FGL_SHADE_WAVES : 1) void k
             carry = __builtin_amdgcn_readlane(om, 63);
             __builtin_amdgcn_wave_barrier();
             const int ow = (int)om - 1 < 0 ? 0 : (int)om - 1;
-#else
-            // owner: the number of lanes whose running count is <= p
-            int lo = 0;
-#pragma unroll
-            for (int step = 32; step > 0; step >>= 1) {
-                const uint32_t v = __shfl(incl, lo + step - 1);
-                if (v <= p) lo += step;
-            }
-            const int ow = lo < 64 ? lo : 63;
-#endif
             const uint32_t oexcl = __shfl(excl, ow);
             const uint32_t oref = __shfl(sp.ref, ow);
             const uint32_t opn = __shfl(pn, ow);
+            const uint32_t oslot = __shfl(myslot, ow);
             double w0 = __shfl(sp.w[0], ow);
             double w1 = __shfl(sp.w[1], ow);
             double w2 = __shfl(sp.w[2], ow);
             if (p < T) {
+                const DrawArgs &ka = args_reload();
+                const AttrSrc src = attr_src(ka);
                 const uint32_t j = p - oexcl;
-                const PrimRec *pr = prim_of(a, oref);
-                const double a12 = pr->sy[2] - pr->sy[1];
-                const double a20 = pr->sy[0] - pr->sy[2];
-                const double a01 = pr->sy[1] - pr->sy[0];
+                TriIn t;
+                ShadeIn in;
+                if (oslot < (uint32_t)SHADE_SLOTS) {
+                    slot_load<SH>(&slots[wave][oslot][0], t, in);
+                } else {
+                    const PrimRec *pr = prim_of(ka, oref);
+                    tri_from_rec(pr, t);
+                    if (L::kPN || L::kUV) load_in<SH, false>(ka.sh, src, t.attr, in);
+                }
+                load_extra<SH>(ka.sh, src, t.attr, in);
                 // the reference's per-pixel step, j times (context.go:208-213)
 #pragma unroll 4
                 for (uint32_t k = 0; k < j; k++) {
-                    w0 += a12;
-                    w1 += a20;
-                    w2 += a01;
+                    w0 += t.a12;
+                    w1 += t.a20;
+                    w2 += t.a01;
                 }
-                const double ra = pr->ra;
-                const double b0 = w0 * ra, b1 = w1 * ra, b2 = w2 * ra;
+                const double b0 = w0 * t.ra, b1 = w1 * t.ra, b2 = w2 * t.ra;
                 Frag f;
-                f.z = b0 * pr->sz[0] + b1 * pr->sz[1] + b2 * pr->sz[2];
+                f.z = b0 * t.sz[0] + b1 * t.sz[1] + b2 * t.sz[2];
                 // perspective-correct interpolation weights (context.go:236-237)
-                V4 bw = v4(b0 * pr->r[0], b1 * pr->r[1], b2 * pr->r[2], 0);
+                V4 bw = v4(b0 * t.r[0], b1 * t.r[1], b2 * t.r[2], 0);
                 bw.w = 1.0 / (bw.x + bw.y + bw.z);
-                C4 c;
-                if (SH == 3 && FGL_SHADE_STAGED) {
-                    c = eval_phong_staged(a.sh, src, pr->attr, bw);
-                } else {
-                    ShadeIn in;
-                    load_in<SH>(a.sh, src, pr->attr, in);
-                    c = eval<SH>(a.sh, in, bw);
-                }
+                const C4 c = eval<SH>(ka.sh, in, bw);
                 f.color = nrgba_pack(c);
                 f.pix = (uint16_t)((opn & 0xffffu) + j);
                 f.flags = ((opn >> 16) & SPAN_INB ? FRAG_INB : (uint16_t)0) | (c.a < 1 ? FRAG_BLEND : (uint16_t)0) |
