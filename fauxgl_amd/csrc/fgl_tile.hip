@@ -23,7 +23,7 @@
 namespace fgl {
 
 #ifndef FGL_ROP_CHUNK
-#define FGL_ROP_CHUNK 2048
+#define FGL_ROP_CHUNK 1024
 #endif
 constexpr int ROP_CHUNK = FGL_ROP_CHUNK;                 // fragments per round set
 constexpr int ROP_PER_THREAD = ROP_CHUNK / TILE_THREADS;
