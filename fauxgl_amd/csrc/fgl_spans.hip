@@ -93,15 +93,28 @@ FGL_D void add_frags(const DrawArgs &a, uint32_t *lfb, uint32_t blk0, uint32_t b
     else atomicAdd((unsigned long long *)&a.frag_blocksum[blk], (unsigned long long)cnt);
 }
 
-// One pair: setup once, then the rows in order (spans written at base..).
-FGL_D void walk_pair(const DrawArgs &a, uint32_t p, uint32_t n, uint64_t base, const uint32_t *keys,
-                     const uint32_t *vals, uint32_t *lfb, uint32_t blk0) {
+// Per-pair setup a wave shares through LDS: the edge values at the first
+// row the tile needs (the row recurrence of context.go:275-277 run up to
+// it), the steps, the reciprocals of the skip (context.go:185-206) and the
+// box.
+struct PairSetup {
+    double w0, w1, w2;        // edge values at (x0 + 0.5, ya + 0.5)
+    double a12, a20, a01;     // per-pixel steps
+    double b12, b20, b01;     // per-row steps
+    double ra12, ra20, ra01;  // 1 / a12, 1 / a20, 1 / a01
+    double ra;                // 1 / edge(s0, s1, s2)
+    int32_t x0, x1, ya, kmin, nk;
+    int32_t cx0, cx1, cy0, cy1;
+    uint32_t ref;
+};
+
+FGL_D void pair_setup(const DrawArgs &a, const uint32_t *keys, const uint32_t *vals, uint32_t p, PairSetup &q) {
     const uint32_t ref = vals[p];
     int32_t cx0, cx1, cy0, cy1;
     tile_rect(a, keys[p], cx0, cx1, cy0, cy1);
     const PrimRec *pr = prim_rec(a, ref);
     const int32_t x0 = pr->x0, y0 = pr->y0, x1 = pr->x1, y1 = pr->y1;
-    ItemGeo g = item_geo(x0, y0, x1, y1, a.W, cx0, cx1, cy0, cy1);
+    const ItemGeo g = item_geo(x0, y0, x1, y1, a.W, cx0, cx1, cy0, cy1);
     const double s0x = pr->sx[0], s0y = pr->sy[0], s1x = pr->sx[1], s1y = pr->sy[1], s2x = pr->sx[2],
                  s2y = pr->sy[2];
     // forward differencing variables and reciprocals (context.go:162-181)
@@ -109,99 +122,119 @@ FGL_D void walk_pair(const DrawArgs &a, uint32_t p, uint32_t n, uint64_t base, c
     double w00 = edge(s1x, s1y, s2x, s2y, px, py);
     double w01 = edge(s2x, s2y, s0x, s0y, px, py);
     double w02 = edge(s0x, s0y, s1x, s1y, px, py);
-    const double a01 = s1y - s0y, b01 = s0x - s1x;
-    const double a12 = s2y - s1y, b12 = s1x - s2x;
-    const double a20 = s0y - s2y, b20 = s2x - s0x;
-    const double ra = pr->ra;  // 1 / edge(s0, s1, s2), computed with the record
-    const double ra12 = 1.0 / a12, ra20 = 1.0 / a20, ra01 = 1.0 / a01;
+    q.a01 = s1y - s0y;
+    q.b01 = s0x - s1x;
+    q.a12 = s2y - s1y;
+    q.b12 = s1x - s2x;
+    q.a20 = s0y - s2y;
+    q.b20 = s2x - s0x;
+    q.ra = pr->ra;  // 1 / edge(s0, s1, s2), computed with the record
+    q.ra12 = 1.0 / q.a12;
+    q.ra20 = 1.0 / q.a20;
+    q.ra01 = 1.0 / q.a01;
     // rows y0 .. ya-1 of the row recurrence (context.go:275-277)
     for (int32_t y = y0; y < g.ya; y++) {
-        w00 += b12;
-        w01 += b20;
-        w02 += b01;
+        w00 += q.b12;
+        w01 += q.b20;
+        w02 += q.b01;
     }
-    Span *out = a.spans + base;
-    uint32_t m = 0;
-    // fragments per SPAN_THREADS-span block, added once per block touched
-    uint32_t fblk = (uint32_t)(base / SPAN_THREADS);
-    uint64_t facc = 0;
-    for (int32_t yy = 0; yy < g.nY && m < n; yy++) {
-        const int32_t y = g.ya + yy;
-        // skip to the first pixel that can be inside (context.go:185-206)
-        double d = 0;
-        double d0 = -w00 * ra12;
-        double d1 = -w01 * ra20;
-        double d2 = -w02 * ra01;
-        if (w00 < 0 && d0 > d) d = d0;
-        if (w01 < 0 && d1 > d) d = d1;
-        if (w02 < 0 && d2 > d) d = d2;
-        d = (double)go_f2i(d);
-        if (d < 0) d = 0;
-        const double w0s = w00 + a12 * d;
-        const double w1s = w01 + a20 * d;
-        const double w2s = w02 + a01 * d;
-        const int64_t id = go_f2i(d);
-        const int32_t xs = (id > (int64_t)x1 - x0) ? x1 + 1 : x0 + (int32_t)id;
-        for (int32_t kk = 0; kk < g.nk && m < n; kk++, m++) {
-            const int32_t k = g.kmin + kk;
-            const int32_t kw = k * a.W;
-            const int32_t pyk = y + k;
-            int32_t xa = x0 > kw + cx0 ? x0 : kw + cx0;
-            int32_t xb = x1 < kw + cx1 ? x1 : kw + cx1;
-            Span sp;
-            sp.w[0] = 0;
-            sp.w[1] = 0;
-            sp.w[2] = 0;
-            sp.ref = ref;
-            sp.pix = 0;
-            sp.n = 0;
-            if (pyk >= cy0 && pyk <= cy1 && xa <= xb) {
-                double w0 = w0s, w1 = w1s, w2 = w2s;
-                bool was_inside = false;
-                uint32_t cnt = 0;
-                for (int32_t x = xs; x <= xb; x++) {
-                    double b0 = w0 * ra, b1 = w1 * ra, b2 = w2 * ra;
-                    if (b0 < 0 || b1 < 0 || b2 < 0) {
-                        if (was_inside) break;
-                    } else {
-                        was_inside = true;
-                        if (x >= xa) {
-                            if (cnt == 0) {
-                                sp.w[0] = w0;
-                                sp.w[1] = w1;
-                                sp.w[2] = w2;
-                                sp.pix = (uint16_t)((pyk - cy0) * TILE_W + (x - kw - cx0));
-                            }
-                            cnt++;
-                        }
-                    }
-                    w0 += a12;
-                    w1 += a20;
-                    w2 += a01;
-                }
-                sp.n = (uint16_t)(cnt | (k == 0 ? SPAN_INB : 0u));
-                const uint32_t blk = (uint32_t)((base + m) / SPAN_THREADS);
-                if (blk != fblk) {
-                    add_frags(a, lfb, blk0, fblk, facc);
-                    fblk = blk;
-                    facc = 0;
-                }
-                facc += cnt;
-            }
-            out[m] = sp;
-        }
-        w00 += b12;
-        w01 += b20;
-        w02 += b01;
-    }
-    add_frags(a, lfb, blk0, fblk, facc);
+    q.w0 = w00;
+    q.w1 = w01;
+    q.w2 = w02;
+    q.x0 = x0;
+    q.x1 = x1;
+    q.ya = g.ya;
+    q.kmin = g.kmin;
+    q.nk = g.nk;
+    q.cx0 = cx0;
+    q.cx1 = cx1;
+    q.cy0 = cy0;
+    q.cy1 = cy1;
+    q.ref = ref;
 }
 
-// One thread per pair.  Also adds the covered pixels into the fragment
-// count of each span block it writes.
+// Span m of a pair (row yy = m / nk, wrapped column kk = m % nk): the row's
+// edge values (yy row steps from ya), the skip to the first pixel that can
+// be inside, then the pixel walk of the reference's row loop
+// (context.go:185-221) through the tile's columns.  Returns the span and
+// its covered-pixel count.
+FGL_D Span pair_span(const PairSetup &q, uint32_t m, int32_t W, uint32_t &cnt_out) {
+    const int32_t yy = (int32_t)(m / (uint32_t)q.nk), kk = (int32_t)(m % (uint32_t)q.nk);
+    double w00 = q.w0, w01 = q.w1, w02 = q.w2;
+    for (int32_t r = 0; r < yy; r++) {
+        w00 += q.b12;
+        w01 += q.b20;
+        w02 += q.b01;
+    }
+    const int32_t y = q.ya + yy;
+    // skip to the first pixel that can be inside (context.go:185-206)
+    double d = 0;
+    const double d0 = -w00 * q.ra12;
+    const double d1 = -w01 * q.ra20;
+    const double d2 = -w02 * q.ra01;
+    if (w00 < 0 && d0 > d) d = d0;
+    if (w01 < 0 && d1 > d) d = d1;
+    if (w02 < 0 && d2 > d) d = d2;
+    d = (double)go_f2i(d);
+    if (d < 0) d = 0;
+    double w0 = w00 + q.a12 * d;
+    double w1 = w01 + q.a20 * d;
+    double w2 = w02 + q.a01 * d;
+    const int64_t id = go_f2i(d);
+    const int32_t xs = (id > (int64_t)q.x1 - q.x0) ? q.x1 + 1 : q.x0 + (int32_t)id;
+    const int32_t k = q.kmin + kk;
+    const int32_t kw = k * W;
+    const int32_t pyk = y + k;
+    const int32_t xa = q.x0 > kw + q.cx0 ? q.x0 : kw + q.cx0;
+    const int32_t xb = q.x1 < kw + q.cx1 ? q.x1 : kw + q.cx1;
+    Span sp;
+    sp.w[0] = 0;
+    sp.w[1] = 0;
+    sp.w[2] = 0;
+    sp.ref = q.ref;
+    sp.pix = 0;
+    sp.n = 0;
+    uint32_t cnt = 0;
+    if (pyk >= q.cy0 && pyk <= q.cy1 && xa <= xb) {
+        bool was_inside = false;
+        for (int32_t x = xs; x <= xb; x++) {
+            const double b0 = w0 * q.ra, b1 = w1 * q.ra, b2 = w2 * q.ra;
+            if (b0 < 0 || b1 < 0 || b2 < 0) {
+                if (was_inside) break;
+            } else {
+                was_inside = true;
+                if (x >= xa) {
+                    if (cnt == 0) {
+                        sp.w[0] = w0;
+                        sp.w[1] = w1;
+                        sp.w[2] = w2;
+                        sp.pix = (uint16_t)((pyk - q.cy0) * TILE_W + (x - kw - q.cx0));
+                    }
+                    cnt++;
+                }
+            }
+            w0 += q.a12;
+            w1 += q.a20;
+            w2 += q.a01;
+        }
+        sp.n = (uint16_t)(cnt | (k == 0 ? SPAN_INB : 0u));
+    }
+    cnt_out = cnt;
+    return sp;
+}
+
+// One thread per pair sets up; then each wave writes its pairs' spans 64
+// at a time, one span per lane, so lanes share the walks evenly however
+// many rows each pair has.  A lane finds its span's pair as the shading
+// kernel finds a fragment's span (each pair marks its first span's window
+// position, a max-scan carries the mark) and reads the pair's setup from
+// LDS.  Also adds the covered pixels into the fragment count of each span
+// block it writes.
 __global__ __launch_bounds__(SPAN_THREADS) void k_spans(DrawArgs a) {
     __shared__ uint64_t wsum[SPAN_THREADS / 64];
     __shared__ uint32_t lfb[FB_SLOTS];
+    __shared__ uint32_t ostart[SPAN_THREADS / 64][64];
+    __shared__ PairSetup setup[SPAN_THREADS / 64][64];
     if (a.ctr->overflow) return;
     const uint32_t E = a.ctr->npairs;
     const uint32_t *keys = a.keys[a.sorted_buf];
@@ -220,12 +253,36 @@ __global__ __launch_bounds__(SPAN_THREADS) void k_spans(DrawArgs a) {
     __syncthreads();
     const uint64_t bbase = a.span_blockoff[blockIdx.x];
     const uint32_t blk0 = (uint32_t)(bbase / SPAN_THREADS);
-    uint64_t base = bbase;
-    for (int w = 0; w < wave; w++) base += wsum[w];
-    base += v - n;
-    if (p < E) {
-        a.poff[p] = (uint32_t)base;
-        if (n) walk_pair(a, p, n, base, keys, vals, lfb, blk0);
+    uint64_t wbase = bbase;
+    for (int w = 0; w < wave; w++) wbase += wsum[w];
+    // this lane's pair's spans: [wbase + excl, wbase + excl + n)
+    const uint32_t excl = (uint32_t)(v - n);
+    const uint32_t T = (uint32_t)__shfl(v, 63);  // the wave's spans
+    if (p < E) a.poff[p] = (uint32_t)(wbase + excl);
+    if (n) pair_setup(a, keys, vals, p, setup[wave][lane]);
+    __builtin_amdgcn_wave_barrier();
+    Span *out = a.spans + wbase;
+    uint32_t carry = 0;  // pair lane + 1 of the last span of the previous window
+    for (uint32_t base = 0; base < T; base += 64) {
+        const uint32_t q = base + (uint32_t)lane;
+        ostart[wave][lane] = 0;
+        if (n && excl >= base && excl - base < 64) ostart[wave][excl - base] = (uint32_t)lane + 1;
+        __builtin_amdgcn_wave_barrier();
+        uint32_t om = wave_incl_max(ostart[wave][lane]);
+        om = om ? om : carry;
+        carry = __builtin_amdgcn_readlane(om, 63);
+        __builtin_amdgcn_wave_barrier();
+        const int ow = (int)om - 1 < 0 ? 0 : (int)om - 1;
+        const uint32_t oexcl = __shfl(excl, ow);
+        if (q < T) {
+            uint32_t cnt;
+            const Span sp = pair_span(setup[wave][ow], q - oexcl, a.W, cnt);
+            out[q] = sp;
+            if (cnt) {
+                const uint32_t blk = (uint32_t)((wbase + q) / SPAN_THREADS);
+                add_frags(a, lfb, blk0, blk, cnt);
+            }
+        }
     }
     __syncthreads();
     if (threadIdx.x < FB_SLOTS && lfb[threadIdx.x])

@@ -405,18 +405,6 @@ __global__ __launch_bounds__(FRAG_GROUP) void k_frag_scan(DrawArgs a) {
 // span's edge values from the owning lane, replays the reference's
 // per-pixel additions up to its pixel and runs the shader.
 
-// Inclusive max-scan over the 64 lanes of a wave with DPP row shifts and
-// row broadcasts (values >= 0; lanes with no source contribute 0).
-FGL_D uint32_t wave_incl_max(uint32_t v) {
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false));  // row_shr:1
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false));  // row_shr:2
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false));  // row_shr:4
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false));  // row_shr:8
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));  // row_bcast:15
-    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));  // row_bcast:31
-    return v;
-}
-
 // Waves per SIMD the untextured Phong kernel is compiled for (its register
 // budget).
 #ifndef FGL_SHADE_WAVES
