@@ -367,6 +367,9 @@ constexpr uint32_t SPAN_GRID_MAX = 2048;
 // (coalesced) into group-local offsets and writes the group's total; the
 // shading kernel adds the totals of the groups before its block.
 constexpr int FRAG_GROUP = 1024;
+// Group offsets the shading kernel keeps in LDS (groups beyond, more than
+// FRAG_GTAB * FRAG_GROUP * SPAN_THREADS spans, add their sums one by one).
+constexpr int FRAG_GTAB = 256;
 
 __global__ __launch_bounds__(FRAG_GROUP) void k_frag_scan(DrawArgs a) {
     __shared__ uint64_t wsum[FRAG_GROUP / 64];
@@ -536,13 +539,35 @@ __global__ __launch_bounds__(SPAN_THREADS, SH == 3 ? FGL_SHADE_WAVES : 1) void k
     __shared__ uint32_t wsum[SPAN_THREADS / 64];
     __shared__ uint32_t ostart[SPAN_THREADS / 64][64];
     __shared__ double slots[SPAN_THREADS / 64][SHADE_SLOTS][L::N];
+    __shared__ uint64_t gpre[FRAG_GTAB];  // exclusive prefix of the group sums
+    __shared__ uint64_t gw[SPAN_THREADS / 64];
     if (a.ctr->overflow) return;
     const uint32_t NS = a.ctr->nspans;
     const uint32_t nb = (NS + SPAN_THREADS - 1) / SPAN_THREADS;
     const uint32_t ngroups = (nb + FRAG_GROUP - 1) / FRAG_GROUP;
-    // fragment total from the group sums; every workgroup sees the same
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // fragment total and the groups' offsets from the group sums, loaded in
+    // parallel and scanned (every workgroup sees the same)
     uint64_t total = 0;
-    for (uint32_t g = 0; g < ngroups; g++) total += a.frag_gsum[g];
+    for (uint32_t g0 = 0; g0 < ngroups; g0 += SPAN_THREADS) {
+        const uint32_t g = g0 + threadIdx.x;
+        const uint64_t gv = g < ngroups ? a.frag_gsum[g] : 0;
+        uint64_t gi = gv;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint64_t y = __shfl_up(gi, off);
+            if (lane >= off) gi += y;
+        }
+        if (lane == 63) gw[wave] = gi;
+        __syncthreads();
+        uint64_t ge = total + gi - gv, gt = 0;
+        for (int w = 0; w < SPAN_THREADS / 64; w++) {
+            if (w < wave) ge += gw[w];
+            gt += gw[w];
+        }
+        if (g < (uint32_t)FRAG_GTAB) gpre[g] = ge;
+        total += gt;
+        __syncthreads();
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (total > a.frag_cap) atomicOr(&a.ctr->overflow, 32u);
         a.ctr->nfrags = total > a.frag_cap ? a.frag_cap : (uint32_t)total;
@@ -553,7 +578,6 @@ __global__ __launch_bounds__(SPAN_THREADS, SH == 3 ? FGL_SHADE_WAVES : 1) void k
         a.ctr->total = total;
     }
     if (total > a.frag_cap) return;  // the host grows the fragment buffer and reruns
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (uint32_t blk = blockIdx.x; blk < nb; blk += gridDim.x) {
         const uint32_t i = blk * SPAN_THREADS + threadIdx.x;
         const Span sp = i < NS ? a.spans[i] : Span{{0, 0, 0}, 0, 0, 0};
@@ -565,8 +589,14 @@ __global__ __launch_bounds__(SPAN_THREADS, SH == 3 ? FGL_SHADE_WAVES : 1) void k
         }
         if (lane == 63) wsum[wave] = incl;
         __syncthreads();
+        const uint32_t grp = blk / FRAG_GROUP;
         uint64_t fo = a.frag_blockoff[blk];
-        for (uint32_t g = 0; g < blk / FRAG_GROUP; g++) fo += a.frag_gsum[g];
+        if (grp < (uint32_t)FRAG_GTAB) {
+            fo += gpre[grp];
+        } else {
+            fo += gpre[FRAG_GTAB - 1] + a.frag_gsum[FRAG_GTAB - 1];
+            for (uint32_t g = FRAG_GTAB; g < grp; g++) fo += a.frag_gsum[g];
+        }
         for (int w = 0; w < wave; w++) fo += wsum[w];
         __syncthreads();
         const uint32_t excl = incl - n;
