@@ -16,6 +16,7 @@ FGL_E_INVALID = -1
 FGL_E_NOMEM = -2
 FGL_E_DEVICE = -3
 FGL_E_UNSUPPORTED = -4
+FGL_E_IO = -5
 
 FGL_FACE_CW, FGL_FACE_CCW = 1, 2
 FGL_CULL_NONE, FGL_CULL_FRONT, FGL_CULL_BACK = 1, 2, 3
@@ -36,6 +37,8 @@ EXPORTED = [
     "fgl_mesh_smooth_normals", "fgl_mesh_smooth_normals_threshold", "fgl_mesh_load_stl", "fgl_mesh_load_obj",
     "fgl_encode_png_bound", "fgl_encode_png",
     "fgl_batch_create", "fgl_batch_destroy", "fgl_batch_error", "fgl_batch_add", "fgl_batch_run",
+    "fgl_tile_shapes", "fgl_ctx_set_tile", "fgl_ctx_get_tile", "fgl_autotune_mesh", "fgl_tune_save",
+    "fgl_tune_load", "fgl_tune_clear",
 ]
 
 _d = ctypes.c_double
@@ -133,6 +136,14 @@ def load():
         "fgl_batch_add": (i32, [vp, vp, ctypes.POINTER(ctypes.c_uint8), _d, ctypes.POINTER(FglState),
                                 ctypes.POINTER(FglShader), vp]),
         "fgl_batch_run": (i32, [vp, u64p]),
+        "fgl_tile_shapes": (i32, [ctypes.POINTER(i32), i32, ctypes.POINTER(i32)]),
+        "fgl_ctx_set_tile": (i32, [vp, i32, i32]),
+        "fgl_ctx_get_tile": (i32, [vp, ctypes.c_int64, ctypes.POINTER(i32)]),
+        "fgl_autotune_mesh": (i32, [vp, ctypes.POINTER(FglState), ctypes.POINTER(FglShader), vp, i32,
+                                    ctypes.POINTER(i32)]),
+        "fgl_tune_save": (i32, [ctypes.c_char_p]),
+        "fgl_tune_load": (i32, [ctypes.c_char_p]),
+        "fgl_tune_clear": (i32, []),
         "fgl_encode_png_bound": (i32, [vp, ctypes.POINTER(ctypes.c_int64)]),
         "fgl_encode_png": (i32, [vp, ctypes.POINTER(ctypes.c_uint8), ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]),
     }

@@ -14,6 +14,7 @@ Every draw runs on the GPU; a shader that is not one of the three built-ins
 raises (the reference's Go CPU path is what serves custom shaders).
 """
 import ctypes
+import os
 import math
 from collections import namedtuple
 
@@ -706,6 +707,33 @@ class Context:
         """Keeps mesh resident on this context's device for repeated DrawMesh."""
         return DeviceMesh(self, mesh)
 
+    # tile shape and the autotuner (no reference counterpart: the images do
+    # not depend on the shape, only the speed does)
+    def SetTileShape(self, width=0, height=0):
+        """Fixes this context's tile shape (one of TileShapes()); (0, 0)
+        returns to the tuning table / the 32 x 32 default."""
+        N.check(N.load().fgl_ctx_set_tile(self._ptr(), int(width), int(height)), self._ptr())
+
+    def TileShape(self, ntri):
+        """The (w, h) a draw of ntri triangles on this context would use."""
+        out = (ctypes.c_int32 * 2)()
+        N.check(N.load().fgl_ctx_get_tile(self._ptr(), int(ntri), out), self._ptr())
+        return int(out[0]), int(out[1])
+
+    def Autotune(self, mesh, reps=3):
+        """Times DrawMesh(mesh) with the current state and shader for every
+        tile shape, records the fastest (and the capacities its draw needed)
+        in the process-wide tuning table and returns it as (w, h).  The
+        framebuffer is left as it was."""
+        if not isinstance(mesh, DeviceMesh):
+            mesh = DeviceMesh(self, mesh)
+        out = (ctypes.c_int32 * 2)()
+        st = self._state()
+        sh = shader_struct(self.Shader, self)
+        N.check(N.load().fgl_autotune_mesh(self._ptr(), ctypes.byref(st), ctypes.byref(sh), mesh.h, int(reps), out),
+                self._ptr())
+        return int(out[0]), int(out[1])
+
     # readback (context.go:83-117)
     def Image(self):
         out = np.empty((self.Height, self.Width, 4), dtype=np.uint8)
@@ -788,6 +816,28 @@ class Context:
         st = N.FglDrawStat()
         N.check(N.load().fgl_draw_stats(self._ptr(), ctypes.byref(st)), self._ptr())
         return {name: int(getattr(st, name)) for name, _ in N.FglDrawStat._fields_}
+
+
+def TileShapes():
+    """The compiled tile shapes as (w, h) pairs."""
+    n = ctypes.c_int32()
+    buf = (ctypes.c_int32 * 32)()
+    N.check(N.load().fgl_tile_shapes(buf, 16, ctypes.byref(n)))
+    return [(int(buf[2 * i]), int(buf[2 * i + 1])) for i in range(min(n.value, 16))]
+
+
+def SaveTuning(path):
+    """Writes the tuning table (tile shape and capacities per workload)."""
+    N.check(N.load().fgl_tune_save(os.fsencode(path)))
+
+
+def LoadTuning(path):
+    """Merges a tuning file written by SaveTuning into the table."""
+    N.check(N.load().fgl_tune_load(os.fsencode(path)))
+
+
+def ClearTuning():
+    N.check(N.load().fgl_tune_clear())
 
 
 def NewContext(width, height):

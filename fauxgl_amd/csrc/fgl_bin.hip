@@ -9,6 +9,7 @@
 #include "fgl_tiles.h"
 
 namespace fgl {
+namespace FGL_TILE_NS {
 
 // ---------------------------------------------------------------------------
 // per-draw reset
@@ -284,4 +285,5 @@ hipError_t launch_ranges(const DrawArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
+}  // namespace FGL_TILE_NS
 }  // namespace fgl

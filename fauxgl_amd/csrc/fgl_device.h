@@ -22,6 +22,12 @@ namespace fgl {
 #endif
 constexpr int TILE_W = FGL_TILE_W;
 constexpr int TILE_H = FGL_TILE_H;
+// The draw kernels (fgl_vs, fgl_bin, fgl_spans, fgl_tile) are compiled once
+// per tile shape, each compilation in its own namespace (t32x32, t64x32, ...);
+// the host picks a shape per draw.
+#define FGL_TNS2(w, h) t##w##x##h
+#define FGL_TNS1(w, h) FGL_TNS2(w, h)
+#define FGL_TILE_NS FGL_TNS1(FGL_TILE_W, FGL_TILE_H)
 constexpr int TILE_PIX = TILE_W * TILE_H;
 constexpr int TILE_THREADS = 256;
 // Raster triangles whose integer bounding box leaves [-BBOX_LIM, BBOX_LIM]

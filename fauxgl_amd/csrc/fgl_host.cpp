@@ -3,12 +3,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cfloat>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -98,6 +100,8 @@ struct fgl_ctx {
     DevBuf dpart, dimg;
     uint32_t side_rec_cap = 0, side_vtx_cap = 0, pair_cap = 0;
     DrawCounters last{};
+
+    int32_t tile_w = 0, tile_h = 0;  // tile shape set for this context (0: from the tuning table)
 
     bool prof = false;
     std::string prof_only;  // time only launches of this kernel (empty: all)
@@ -325,6 +329,70 @@ constexpr uint64_t kMaxSideVtx = (uint64_t)1 << 28;
 // they are enqueued once with the context's current capacities (buffers
 // already sized by an earlier draw of the same inputs), the draw counters
 // are copied to capture_ctr by a kernel, and nothing is read back.
+// ---------------------------------------------------------------------------
+// Tile shape and starting capacities per workload.  The autotuner
+// (fgl_autotune_mesh) times every compiled shape on a real draw and records
+// the fastest, with the pair / span / fragment capacities that draw needed,
+// under (width, height, primitive kind, bit length of the primitive count);
+// draws look the table up.  Every shape gives the same images: the shape
+// only changes how the framebuffer is cut into tiles.
+
+const TileOps *const kShapes[] = {
+#define FGL_SHAPE_PTR(w, h) &tile_ops_##w##x##h(),
+    FGL_TILE_SHAPES(FGL_SHAPE_PTR)
+#undef FGL_SHAPE_PTR
+};
+constexpr int kNumShapes = (int)(sizeof kShapes / sizeof kShapes[0]);
+
+const TileOps *find_shape(int32_t w, int32_t h) {
+    for (const TileOps *t : kShapes)
+        if (t->tw == w && t->th == h) return t;
+    return nullptr;
+}
+
+struct TuneKey {
+    int32_t W, H, kind, size_class;
+    bool operator<(const TuneKey &o) const {
+        if (W != o.W) return W < o.W;
+        if (H != o.H) return H < o.H;
+        if (kind != o.kind) return kind < o.kind;
+        return size_class < o.size_class;
+    }
+};
+struct TuneEntry {
+    int32_t tw = 32, th = 32;
+    uint32_t pair_cap = 0, span_cap = 0, frag_cap = 0;
+    double ms = 0;  // the timed draw (informative)
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneEntry> g_tune;
+
+int32_t size_class(int64_t n) {
+    int32_t b = 0;
+    while (b < 62 && ((int64_t)1 << b) <= n) b++;
+    return b;
+}
+
+// The shape a draw of n primitives uses: the context's own setting, else
+// the table's entry (whose capacities are reserved up front), else 32 x 32.
+const TileOps *draw_ops(fgl_ctx *c, int kind, int64_t n) {
+    if (c->tile_w) return find_shape(c->tile_w, c->tile_h);
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(TuneKey{c->W, c->H, kind, size_class(n)});
+    if (it == g_tune.end()) return kShapes[0];
+    const TuneEntry &e = it->second;
+    c->pair_cap = std::max(c->pair_cap, e.pair_cap);
+    c->span_cap = std::max(c->span_cap, e.span_cap);
+    c->frag_cap = std::max(c->frag_cap, e.frag_cap);
+    const TileOps *t = find_shape(e.tw, e.th);
+    return t ? t : kShapes[0];
+}
+
+uint32_t cap_for(uint32_t lo, uint32_t hi) {
+    const uint64_t v = ((uint64_t)hi << 32 | lo) * 5 / 4 + 65536;
+    return (uint32_t)std::min<uint64_t>(v, 0xfffff000ull);
+}
+
 int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, const DevPrims &in, uint64_t info[2],
              uint32_t *capture_ctr = nullptr) {
     if (info) info[0] = info[1] = 0;
@@ -344,8 +412,9 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
 
     a.W = c->W;
     a.H = c->H;
-    a.ntx = (c->W + TILE_W - 1) / TILE_W;
-    a.nty = (c->H + TILE_H - 1) / TILE_H;
+    const TileOps &T = *draw_ops(c, kind, in.n);
+    a.ntx = (c->W + T.tw - 1) / T.tw;
+    a.nty = (c->H + T.th - 1) / T.th;
     a.ntiles = a.ntx * a.nty;
     a.kind = kind;
     a.in = in;
@@ -465,26 +534,26 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
         a.ctr = c->ctr.as<DrawCounters>();
         a.zeros = c->zeros.as<double>();
 
-        if ((r = launch(c, "zero", [&] { return launch_zero(a, c->stream); }))) return r;
-        if ((r = launch(c, "vertex", [&] { return launch_vs(a, c->stream); }))) return r;
+        if ((r = launch(c, "zero", [&] { return T.zero(a, c->stream); }))) return r;
+        if ((r = launch(c, "vertex", [&] { return T.vs(a, c->stream); }))) return r;
         if (kind == PRIM_TRI && !a.st.wireframe)
-            if ((r = launch(c, "vertex_clip", [&] { return launch_vs_deferred(a, c->stream); }))) return r;
-        if ((r = launch(c, "scan", [&] { return launch_scan_top(a, c->stream); }))) return r;
-        if ((r = launch(c, "emit", [&] { return launch_emit(a, c->stream); }))) return r;
+            if ((r = launch(c, "vertex_clip", [&] { return T.vs_deferred(a, c->stream); }))) return r;
+        if ((r = launch(c, "scan", [&] { return T.scan_top(a, c->stream); }))) return r;
+        if ((r = launch(c, "emit", [&] { return T.emit(a, c->stream); }))) return r;
         for (int p = 0; p < npass; p++) {
-            if ((r = launch(c, "radix_hist", [&] { return launch_radix_hist(a, p, c->stream); }))) return r;
-            if ((r = launch(c, "radix_scan", [&] { return launch_radix_scan(a, c->stream); }))) return r;
-            if ((r = launch(c, "radix_scatter", [&] { return launch_radix_scatter(a, p, c->stream); }))) return r;
+            if ((r = launch(c, "radix_hist", [&] { return T.radix_hist(a, p, c->stream); }))) return r;
+            if ((r = launch(c, "radix_scan", [&] { return T.radix_scan(a, c->stream); }))) return r;
+            if ((r = launch(c, "radix_scatter", [&] { return T.radix_scatter(a, p, c->stream); }))) return r;
         }
-        if ((r = launch(c, "ranges", [&] { return launch_ranges(a, c->stream); }))) return r;
-        if ((r = launch(c, "span_count", [&] { return launch_span_count(a, c->stream); }))) return r;
-        if ((r = launch(c, "span_scan", [&] { return launch_span_scan(a, c->stream); }))) return r;
-        if ((r = launch(c, "spans", [&] { return launch_spans(a, c->stream); }))) return r;
-        if ((r = launch(c, "frag_scan", [&] { return launch_frag_scan(a, c->stream); }))) return r;
-        if ((r = launch(c, "shade", [&] { return launch_shade(a, c->stream); }))) return r;
-        if ((r = launch(c, "tile_frags", [&] { return launch_tile_frags(a, c->stream); }))) return r;
-        if ((r = launch(c, "rop", [&] { return launch_tile(a, c->stream); }))) return r;
-        if ((r = launch(c, "updated", [&] { return launch_updated(a, c->stream); }))) return r;
+        if ((r = launch(c, "ranges", [&] { return T.ranges(a, c->stream); }))) return r;
+        if ((r = launch(c, "span_count", [&] { return T.span_count(a, c->stream); }))) return r;
+        if ((r = launch(c, "span_scan", [&] { return T.span_scan(a, c->stream); }))) return r;
+        if ((r = launch(c, "spans", [&] { return T.spans(a, c->stream); }))) return r;
+        if ((r = launch(c, "frag_scan", [&] { return T.frag_scan(a, c->stream); }))) return r;
+        if ((r = launch(c, "shade", [&] { return T.shade(a, c->stream); }))) return r;
+        if ((r = launch(c, "tile_frags", [&] { return T.tile_frags(a, c->stream); }))) return r;
+        if ((r = launch(c, "rop", [&] { return T.rop(a, c->stream); }))) return r;
+        if ((r = launch(c, "updated", [&] { return T.updated(a, c->stream); }))) return r;
         if (capture_ctr)
             return launch(c, "counters", [&] {
                 return launch_copy_words(reinterpret_cast<const uint32_t *>(a.ctr), capture_ctr,
@@ -843,6 +912,200 @@ int fgl_draw_mesh(fgl_ctx *ctx, const fgl_state *state, const fgl_shader *shader
         info[0] = i1[0] + i2[0];
         info[1] = i1[1] + i2[1];
     }
+    return FGL_OK;
+}
+
+int fgl_tile_shapes(int32_t *wh, int32_t cap, int32_t *n) {
+    if (!n || (cap > 0 && !wh)) {
+        g_err = "fgl_tile_shapes: null argument";
+        return FGL_E_INVALID;
+    }
+    *n = kNumShapes;
+    for (int i = 0; i < kNumShapes && i < cap; i++) {
+        wh[2 * i] = kShapes[i]->tw;
+        wh[2 * i + 1] = kShapes[i]->th;
+    }
+    return FGL_OK;
+}
+
+int fgl_ctx_set_tile(fgl_ctx *ctx, int32_t tile_w, int32_t tile_h) {
+    if (check_ctx(ctx)) return FGL_E_INVALID;
+    if ((tile_w || tile_h) && !find_shape(tile_w, tile_h)) {
+        set_err(ctx->err, "fgl_ctx_set_tile: %d x %d is not a compiled tile shape", tile_w, tile_h);
+        return FGL_E_INVALID;
+    }
+    ctx->tile_w = tile_w;
+    ctx->tile_h = tile_h;
+    return FGL_OK;
+}
+
+int fgl_ctx_get_tile(fgl_ctx *ctx, int64_t ntri, int32_t out_wh[2]) {
+    if (check_ctx(ctx)) return FGL_E_INVALID;
+    if (!out_wh || ntri < 0) {
+        ctx->err = "fgl_ctx_get_tile: invalid argument";
+        return FGL_E_INVALID;
+    }
+    const int32_t sw = ctx->tile_w, sh = ctx->tile_h;
+    const uint32_t caps[3] = {ctx->pair_cap, ctx->span_cap, ctx->frag_cap};
+    const TileOps *t = draw_ops(ctx, PRIM_TRI, ntri);
+    ctx->pair_cap = caps[0];  // a query reserves nothing
+    ctx->span_cap = caps[1];
+    ctx->frag_cap = caps[2];
+    ctx->tile_w = sw;
+    ctx->tile_h = sh;
+    out_wh[0] = t->tw;
+    out_wh[1] = t->th;
+    return FGL_OK;
+}
+
+int fgl_autotune_mesh(fgl_ctx *ctx, const fgl_state *state, const fgl_shader *shader, const fgl_mesh *mesh,
+                      int32_t reps, int32_t out_wh[2]) {
+    if (check_ctx(ctx)) return FGL_E_INVALID;
+    if (!state || !shader || !mesh || reps < 1 || reps > 100) {
+        ctx->err = "fgl_autotune_mesh: invalid argument";
+        return FGL_E_INVALID;
+    }
+    if (hipSetDevice(ctx->device) != hipSuccess) {
+        ctx->err = "hipSetDevice failed";
+        return FGL_E_DEVICE;
+    }
+    // the context's framebuffer, pending clears, shape and counters are put
+    // back afterwards: tuning leaves no trace but the table entry
+    const size_t npix = (size_t)ctx->W * ctx->H;
+    DevBuf keep_color, keep_depth;
+    HIPCHK(ctx, keep_color.ensure(npix * 4));
+    HIPCHK(ctx, keep_depth.ensure(npix * 8));
+    HIPCHK(ctx, hipMemcpyAsync(keep_color.p, ctx->color.p, npix * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(keep_depth.p, ctx->depth.p, npix * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    const bool pc = ctx->pend_color, pd = ctx->pend_depth;
+    const int32_t sw = ctx->tile_w, sh = ctx->tile_h;
+    const DrawCounters last = ctx->last;
+    const bool prof = ctx->prof;
+    ctx->prof = false;
+    TuneEntry best;
+    best.ms = -1;
+    int r = FGL_OK;
+    for (int k = 0; k < kNumShapes && r == FGL_OK; k++) {
+        ctx->tile_w = kShapes[k]->tw;
+        ctx->tile_h = kShapes[k]->th;
+        double t_min = 1e300;
+        for (int i = 0; i <= reps && r == FGL_OK; i++) {  // the first draw grows the buffers
+            ctx->pend_color = ctx->pend_depth = true;
+            HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+            const auto t0 = std::chrono::steady_clock::now();
+            r = fgl_draw_mesh(ctx, state, shader, mesh, nullptr);
+            const auto t1 = std::chrono::steady_clock::now();
+            if (i > 0) t_min = std::min(t_min, std::chrono::duration<double, std::milli>(t1 - t0).count());
+        }
+        if (r) break;
+        if (best.ms < 0 || t_min < best.ms) {
+            best.tw = kShapes[k]->tw;
+            best.th = kShapes[k]->th;
+            best.ms = t_min;
+            best.pair_cap = cap_for(ctx->last.pairs_lo, ctx->last.pairs_hi);
+            best.span_cap = cap_for(ctx->last.spans_lo, ctx->last.spans_hi);
+            best.frag_cap = cap_for(ctx->last.frags_lo, ctx->last.frags_hi);
+        }
+    }
+    ctx->tile_w = sw;
+    ctx->tile_h = sh;
+    ctx->prof = prof;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->color.p, keep_color.p, npix * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->depth.p, keep_depth.p, npix * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->pend_color = pc;
+    ctx->pend_depth = pd;
+    ctx->last = last;
+    if (r) return r;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune[TuneKey{ctx->W, ctx->H, PRIM_TRI, size_class(mesh->ntri)}] = best;
+    }
+    if (out_wh) {
+        out_wh[0] = best.tw;
+        out_wh[1] = best.th;
+    }
+    return FGL_OK;
+}
+
+static const char kTuneHeader[] = "fauxgl_amd tile tuning 1";
+
+int fgl_tune_save(const char *path) {
+    if (!path) {
+        g_err = "fgl_tune_save: null path";
+        return FGL_E_INVALID;
+    }
+    FILE *f = std::fopen(path, "w");
+    if (!f) {
+        set_err(g_err, "fgl_tune_save: cannot open %s", path);
+        return FGL_E_IO;
+    }
+    std::fprintf(f, "%s\n# width height kind size_class tile_w tile_h pair_cap span_cap frag_cap ms\n", kTuneHeader);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        for (const auto &kv : g_tune)
+            std::fprintf(f, "%d %d %d %d %d %d %u %u %u %.6f\n", kv.first.W, kv.first.H, kv.first.kind,
+                         kv.first.size_class, kv.second.tw, kv.second.th, kv.second.pair_cap, kv.second.span_cap,
+                         kv.second.frag_cap, kv.second.ms);
+    }
+    const bool ok = std::fclose(f) == 0;
+    if (!ok) {
+        set_err(g_err, "fgl_tune_save: writing %s failed", path);
+        return FGL_E_IO;
+    }
+    return FGL_OK;
+}
+
+int fgl_tune_load(const char *path) {
+    if (!path) {
+        g_err = "fgl_tune_load: null path";
+        return FGL_E_INVALID;
+    }
+    FILE *f = std::fopen(path, "r");
+    if (!f) {
+        set_err(g_err, "fgl_tune_load: cannot open %s", path);
+        return FGL_E_IO;
+    }
+    char line[512];
+    std::map<TuneKey, TuneEntry> got;
+    int lineno = 0, r = FGL_OK;
+    while (std::fgets(line, sizeof line, f)) {
+        lineno++;
+        if (lineno == 1) {
+            if (std::strncmp(line, kTuneHeader, sizeof kTuneHeader - 1) != 0) {
+                set_err(g_err, "fgl_tune_load: %s is not a tuning file", path);
+                r = FGL_E_INVALID;
+                break;
+            }
+            continue;
+        }
+        if (line[0] == '#' || line[0] == '\n' || line[0] == 0) continue;
+        TuneKey k{};
+        TuneEntry e;
+        if (std::sscanf(line, "%d %d %d %d %d %d %u %u %u %lf", &k.W, &k.H, &k.kind, &k.size_class, &e.tw, &e.th,
+                        &e.pair_cap, &e.span_cap, &e.frag_cap, &e.ms) != 10 ||
+            !find_shape(e.tw, e.th) || k.W <= 0 || k.H <= 0 || (k.kind != PRIM_TRI && k.kind != PRIM_LINE) ||
+            e.pair_cap > 0xfffff000u || e.span_cap > 0xfffff000u || e.frag_cap > 0xfffff000u) {
+            set_err(g_err, "fgl_tune_load: %s line %d is malformed or names an unknown tile shape", path, lineno);
+            r = FGL_E_INVALID;
+            break;
+        }
+        got[k] = e;
+    }
+    std::fclose(f);
+    if (r == FGL_OK && lineno == 0) {
+        set_err(g_err, "fgl_tune_load: %s is empty", path);
+        r = FGL_E_INVALID;
+    }
+    if (r) return r;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (const auto &kv : got) g_tune[kv.first] = kv.second;
+    return FGL_OK;
+}
+
+int fgl_tune_clear(void) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune.clear();
     return FGL_OK;
 }
 

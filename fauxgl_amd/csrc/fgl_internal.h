@@ -92,6 +92,8 @@ constexpr int RS_BLOCK = RS_THREADS * RS_ITEMS;
 
 // Kernel launchers (stream-ordered).  Each returns the hipError_t of the launch.
 
+// The draw pipeline's launchers, one set per tile shape (see FGL_TILE_NS).
+namespace FGL_TILE_NS {
 hipError_t launch_zero(const DrawArgs &a, hipStream_t s);
 hipError_t launch_vs(const DrawArgs &a, hipStream_t s);
 hipError_t launch_vs_deferred(const DrawArgs &a, hipStream_t s);
@@ -109,6 +111,26 @@ hipError_t launch_shade(const DrawArgs &a, hipStream_t s);
 hipError_t launch_tile_frags(const DrawArgs &a, hipStream_t s);
 hipError_t launch_tile(const DrawArgs &a, hipStream_t s);
 hipError_t launch_updated(const DrawArgs &a, hipStream_t s);
+}  // namespace FGL_TILE_NS
+
+using DrawLaunch = hipError_t (*)(const DrawArgs &, hipStream_t);
+using PassLaunch = hipError_t (*)(const DrawArgs &, int, hipStream_t);
+
+// One tile shape's launchers, as the host calls them.
+struct TileOps {
+    int32_t tw, th;
+    DrawLaunch zero, vs, vs_deferred, scan_top, emit;
+    PassLaunch radix_hist;
+    DrawLaunch radix_scan;
+    PassLaunch radix_scatter;
+    DrawLaunch ranges, span_count, span_scan, spans, frag_scan, shade, tile_frags, rop, updated;
+};
+
+// The compiled shapes (width, height); 32 x 32 is the default.
+#define FGL_TILE_SHAPES(X) X(32, 32) X(64, 32) X(32, 16) X(64, 64)
+#define FGL_DECLARE_TILE_OPS(w, h) const TileOps &tile_ops_##w##x##h();
+FGL_TILE_SHAPES(FGL_DECLARE_TILE_OPS)
+#undef FGL_DECLARE_TILE_OPS
 
 hipError_t launch_copy_words(const uint32_t *src, uint32_t *dst, int32_t n, hipStream_t s);
 hipError_t launch_fill_color(uint32_t *color, int64_t n, uint32_t v, hipStream_t s);

@@ -13,6 +13,7 @@
 #include "fgl_tiles.h"
 
 namespace fgl {
+namespace FGL_TILE_NS {
 
 FGL_D const PrimRec *prim_rec(const DrawArgs &a, uint32_t ref) {
     return (ref & REF_SIDE) ? a.side_rec + (ref & ~REF_SIDE) : a.rec_main + ref;
@@ -306,4 +307,5 @@ hipError_t launch_spans(const DrawArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
+}  // namespace FGL_TILE_NS
 }  // namespace fgl

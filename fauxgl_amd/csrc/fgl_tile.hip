@@ -21,6 +21,7 @@
 #include "fgl_tiles.h"
 
 namespace fgl {
+namespace FGL_TILE_NS {
 
 #ifndef FGL_ROP_CHUNK
 #define FGL_ROP_CHUNK 1024
@@ -828,8 +829,21 @@ __global__ __launch_bounds__(TILE_THREADS, FGL_ROP_WAVES) void k_rop(DrawArgs a)
 __global__ __launch_bounds__(1024) void k_updated(DrawArgs a) {
     __shared__ unsigned long long wsum[1024 / 64];
     if (a.ctr->overflow) return;
+    // 16 independent loads per thread in flight (clamped addresses, the
+    // copies past the end masked), not one dependent load per pass
+    constexpr int ITEMS = 16;
+    const int nt = a.ntiles;
     unsigned long long v = 0;
-    for (int t = threadIdx.x; t < a.ntiles; t += 1024) v += a.tupd[t];
+    for (int t0 = 0; t0 < nt; t0 += 1024 * ITEMS) {
+        uint32_t x[ITEMS];
+#pragma unroll
+        for (int q = 0; q < ITEMS; q++) {
+            const int t = t0 + q * 1024 + (int)threadIdx.x;
+            x[q] = a.tupd[t < nt ? t : nt - 1];
+        }
+#pragma unroll
+        for (int q = 0; q < ITEMS; q++) v += t0 + q * 1024 + (int)threadIdx.x < nt ? x[q] : 0u;
+    }
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
     __syncthreads();
@@ -879,5 +893,38 @@ hipError_t launch_updated(const DrawArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_updated, dim3(1), dim3(1024), 0, s, a);
     return hipGetLastError();
 }
+
+}  // namespace FGL_TILE_NS
+
+// This shape's table (a host function: nothing of it is compiled for the
+// device).
+#define FGL_OPS2(w, h) tile_ops_##w##x##h
+#define FGL_OPS1(w, h) FGL_OPS2(w, h)
+const TileOps &FGL_OPS1(FGL_TILE_W, FGL_TILE_H)() {
+    static const TileOps ops = {
+        TILE_W,
+        TILE_H,
+        FGL_TILE_NS::launch_zero,
+        FGL_TILE_NS::launch_vs,
+        FGL_TILE_NS::launch_vs_deferred,
+        FGL_TILE_NS::launch_scan_top,
+        FGL_TILE_NS::launch_emit,
+        FGL_TILE_NS::launch_radix_hist,
+        FGL_TILE_NS::launch_radix_scan,
+        FGL_TILE_NS::launch_radix_scatter,
+        FGL_TILE_NS::launch_ranges,
+        FGL_TILE_NS::launch_span_count,
+        FGL_TILE_NS::launch_span_scan,
+        FGL_TILE_NS::launch_spans,
+        FGL_TILE_NS::launch_frag_scan,
+        FGL_TILE_NS::launch_shade,
+        FGL_TILE_NS::launch_tile_frags,
+        FGL_TILE_NS::launch_tile,
+        FGL_TILE_NS::launch_updated,
+    };
+    return ops;
+}
+#undef FGL_OPS1
+#undef FGL_OPS2
 
 }  // namespace fgl

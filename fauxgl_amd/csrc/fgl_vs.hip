@@ -9,6 +9,7 @@
 #include "fgl_tiles.h"
 
 namespace fgl {
+namespace FGL_TILE_NS {
 
 // A vertex as the pipeline carries it (vertex.go:3-12; Texture.Z dropped,
 // no built-in shader reads it).
@@ -578,4 +579,5 @@ hipError_t launch_vs_deferred(const DrawArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
+}  // namespace FGL_TILE_NS
 }  // namespace fgl

@@ -38,7 +38,8 @@ typedef enum fgl_status {
     FGL_E_INVALID = -1,     /* bad argument (null pointer, size, enum value) */
     FGL_E_NOMEM = -2,       /* device or host allocation failed */
     FGL_E_DEVICE = -3,      /* HIP runtime error or no usable gfx950 device */
-    FGL_E_UNSUPPORTED = -4  /* request outside what the device path implements */
+    FGL_E_UNSUPPORTED = -4, /* request outside what the device path implements */
+    FGL_E_IO = -5           /* a file could not be opened, read or written */
 } fgl_status;
 
 /* context.go:11-26 */
@@ -163,8 +164,40 @@ void fgl_mesh_destroy(fgl_mesh *mesh);
 int fgl_draw_mesh(fgl_ctx *ctx, const fgl_state *state, const fgl_shader *shader, const fgl_mesh *mesh,
                   uint64_t info[2]);
 
+/* ---- tile shape and the autotuner (no reference counterpart: the
+ * reference has no tiles; for callers that want the fastest shape for their
+ * workload) ----
+ *
+ * The draw kernels are compiled for several tile shapes; every shape gives
+ * the same images.  A draw uses the context's shape if one is set, else the
+ * tuning table's entry for (width, height, primitive kind, bit length of the
+ * primitive count), whose pair / span / fragment capacities it also reserves
+ * up front, else 32 x 32.
+ *
+ * fgl_tile_shapes: the compiled shapes as (w, h) pairs; *n = their count,
+ *   at most cap written.
+ * fgl_ctx_set_tile: this context's shape (0, 0: the table / default).
+ * fgl_ctx_get_tile: the shape a draw of ntri triangles on ctx would use.
+ * fgl_autotune_mesh: draws mesh reps + 1 times with each shape (the first
+ *   draw per shape sizes the buffers; the fastest of the others counts),
+ *   records the fastest shape and the capacities its draw needed in the
+ *   table, and returns the shape.  The context's framebuffer, pending clears
+ *   and last-draw counters are restored afterwards.
+ * fgl_tune_save / fgl_tune_load: the table as a text file (load merges,
+ *   rejecting a file with any malformed line or unknown shape);
+ *   fgl_tune_clear empties it.  The table is process-wide. */
+int fgl_tile_shapes(int32_t *wh, int32_t cap, int32_t *n);
+int fgl_ctx_set_tile(fgl_ctx *ctx, int32_t tile_w, int32_t tile_h);
+int fgl_ctx_get_tile(fgl_ctx *ctx, int64_t ntri, int32_t out_wh[2]);
+int fgl_autotune_mesh(fgl_ctx *ctx, const fgl_state *state, const fgl_shader *shader, const fgl_mesh *mesh,
+                      int32_t reps, int32_t out_wh[2]);
+int fgl_tune_save(const char *path);
+int fgl_tune_load(const char *path);
+int fgl_tune_clear(void);
+
 /* ---- batched views (no reference counterpart: callers that render many
- * frames of resident meshes per step) ----
+ * views of resident meshes per step, hundreds of views of meshes of uneven
+ * size at mixed resolutions) ----
  *
  * A batch is a list of views: (context, clear color and depth, state,
  * shader, resident mesh) = ClearColorBufferWith + ClearDepthBufferWith +
