@@ -53,6 +53,8 @@ def gpu_frames(mesh, W, H, ssaa, steps, warmup):
     out = A.NewContext(W, H) if s > 1 else None
     big.Shader = M.phong_shader(W, H)
     dm = big.Upload(mesh)
+    tile = big.Autotune(dm, reps=3)  # the tile shape for this workload (same images)
+    log(f"[bench] {W * s}x{H * s}: tile shape {tile[0]}x{tile[1]}")
 
     def frame():
         big.ClearColorBufferWith(M.BACKGROUND)
@@ -89,7 +91,7 @@ def gpu_frames(mesh, W, H, ssaa, steps, warmup):
     stats = big.DrawStats()
     img = (out if out is not None else big).Image()
     dm.close()
-    return ms, info, table, img, stats, dom, live
+    return ms, info, table, img, stats, dom, live, tile
 
 
 def nn_frames(mesh, W, H, steps, warmup):
@@ -134,10 +136,12 @@ def nn_frames(mesh, W, H, steps, warmup):
 
 
 def batch_views(steps, warmup):
-    """Config 4: 32 noise spheres (1k..1M triangles, log-uniform), 8 views
-    each at {256^2, 512^2, 1024^2, 1920x1080}; meshes prepared on the device
-    (BiUnitCube, SmoothNormalsThreshold 30 degrees).  Times one step (all 256
-    views) drawn view by view, and replayed as one recorded HIP graph."""
+    """Batched multi-view: 32 noise spheres (1k..1M triangles, log-uniform),
+    8 views each at {256^2, 512^2, 1024^2, 1920x1080}; meshes prepared on the
+    device (BiUnitCube, SmoothNormalsThreshold 30 degrees); tile shapes from
+    the autotuner (one tuning draw set per distinct resolution and mesh size
+    class, before timing).  Times one step (all 256 views) drawn view by
+    view, and replayed as one recorded HIP graph."""
     import math as _m
     from fauxgl_amd import api as A
     from fauxgl_amd import meshes as M
@@ -156,6 +160,14 @@ def batch_views(steps, warmup):
         c = A.NewContext(w, h)
         c.Shader = M.batch_shader(w, h, eye)
         ctxs.append(c)
+    t = time.time()
+    tuned = {}
+    for (m, eye, w, h), c in zip(views, ctxs):
+        key = (w, h, len(meshes[m].position).bit_length())
+        if key not in tuned:
+            tuned[key] = c.Autotune(dms[m], reps=2)
+    log(f"[bench] autotuned {len(tuned)} (resolution, size class) workloads in {time.time() - t:.1f}s: "
+        f"{sorted(set(tuned.values()))}")
 
     def eager():
         for (m, eye, w, h), c in zip(views, ctxs):
@@ -319,18 +331,18 @@ def main():
 
     if args.only:
         ss = SSAA if args.only == "16x" else 1
-        ms, info, prof, _, stats, dom, live = gpu_frames(mesh, W, H, ss, args.steps, args.warmup)
+        ms, info, prof, _, stats, dom, live, tile = gpu_frames(mesh, W, H, ss, args.steps, args.warmup)
         table = {k: {"launches": v[0], "avg_ms": round(v[1] / max(v[0], 1), 4),
                      "ms_per_frame": round(v[1] / PROFILE_FRAMES, 4)} for k, v in sorted(prof.items())}
-        print(json.dumps({"workload": args.only, "ms_per_frame": round(ms, 4), "info": list(info),
+        print(json.dumps({"workload": args.only, "ms_per_frame": round(ms, 4), "info": list(info), "tile": list(tile),
                           "draw_stats": stats, "kernels": table}), flush=True)
         return
 
-    ms16, info16, prof16, img16, st16, dom, live16 = gpu_frames(mesh, W, H, SSAA, args.steps, args.warmup)
+    ms16, info16, prof16, img16, st16, dom, live16, tile16 = gpu_frames(mesh, W, H, SSAA, args.steps, args.warmup)
     log(f"[bench] 16x SSAA: {ms16:.3f} ms/frame, info {tuple(info16)}")
-    ms1, info1, prof1, img1, st1, _, _ = gpu_frames(mesh, W, H, 1, args.steps, args.warmup)
+    ms1, info1, prof1, img1, st1, _, _, tile1 = gpu_frames(mesh, W, H, 1, args.steps, args.warmup)
     log(f"[bench] 1x: {ms1:.3f} ms/frame, info {tuple(info1)}")
-    ms4, _, _, img4, _, _, _ = gpu_frames(mesh, W, H, 4, args.steps, args.warmup)
+    ms4, _, _, img4, _, _, _, _ = gpu_frames(mesh, W, H, 4, args.steps, args.warmup)
     log(f"[bench] 4x SSAA: {ms4:.3f} ms/frame")
     msn, imgn, profn = nn_frames(mesh, W, H, args.steps, args.warmup)
     log(f"[bench] 1x + neural AA: {msn:.3f} ms/frame")
@@ -408,7 +420,9 @@ def main():
         "config": {"workload": (f"{ntri}-triangle seeded noise-displaced icosphere, Phong, rendered at "
                                 "7680x4320 (4x4 SSAA) and tent-resolved to 1920x1080; mesh resident in HBM"),
                    "triangles": ntri, "width": W, "height": H, "ssaa": SSAA, "shader": "phong",
-                   "path": "16x SSAA (neural AA not in this measurement)"},
+                   "path": "16x SSAA (neural AA not in this measurement)",
+                   "tile": f"{tile16[0]}x{tile16[1]} (autotuned)"},
+        "tile_1x": f"{tile1[0]}x{tile1[1]} (autotuned)",
         "frames_per_s_1x": round(1000.0 / ms1, 3),
         "ms_per_frame_1x": round(ms1, 4),
         "triangles_per_s_16x": round(ntri * fps16, 1),

@@ -348,7 +348,7 @@ BATCH_RESOLUTIONS = [(256, 256), (512, 512), (1024, 1024), (1920, 1080)]
 
 def batch_workload(nmesh=32, views_per_mesh=8, seed=4, nmin=1e3, nmax=1e6, radius=3.5,
                    resolutions=BATCH_RESOLUTIONS):
-    """Config-2-style noise spheres with triangle counts log-uniform in
+    """Noise-displaced icospheres (as the headline mesh) with triangle counts log-uniform in
     [nmin, nmax] (raw: face normals, before BiUnitCube and smoothing), and
     views_per_mesh views of each: eye on a sphere of the given radius at a
     random azimuth and elevation, looking at the origin, resolution drawn

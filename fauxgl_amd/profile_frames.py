@@ -5,7 +5,13 @@ readings calibrate the counters (bytes per counter unit).
     rocprofv3 --pmc FETCH_SIZE --kernel-include-regex 'k_|copy' -- \\
         python3 -m fauxgl_amd.profile_frames --frames 3
 
-Prints the copy size and the draw statistics of the last frame.
+The tile shape is the autotuner's pick for this workload (or --tile WxH);
+the tuning draws run before the frames, under other kernel names
+(fgl::t<W>x<H>::...), so counters of the frames are told apart by the shape
+printed at the end.
+
+Prints the copy size, the tile shape and the draw statistics of the last
+frame.
 """
 import argparse
 import json
@@ -17,6 +23,7 @@ def main():
     ap.add_argument("--frames", type=int, default=3)
     ap.add_argument("--ssaa", type=int, default=16, choices=[1, 16])
     ap.add_argument("--copy-mib", type=int, default=512)
+    ap.add_argument("--tile", default="auto", help="WxH, or auto (the autotuner's pick)")
     args = ap.parse_args()
 
     import torch  # device memory for the calibration copy only
@@ -38,6 +45,11 @@ def main():
     out = A.NewContext(W, H) if s > 1 else None
     big.Shader = M.phong_shader(W, H)
     dm = big.Upload(mesh)
+    if args.tile == "auto":
+        tile = big.Autotune(dm, reps=3)
+    else:
+        tile = tuple(int(v) for v in args.tile.split("x"))
+        big.SetTileShape(*tile)
     info = None
     for _ in range(args.frames):
         big.ClearColorBufferWith(M.BACKGROUND)
@@ -46,7 +58,7 @@ def main():
         if out is not None:
             big.Resolve(out, "tent")
     big.Sync()
-    print(json.dumps({"copy_bytes": copy_bytes, "frames": args.frames, "info": list(info),
+    print(json.dumps({"copy_bytes": copy_bytes, "frames": args.frames, "tile": list(tile), "info": list(info),
                       "draw_stats": big.DrawStats()}), flush=True)
     dm.close()
     return 0
