@@ -105,6 +105,7 @@ def nn_frames(mesh, W, H, steps, warmup):
     out = A.NewContext(W, H)
     lo.Shader = M.phong_shader(W, H)
     dm = lo.Upload(mesh)
+    lo.Autotune(dm, reps=3)
     net = nnaa.Network(lo, weights=nnaa.DEFAULT_WEIGHTS)
 
     def frame():
@@ -317,8 +318,8 @@ def main():
     ap.add_argument("--cpu-frames-1t", type=int, default=3, help="timed single-thread CPU frames at 1x (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = CPUs in this process's affinity mask")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-view line")
-    ap.add_argument("--only", choices=["16x", "1x"], default=None,
-                    help="time one resolution only and print its kernel table (for profiler runs)")
+    ap.add_argument("--only", choices=["16x", "1x", "nn"], default=None,
+                    help="time one leg only (16x, 1x, or 1x + neural AA) and print its kernel table (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -329,6 +330,12 @@ def main():
     mesh = build_scene()
     ntri = len(mesh.Triangles)
 
+    if args.only == "nn":
+        ms, _, prof = nn_frames(mesh, W, H, args.steps, args.warmup)
+        table = {k: {"launches": v[0], "avg_ms": round(v[1] / max(v[0], 1), 4),
+                     "ms_per_frame": round(v[1] / PROFILE_FRAMES, 4)} for k, v in sorted(prof.items())}
+        print(json.dumps({"workload": "1x + neural AA", "ms_per_frame": round(ms, 4), "kernels": table}), flush=True)
+        return
     if args.only:
         ss = SSAA if args.only == "16x" else 1
         ms, info, prof, _, stats, dom, live, tile = gpu_frames(mesh, W, H, ss, args.steps, args.warmup)
