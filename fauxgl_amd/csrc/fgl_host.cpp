@@ -1002,9 +1002,13 @@ int fgl_autotune_mesh(fgl_ctx *ctx, const fgl_state *state, const fgl_shader *sh
             best.tw = kShapes[k]->tw;
             best.th = kShapes[k]->th;
             best.ms = t_min;
-            best.pair_cap = cap_for(ctx->last.pairs_lo, ctx->last.pairs_hi);
-            best.span_cap = cap_for(ctx->last.spans_lo, ctx->last.spans_hi);
-            best.frag_cap = cap_for(ctx->last.frags_lo, ctx->last.frags_hi);
+            // the counters are the last draw's: the triangles' when the mesh
+            // has no lines (a lines draw would overwrite them; then no
+            // capacities are recorded)
+            const bool tris_last = mesh->nline == 0;
+            best.pair_cap = tris_last ? cap_for(ctx->last.pairs_lo, ctx->last.pairs_hi) : 0;
+            best.span_cap = tris_last ? cap_for(ctx->last.spans_lo, ctx->last.spans_hi) : 0;
+            best.frag_cap = tris_last ? cap_for(ctx->last.frags_lo, ctx->last.frags_hi) : 0;
         }
     }
     ctx->tile_w = sw;
