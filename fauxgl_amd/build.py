@@ -18,7 +18,7 @@ SOURCES = ["fgl_vs.hip", "fgl_bin.hip", "fgl_spans.hip", "fgl_tile.hip", "fgl_mi
 # The draw kernels are compiled once per tile shape (the shapes of
 # FGL_TILE_SHAPES in csrc/fgl_internal.h); the other sources once.
 TILE_SOURCES = ["fgl_vs.hip", "fgl_bin.hip", "fgl_spans.hip", "fgl_tile.hip"]
-TILE_SHAPES = [(32, 32), (64, 32), (32, 16), (64, 64)]
+TILE_SHAPES = [(32, 32), (64, 32), (32, 16), (64, 64), (128, 64)]
 
 # -ffp-contract=off: Go evaluates a*b+c as two roundings; the FP64 coverage
 # and depth arithmetic must match it bit for bit.  Packed FP32 ops are

@@ -127,7 +127,7 @@ struct TileOps {
 };
 
 // The compiled shapes (width, height); 32 x 32 is the default.
-#define FGL_TILE_SHAPES(X) X(32, 32) X(64, 32) X(32, 16) X(64, 64)
+#define FGL_TILE_SHAPES(X) X(32, 32) X(64, 32) X(32, 16) X(64, 64) X(128, 64)
 #define FGL_DECLARE_TILE_OPS(w, h) const TileOps &tile_ops_##w##x##h();
 FGL_TILE_SHAPES(FGL_DECLARE_TILE_OPS)
 #undef FGL_DECLARE_TILE_OPS
