@@ -88,7 +88,7 @@ struct fgl_ctx {
     int64_t next_tex = 1;
 
     DevBuf ctr, ntl, nrast, sstart, blocksum, blockoff, rec_main, side_rec, side_vtx;
-    DevBuf keys0, vals0, keys1, vals1, hist, dtot, tstart, tend, tfrag, tupd, deferred;
+    DevBuf keys0, vals0, keys1, vals1, hist, dtot, tstart, tend, tupd, deferred;
     DevBuf pspans, span_bsum, span_boff, poff, spans;
     DevBuf frag_bsum, frag_boff, frag_gsum, foff, frags;
     uint32_t span_cap = 0, frag_cap = 0;
@@ -468,7 +468,6 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
     HIPCHK(c, c->dtot.ensure(256 * 4));
     HIPCHK(c, c->tstart.ensure((size_t)a.ntiles * 4));
     HIPCHK(c, c->tend.ensure((size_t)a.ntiles * 4));
-    HIPCHK(c, c->tfrag.ensure((size_t)a.ntiles * 8));
     HIPCHK(c, c->tupd.ensure((size_t)a.ntiles * 4));
 
     for (int attempt = 0; attempt < 6; attempt++) {
@@ -525,7 +524,6 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
         a.deferred = c->deferred.as<uint32_t>();
         a.tstart = c->tstart.as<uint32_t>();
         a.tend = c->tend.as<uint32_t>();
-        a.tfrag = c->tfrag.as<uint2>();
         a.tupd = c->tupd.as<uint32_t>();
         a.clear_color_pending = c->pend_color;
         a.clear_depth_pending = c->pend_depth;
@@ -551,7 +549,6 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
         if ((r = launch(c, "spans", [&] { return T.spans(a, c->stream); }))) return r;
         if ((r = launch(c, "frag_scan", [&] { return T.frag_scan(a, c->stream); }))) return r;
         if ((r = launch(c, "shade", [&] { return T.shade(a, c->stream); }))) return r;
-        if ((r = launch(c, "tile_frags", [&] { return T.tile_frags(a, c->stream); }))) return r;
         if ((r = launch(c, "rop", [&] { return T.rop(a, c->stream); }))) return r;
         if ((r = launch(c, "updated", [&] { return T.updated(a, c->stream); }))) return r;
         if (capture_ctr)

@@ -71,7 +71,6 @@ struct DrawArgs {
     uint32_t *foff;             // first fragment of each span (span_cap + 1 entries)
     Frag *frags;
     uint32_t frag_cap;
-    uint2 *tfrag;               // per tile: [first, end) of its fragments
     uint32_t *tupd;             // per tile: pixels updated
 
     // framebuffer and deferred clears
@@ -108,7 +107,6 @@ hipError_t launch_span_scan(const DrawArgs &a, hipStream_t s);
 hipError_t launch_spans(const DrawArgs &a, hipStream_t s);
 hipError_t launch_frag_scan(const DrawArgs &a, hipStream_t s);
 hipError_t launch_shade(const DrawArgs &a, hipStream_t s);
-hipError_t launch_tile_frags(const DrawArgs &a, hipStream_t s);
 hipError_t launch_tile(const DrawArgs &a, hipStream_t s);
 hipError_t launch_updated(const DrawArgs &a, hipStream_t s);
 }  // namespace FGL_TILE_NS
@@ -123,7 +121,7 @@ struct TileOps {
     PassLaunch radix_hist;
     DrawLaunch radix_scan;
     PassLaunch radix_scatter;
-    DrawLaunch ranges, span_count, span_scan, spans, frag_scan, shade, tile_frags, rop, updated;
+    DrawLaunch ranges, span_count, span_scan, spans, frag_scan, shade, rop, updated;
 };
 
 // The compiled shapes (width, height); 32 x 32 is the default.

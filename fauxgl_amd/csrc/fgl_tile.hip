@@ -685,20 +685,6 @@ __global__ __launch_bounds__(SPAN_THREADS, SH == 3 ? FGL_SHADE_WAVES : 1) void k
     }
 }
 
-// Each tile's fragment range [foff[poff[tstart]], foff[poff[tend]]).
-__global__ __launch_bounds__(256) void k_tile_frags(DrawArgs a) {
-    if (a.ctr->overflow) return;
-    const int t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= a.ntiles) return;
-    const uint32_t ps = a.tstart[t], pe = a.tend[t];
-    uint2 r = make_uint2(0u, 0u);
-    if (pe > ps) {
-        r.x = a.foff[a.poff[ps]];
-        r.y = a.foff[a.poff[pe]];
-    }
-    a.tfrag[t] = r;
-}
-
 // ---------------------------------------------------------------------------
 // the ordered framebuffer update, one workgroup per tile
 
@@ -718,8 +704,10 @@ __global__ __launch_bounds__(TILE_THREADS, FGL_ROP_WAVES) void k_rop(DrawArgs a)
     const int tx = t % a.ntx, ty = t / a.ntx;
     const int cx0 = tx * TILE_W, cy0 = ty * TILE_H;
     const int cw = min(TILE_W, a.W - cx0), chh = min(TILE_H, a.H - cy0);
-    const uint2 fr01 = a.tfrag[t];
-    const uint32_t f0 = fr01.x, f1 = fr01.y;
+    // the tile's fragment range [foff[poff[tstart]], foff[poff[tend]])
+    const uint32_t ps = a.tstart[t], pe = a.tend[t];
+    const uint32_t f0 = pe > ps ? a.foff[a.poff[ps]] : 0u;
+    const uint32_t f1 = pe > ps ? a.foff[a.poff[pe]] : 0u;
     if (f1 <= f0) {
         if (tid == 0) a.tupd[t] = 0;
         if (a.clear_color_pending || a.clear_depth_pending) {
@@ -881,10 +869,6 @@ hipError_t launch_shade(const DrawArgs &a, hipStream_t s) {
 #undef K_SHADE
     return hipGetLastError();
 }
-hipError_t launch_tile_frags(const DrawArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_tile_frags, dim3((a.ntiles + 255) / 256), dim3(256), 0, s, a);
-    return hipGetLastError();
-}
 hipError_t launch_tile(const DrawArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_rop, dim3(a.ntiles), dim3(TILE_THREADS), 0, s, a);
     return hipGetLastError();
@@ -918,7 +902,6 @@ const TileOps &FGL_OPS1(FGL_TILE_W, FGL_TILE_H)() {
         FGL_TILE_NS::launch_spans,
         FGL_TILE_NS::launch_frag_scan,
         FGL_TILE_NS::launch_shade,
-        FGL_TILE_NS::launch_tile_frags,
         FGL_TILE_NS::launch_tile,
         FGL_TILE_NS::launch_updated,
     };
