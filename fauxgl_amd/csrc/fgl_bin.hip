@@ -262,28 +262,5 @@ hipError_t launch_radix_scatter(const DrawArgs &a, int pass, hipStream_t s) {
     return hipGetLastError();
 }
 
-// ---------------------------------------------------------------------------
-// per-tile ranges in the sorted pairs
-
-__global__ void k_ranges(DrawArgs a) {
-    if (a.ctr->overflow) return;
-    const uint32_t E = a.ctr->npairs;
-    const uint32_t *keys = a.keys[a.sorted_buf];
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < E;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        uint32_t t = keys[i];
-        if (i == 0 || keys[i - 1] != t) a.tstart[t] = (uint32_t)i;
-        if (i + 1 == E || keys[i + 1] != t) a.tend[t] = (uint32_t)(i + 1);
-    }
-}
-
-hipError_t launch_ranges(const DrawArgs &a, hipStream_t s) {
-    uint64_t blocks = ((uint64_t)a.pair_cap + 255) / 256;
-    if (blocks > 2048) blocks = 2048;
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_ranges, dim3((unsigned)blocks), dim3(256), 0, s, a);
-    return hipGetLastError();
-}
-
 }  // namespace FGL_TILE_NS
 }  // namespace fgl

@@ -543,7 +543,6 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
             if ((r = launch(c, "radix_scan", [&] { return T.radix_scan(a, c->stream); }))) return r;
             if ((r = launch(c, "radix_scatter", [&] { return T.radix_scatter(a, p, c->stream); }))) return r;
         }
-        if ((r = launch(c, "ranges", [&] { return T.ranges(a, c->stream); }))) return r;
         if ((r = launch(c, "span_count", [&] { return T.span_count(a, c->stream); }))) return r;
         if ((r = launch(c, "span_scan", [&] { return T.span_scan(a, c->stream); }))) return r;
         if ((r = launch(c, "spans", [&] { return T.spans(a, c->stream); }))) return r;

@@ -101,7 +101,6 @@ hipError_t launch_emit(const DrawArgs &a, hipStream_t s);
 hipError_t launch_radix_hist(const DrawArgs &a, int pass, hipStream_t s);
 hipError_t launch_radix_scan(const DrawArgs &a, hipStream_t s);
 hipError_t launch_radix_scatter(const DrawArgs &a, int pass, hipStream_t s);
-hipError_t launch_ranges(const DrawArgs &a, hipStream_t s);
 hipError_t launch_span_count(const DrawArgs &a, hipStream_t s);
 hipError_t launch_span_scan(const DrawArgs &a, hipStream_t s);
 hipError_t launch_spans(const DrawArgs &a, hipStream_t s);
@@ -121,7 +120,7 @@ struct TileOps {
     PassLaunch radix_hist;
     DrawLaunch radix_scan;
     PassLaunch radix_scatter;
-    DrawLaunch ranges, span_count, span_scan, spans, frag_scan, shade, rop, updated;
+    DrawLaunch span_count, span_scan, spans, frag_scan, shade, rop, updated;
 };
 
 // The compiled shapes (width, height); 32 x 32 is the default.

@@ -36,7 +36,8 @@ FGL_D uint32_t pair_spans(const DrawArgs &a, const uint32_t *keys, const uint32_
     return n > 0xffffffull ? 0xffffffu : (uint32_t)n;
 }
 
-// Spans per pair, summed per block of SPAN_THREADS pairs.
+// Spans per pair, summed per block of SPAN_THREADS pairs; also each tile's
+// range in the sorted pairs.
 __global__ __launch_bounds__(SPAN_THREADS) void k_span_count(DrawArgs a) {
     __shared__ uint64_t wsum[SPAN_THREADS / 64];
     if (a.ctr->overflow) return;
@@ -45,7 +46,13 @@ __global__ __launch_bounds__(SPAN_THREADS) void k_span_count(DrawArgs a) {
     const uint32_t *vals = a.vals[a.sorted_buf];
     uint32_t p = blockIdx.x * SPAN_THREADS + threadIdx.x;
     uint32_t n = p < E ? pair_spans(a, keys, vals, p) : 0u;
-    if (p < E) a.pspans[p] = n;
+    if (p < E) {
+        a.pspans[p] = n;
+        // the tile's [start, end) in the sorted pairs
+        const uint32_t t = keys[p];
+        if (p == 0 || keys[p - 1] != t) a.tstart[t] = p;
+        if (p + 1 == E || keys[p + 1] != t) a.tend[t] = p + 1;
+    }
     uint64_t v = n;
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;

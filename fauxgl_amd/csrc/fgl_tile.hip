@@ -896,7 +896,6 @@ const TileOps &FGL_OPS1(FGL_TILE_W, FGL_TILE_H)() {
         FGL_TILE_NS::launch_radix_hist,
         FGL_TILE_NS::launch_radix_scan,
         FGL_TILE_NS::launch_radix_scatter,
-        FGL_TILE_NS::launch_ranges,
         FGL_TILE_NS::launch_span_count,
         FGL_TILE_NS::launch_span_scan,
         FGL_TILE_NS::launch_spans,
