@@ -104,7 +104,6 @@ struct DevState {
 // Per-draw counters and flags kept in device memory.
 struct DrawCounters {
     unsigned long long total;    // TotalPixels
-    unsigned long long updated;  // UpdatedPixels
     uint32_t side_rec;           // side raster records allocated
     uint32_t side_vtx;           // side vertices allocated
     uint32_t ndeferred;          // input triangles deferred to the clipping kernel
@@ -116,7 +115,11 @@ struct DrawCounters {
     uint32_t nfrags;             // fragments (clamped to capacity)
     uint32_t frags_lo, frags_hi; // unclamped fragment count
     uint32_t pad2;
+    // UpdatedPixels in UPD_PARTS partial sums (tile t adds into t % UPD_PARTS,
+    // spreading the atomics over as many addresses); the host adds them up
+    uint32_t upd[64];
 };
+constexpr int UPD_PARTS = 64;
 
 // Coverage span: the covered pixels of one row of one raster triangle in
 // one tile, with the edge values the reference holds at the first of them.

@@ -88,7 +88,7 @@ struct fgl_ctx {
     int64_t next_tex = 1;
 
     DevBuf ctr, ntl, nrast, sstart, blocksum, blockoff, rec_main, side_rec, side_vtx;
-    DevBuf keys0, vals0, keys1, vals1, hist, dtot, tstart, tend, tupd, deferred;
+    DevBuf keys0, vals0, keys1, vals1, hist, dtot, tstart, tend, deferred;
     DevBuf pspans, span_bsum, span_boff, poff, spans;
     DevBuf frag_bsum, frag_boff, frag_gsum, foff, frags;
     uint32_t span_cap = 0, frag_cap = 0;
@@ -388,6 +388,13 @@ const TileOps *draw_ops(fgl_ctx *c, int kind, int64_t n) {
     return t ? t : kShapes[0];
 }
 
+// UpdatedPixels of a draw: the sum of the framebuffer update's partials
+uint64_t updated_pixels(const DrawCounters &h) {
+    uint64_t s = 0;
+    for (int k = 0; k < UPD_PARTS; k++) s += h.upd[k];
+    return s;
+}
+
 uint32_t cap_for(uint32_t lo, uint32_t hi) {
     const uint64_t v = ((uint64_t)hi << 32 | lo) * 5 / 4 + 65536;
     return (uint32_t)std::min<uint64_t>(v, 0xfffff000ull);
@@ -468,7 +475,6 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
     HIPCHK(c, c->dtot.ensure(256 * 4));
     HIPCHK(c, c->tstart.ensure((size_t)a.ntiles * 4));
     HIPCHK(c, c->tend.ensure((size_t)a.ntiles * 4));
-    HIPCHK(c, c->tupd.ensure((size_t)a.ntiles * 4));
 
     for (int attempt = 0; attempt < 6; attempt++) {
         HIPCHK(c, c->side_rec.ensure((size_t)c->side_rec_cap * sizeof(PrimRec)));
@@ -524,7 +530,6 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
         a.deferred = c->deferred.as<uint32_t>();
         a.tstart = c->tstart.as<uint32_t>();
         a.tend = c->tend.as<uint32_t>();
-        a.tupd = c->tupd.as<uint32_t>();
         a.clear_color_pending = c->pend_color;
         a.clear_depth_pending = c->pend_depth;
         a.clear_color = c->clear_color;
@@ -549,7 +554,6 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
         if ((r = launch(c, "frag_scan", [&] { return T.frag_scan(a, c->stream); }))) return r;
         if ((r = launch(c, "shade", [&] { return T.shade(a, c->stream); }))) return r;
         if ((r = launch(c, "rop", [&] { return T.rop(a, c->stream); }))) return r;
-        if ((r = launch(c, "updated", [&] { return T.updated(a, c->stream); }))) return r;
         if (capture_ctr)
             return launch(c, "counters", [&] {
                 return launch_copy_words(reinterpret_cast<const uint32_t *>(a.ctr), capture_ctr,
@@ -564,7 +568,7 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
         if (h.overflow == 0) {
             if (info) {
                 info[0] = h.total;
-                info[1] = h.updated;
+                info[1] = updated_pixels(h);
             }
             c->pend_color = false;
             c->pend_depth = false;
@@ -1312,7 +1316,7 @@ int fgl_draw_stats(fgl_ctx *ctx, fgl_draw_stat *out) {
     if (check_ctx(ctx) || !out) return FGL_E_INVALID;
     const DrawCounters &h = ctx->last;
     out->total_pixels = h.total;
-    out->updated_pixels = h.updated;
+    out->updated_pixels = updated_pixels(h);
     out->clipped_triangles = h.ndeferred;
     out->side_triangles = h.side_rec;
     out->pairs = (uint64_t)h.pairs_hi << 32 | h.pairs_lo;
@@ -2521,7 +2525,7 @@ int fgl_batch_run(fgl_batch *b, uint64_t *info) {
             continue;
         }
         out[2 * i] = (has_t ? t.total : 0) + (has_l ? l.total : 0);
-        out[2 * i + 1] = (has_t ? t.updated : 0) + (has_l ? l.updated : 0);
+        out[2 * i + 1] = (has_t ? updated_pixels(t) : 0) + (has_l ? updated_pixels(l) : 0);
         b->views[i].ctx->last = has_l ? l : t;
     }
     if (regrow) {

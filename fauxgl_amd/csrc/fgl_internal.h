@@ -71,7 +71,6 @@ struct DrawArgs {
     uint32_t *foff;             // first fragment of each span (span_cap + 1 entries)
     Frag *frags;
     uint32_t frag_cap;
-    uint32_t *tupd;             // per tile: pixels updated
 
     // framebuffer and deferred clears
     uint32_t *color;       // NRGBA8 as little-endian u32
@@ -107,7 +106,6 @@ hipError_t launch_spans(const DrawArgs &a, hipStream_t s);
 hipError_t launch_frag_scan(const DrawArgs &a, hipStream_t s);
 hipError_t launch_shade(const DrawArgs &a, hipStream_t s);
 hipError_t launch_tile(const DrawArgs &a, hipStream_t s);
-hipError_t launch_updated(const DrawArgs &a, hipStream_t s);
 }  // namespace FGL_TILE_NS
 
 using DrawLaunch = hipError_t (*)(const DrawArgs &, hipStream_t);
@@ -120,7 +118,7 @@ struct TileOps {
     PassLaunch radix_hist;
     DrawLaunch radix_scan;
     PassLaunch radix_scatter;
-    DrawLaunch span_count, span_scan, spans, frag_scan, shade, rop, updated;
+    DrawLaunch span_count, span_scan, spans, frag_scan, shade, rop;
 };
 
 // The compiled shapes (width, height); 32 x 32 is the default.

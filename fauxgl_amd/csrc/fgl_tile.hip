@@ -709,7 +709,6 @@ __global__ __launch_bounds__(TILE_THREADS, FGL_ROP_WAVES) void k_rop(DrawArgs a)
     const uint32_t f0 = pe > ps ? a.foff[a.poff[ps]] : 0u;
     const uint32_t f1 = pe > ps ? a.foff[a.poff[pe]] : 0u;
     if (f1 <= f0) {
-        if (tid == 0) a.tupd[t] = 0;
         if (a.clear_color_pending || a.clear_depth_pending) {
             for (int p = tid; p < TILE_PIX; p += TILE_THREADS) {
                 int lx = p % TILE_W, ly = p / TILE_W;
@@ -800,7 +799,7 @@ __global__ __launch_bounds__(TILE_THREADS, FGL_ROP_WAVES) void k_rop(DrawArgs a)
             a.depth[gi] = dep[p];
         }
     }
-    // UpdatedPixels of this tile (summed by k_updated)
+    // UpdatedPixels of this tile, into one of the partial sums
     uint32_t uu = my_updated;
     for (int off = 32; off > 0; off >>= 1) uu += __shfl_down(uu, off);
     if (lane == 0) upd[tid >> 6] = uu;
@@ -808,39 +807,10 @@ __global__ __launch_bounds__(TILE_THREADS, FGL_ROP_WAVES) void k_rop(DrawArgs a)
     if (tid == 0) {
         uint32_t s = 0;
         for (int w = 0; w < TILE_THREADS / 64; w++) s += upd[w];
-        a.tupd[t] = s;
+        if (s) atomicAdd(&a.ctr->upd[t % UPD_PARTS], s);
     }
 }
 
-// UpdatedPixels: the sum of the per-tile counts (one workgroup; a
-// device-wide atomic per tile would serialise on one address).
-__global__ __launch_bounds__(1024) void k_updated(DrawArgs a) {
-    __shared__ unsigned long long wsum[1024 / 64];
-    if (a.ctr->overflow) return;
-    // 16 independent loads per thread in flight (clamped addresses, the
-    // copies past the end masked), not one dependent load per pass
-    constexpr int ITEMS = 16;
-    const int nt = a.ntiles;
-    unsigned long long v = 0;
-    for (int t0 = 0; t0 < nt; t0 += 1024 * ITEMS) {
-        uint32_t x[ITEMS];
-#pragma unroll
-        for (int q = 0; q < ITEMS; q++) {
-            const int t = t0 + q * 1024 + (int)threadIdx.x;
-            x[q] = a.tupd[t < nt ? t : nt - 1];
-        }
-#pragma unroll
-        for (int q = 0; q < ITEMS; q++) v += t0 + q * 1024 + (int)threadIdx.x < nt ? x[q] : 0u;
-    }
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
-    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long s = 0;
-        for (int w = 0; w < 1024 / 64; w++) s += wsum[w];
-        a.ctr->updated = s;
-    }
-}
 
 // ---------------------------------------------------------------------------
 
@@ -873,10 +843,6 @@ hipError_t launch_tile(const DrawArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_rop, dim3(a.ntiles), dim3(TILE_THREADS), 0, s, a);
     return hipGetLastError();
 }
-hipError_t launch_updated(const DrawArgs &a, hipStream_t s) {
-    hipLaunchKernelGGL(k_updated, dim3(1), dim3(1024), 0, s, a);
-    return hipGetLastError();
-}
 
 }  // namespace FGL_TILE_NS
 
@@ -902,7 +868,6 @@ const TileOps &FGL_OPS1(FGL_TILE_W, FGL_TILE_H)() {
         FGL_TILE_NS::launch_frag_scan,
         FGL_TILE_NS::launch_shade,
         FGL_TILE_NS::launch_tile,
-        FGL_TILE_NS::launch_updated,
     };
     return ops;
 }
