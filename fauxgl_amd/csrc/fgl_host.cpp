@@ -497,7 +497,7 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
         a.poff = c->poff.as<uint32_t>();
         a.spans = c->spans.as<Span>();
         a.span_cap = c->span_cap;
-        const size_t nfb = ((size_t)c->span_cap + SPAN_THREADS - 1) / SPAN_THREADS + 1;
+        const size_t nfb = ((size_t)c->span_cap + FRAG_BLOCK - 1) / FRAG_BLOCK + 1;
         HIPCHK(c, c->frag_bsum.ensure(nfb * 8));
         HIPCHK(c, c->frag_boff.ensure(nfb * 8));
         HIPCHK(c, c->frag_gsum.ensure((nfb / 1024 + 2) * 8));

@@ -84,6 +84,9 @@ struct DrawArgs {
 
 constexpr int VS_THREADS = 256;
 constexpr int SPAN_THREADS = 256;
+// Spans per fragment-count block: the shading kernel's waves each take one
+// such block (one span per lane) and need only its fragment offset.
+constexpr int FRAG_BLOCK = 64;
 constexpr int RS_THREADS = 256;
 constexpr int RS_ITEMS = 16;
 constexpr int RS_BLOCK = RS_THREADS * RS_ITEMS;

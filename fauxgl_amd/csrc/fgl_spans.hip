@@ -74,7 +74,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_span_scan(DrawArgs a) {
     const uint64_t total = scan_block_sums(a.span_blocksum, a.span_blockoff, nb, lds);
     // k_spans adds each span block's fragment count into frag_blocksum
     const uint64_t ns = total > a.span_cap ? a.span_cap : total;
-    const uint32_t nfb = (uint32_t)((ns + SPAN_THREADS - 1) / SPAN_THREADS);
+    const uint32_t nfb = (uint32_t)((ns + FRAG_BLOCK - 1) / FRAG_BLOCK);
     for (uint32_t k = threadIdx.x; k < nfb; k += SCAN_THREADS) a.frag_blocksum[k] = 0;
     if (threadIdx.x == SCAN_THREADS - 1) {
         if (total > a.span_cap) atomicOr(&a.ctr->overflow, 16u);
@@ -90,10 +90,10 @@ FGL_D double edge(double ax, double ay, double bx, double by, double cx, double 
     return (bx - cx) * (ay - cy) - (by - cy) * (ax - cx);
 }
 
-// Fragment counts per SPAN_THREADS-span block: collected in LDS for the
+// Fragment counts per FRAG_BLOCK-span block: collected in LDS for the
 // FB_SLOTS blocks from the workgroup's first span on, in global memory
 // beyond that.
-constexpr uint32_t FB_SLOTS = 64;
+constexpr uint32_t FB_SLOTS = 128;
 
 FGL_D void add_frags(const DrawArgs &a, uint32_t *lfb, uint32_t blk0, uint32_t blk, uint64_t cnt) {
     if (!cnt) return;
@@ -249,7 +249,7 @@ __global__ __launch_bounds__(SPAN_THREADS) void k_spans(DrawArgs a) {
     const uint32_t *vals = a.vals[a.sorted_buf];
     const uint32_t p = blockIdx.x * SPAN_THREADS + threadIdx.x;
     const uint32_t n = p < E ? a.pspans[p] : 0u;
-    if (threadIdx.x < FB_SLOTS) lfb[threadIdx.x] = 0;
+    for (uint32_t k = threadIdx.x; k < FB_SLOTS; k += SPAN_THREADS) lfb[k] = 0;
     // block-exclusive scan of n
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint64_t v = n;
@@ -260,7 +260,7 @@ __global__ __launch_bounds__(SPAN_THREADS) void k_spans(DrawArgs a) {
     if (lane == 63) wsum[wave] = v;
     __syncthreads();
     const uint64_t bbase = a.span_blockoff[blockIdx.x];
-    const uint32_t blk0 = (uint32_t)(bbase / SPAN_THREADS);
+    const uint32_t blk0 = (uint32_t)(bbase / FRAG_BLOCK);
     uint64_t wbase = bbase;
     for (int w = 0; w < wave; w++) wbase += wsum[w];
     // this lane's pair's spans: [wbase + excl, wbase + excl + n)
@@ -287,14 +287,14 @@ __global__ __launch_bounds__(SPAN_THREADS) void k_spans(DrawArgs a) {
             const Span sp = pair_span(setup[wave][ow], q - oexcl, a.W, cnt);
             out[q] = sp;
             if (cnt) {
-                const uint32_t blk = (uint32_t)((wbase + q) / SPAN_THREADS);
+                const uint32_t blk = (uint32_t)((wbase + q) / FRAG_BLOCK);
                 add_frags(a, lfb, blk0, blk, cnt);
             }
         }
     }
     __syncthreads();
-    if (threadIdx.x < FB_SLOTS && lfb[threadIdx.x])
-        atomicAdd((unsigned long long *)&a.frag_blocksum[blk0 + threadIdx.x], (unsigned long long)lfb[threadIdx.x]);
+    for (uint32_t k = threadIdx.x; k < FB_SLOTS; k += SPAN_THREADS)
+        if (lfb[k]) atomicAdd((unsigned long long *)&a.frag_blocksum[blk0 + k], (unsigned long long)lfb[k]);
 }
 
 static unsigned span_blocks(const DrawArgs &a) {

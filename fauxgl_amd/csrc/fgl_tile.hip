@@ -360,7 +360,7 @@ FGL_D const PrimRec *prim_of(const DrawArgs &a, uint32_t ref) {
 // fragment offsets: exclusive scan of the per-block fragment counts k_spans
 // accumulated
 
-// Blocks of the span-indexed kernels loop over the SPAN_THREADS-span blocks
+// Workgroups of the shading kernel loop over the FRAG_BLOCK-span blocks
 // that exist, so the grid (sized from capacity) need not match the count.
 constexpr uint32_t SPAN_GRID_MAX = 2048;
 
@@ -369,14 +369,14 @@ constexpr uint32_t SPAN_GRID_MAX = 2048;
 // shading kernel adds the totals of the groups before its block.
 constexpr int FRAG_GROUP = 1024;
 // Group offsets the shading kernel keeps in LDS (groups beyond, more than
-// FRAG_GTAB * FRAG_GROUP * SPAN_THREADS spans, add their sums one by one).
+// FRAG_GTAB * FRAG_GROUP * FRAG_BLOCK spans, add their sums one by one).
 constexpr int FRAG_GTAB = 256;
 
 __global__ __launch_bounds__(FRAG_GROUP) void k_frag_scan(DrawArgs a) {
     __shared__ uint64_t wsum[FRAG_GROUP / 64];
     if (a.ctr->overflow) return;
     const uint32_t NS = a.ctr->nspans;
-    const uint32_t nb = (NS + SPAN_THREADS - 1) / SPAN_THREADS;
+    const uint32_t nb = (NS + FRAG_BLOCK - 1) / FRAG_BLOCK;
     const uint32_t g = blockIdx.x;
     if ((uint64_t)g * FRAG_GROUP >= nb) return;
     const uint32_t b = g * FRAG_GROUP + threadIdx.x;
@@ -537,14 +537,13 @@ FGL_D void load_extra(const DevShader &sh, const AttrSrc &src, uint64_t attr, Sh
 template <int SH>
 __global__ __launch_bounds__(SPAN_THREADS, SH == 3 ? FGL_SHADE_WAVES : 1) void k_shade(DrawArgs a) {
     using L = SlotLayout<SH>;
-    __shared__ uint32_t wsum[SPAN_THREADS / 64];
     __shared__ uint32_t ostart[SPAN_THREADS / 64][64];
     __shared__ double slots[SPAN_THREADS / 64][SHADE_SLOTS][L::N];
     __shared__ uint64_t gpre[FRAG_GTAB];  // exclusive prefix of the group sums
     __shared__ uint64_t gw[SPAN_THREADS / 64];
     if (a.ctr->overflow) return;
     const uint32_t NS = a.ctr->nspans;
-    const uint32_t nb = (NS + SPAN_THREADS - 1) / SPAN_THREADS;
+    const uint32_t nb = (NS + FRAG_BLOCK - 1) / FRAG_BLOCK;
     const uint32_t ngroups = (nb + FRAG_GROUP - 1) / FRAG_GROUP;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     // fragment total and the groups' offsets from the group sums, loaded in
@@ -579,8 +578,12 @@ __global__ __launch_bounds__(SPAN_THREADS, SH == 3 ? FGL_SHADE_WAVES : 1) void k
         a.ctr->total = total;
     }
     if (total > a.frag_cap) return;  // the host grows the fragment buffer and reruns
-    for (uint32_t blk = blockIdx.x; blk < nb; blk += gridDim.x) {
-        const uint32_t i = blk * SPAN_THREADS + threadIdx.x;
+    // each wave takes its own FRAG_BLOCK-span blocks: no workgroup barrier
+    // in the loop, so a wave with few fragments does not wait for one with
+    // many
+    constexpr uint32_t WAVES = SPAN_THREADS / 64;
+    for (uint32_t blk = blockIdx.x * WAVES + (uint32_t)wave; blk < nb; blk += gridDim.x * WAVES) {
+        const uint32_t i = blk * FRAG_BLOCK + (uint32_t)lane;
         const Span sp = i < NS ? a.spans[i] : Span{{0, 0, 0}, 0, 0, 0};
         const uint32_t n = sp.n & SPAN_N;
         uint32_t incl = n;  // at most 64 * TILE_W
@@ -588,8 +591,6 @@ __global__ __launch_bounds__(SPAN_THREADS, SH == 3 ? FGL_SHADE_WAVES : 1) void k
             const uint32_t y = __shfl_up(incl, off);
             if (lane >= off) incl += y;
         }
-        if (lane == 63) wsum[wave] = incl;
-        __syncthreads();
         const uint32_t grp = blk / FRAG_GROUP;
         uint64_t fo = a.frag_blockoff[blk];
         if (grp < (uint32_t)FRAG_GTAB) {
@@ -598,8 +599,6 @@ __global__ __launch_bounds__(SPAN_THREADS, SH == 3 ? FGL_SHADE_WAVES : 1) void k
             fo += gpre[FRAG_GTAB - 1] + a.frag_gsum[FRAG_GTAB - 1];
             for (uint32_t g = FRAG_GTAB; g < grp; g++) fo += a.frag_gsum[g];
         }
-        for (int w = 0; w < wave; w++) fo += wsum[w];
-        __syncthreads();
         const uint32_t excl = incl - n;
         if (i < NS) a.foff[i] = (uint32_t)(fo + excl);
         const uint32_t T = __shfl(incl, 63);
@@ -815,12 +814,12 @@ __global__ __launch_bounds__(TILE_THREADS, FGL_ROP_WAVES) void k_rop(DrawArgs a)
 // ---------------------------------------------------------------------------
 
 static unsigned span_grid(const DrawArgs &a) {
-    const uint64_t nb = ((uint64_t)a.span_cap + SPAN_THREADS - 1) / SPAN_THREADS;
+    const uint64_t nb = ((uint64_t)a.span_cap + SPAN_THREADS - 1) / SPAN_THREADS;  // workgroups of 4 waves
     return (unsigned)(nb < SPAN_GRID_MAX ? (nb ? nb : 1) : SPAN_GRID_MAX);
 }
 
 hipError_t launch_frag_scan(const DrawArgs &a, hipStream_t s) {
-    const uint64_t nb = ((uint64_t)a.span_cap + SPAN_THREADS - 1) / SPAN_THREADS;
+    const uint64_t nb = ((uint64_t)a.span_cap + FRAG_BLOCK - 1) / FRAG_BLOCK;
     const unsigned groups = (unsigned)((nb + FRAG_GROUP - 1) / FRAG_GROUP);
     hipLaunchKernelGGL(k_frag_scan, dim3(groups ? groups : 1), dim3(FRAG_GROUP), 0, s, a);
     return hipGetLastError();
