@@ -96,6 +96,7 @@ struct fgl_ctx {
     DevBuf zeros;
     DevBuf rs_tmp, rs_wx, rs_ox, rs_wy, rs_oy;
     int32_t rs_key[4] = {0, 0, 0, 0};  // source and destination sizes the tent tables are for
+    ResolveTent4 rs_t4{};              // the tables as packed uniform 4x weights, when they are
     hipEvent_t xev = nullptr;          // orders another context's stream after this one
     DevBuf dpart, dimg;
     uint32_t side_rec_cap = 0, side_vtx_cap = 0, pair_cap = 0;
@@ -1216,6 +1217,33 @@ static void tent_weights(int32_t src, int32_t dst, std::vector<int16_t> &w, std:
     }
 }
 
+// The packed form of one axis's tent table when the factor is 4 and every
+// output has the same eight weights in 0..255, a power-of-two sum and offset
+// 4 * i - 2.  pre_zeros zero weights come before the eight in the packing.
+static bool tent4_pack(const std::vector<int16_t> &w, const std::vector<int32_t> &off, int32_t fl, int pre_zeros,
+                       uint32_t *packed, int npacked, int32_t *shift) {
+    if (fl != 8 || off.empty()) return false;
+    int32_t sum = 0;
+    for (int k = 0; k < 8; k++) {
+        if (w[k] < 0 || w[k] > 255) return false;
+        sum += w[k];
+    }
+    if (sum <= 0 || (sum & (sum - 1)) != 0) return false;
+    for (size_t i = 0; i < off.size(); i++) {
+        if (off[i] != 4 * (int32_t)i - 2) return false;
+        for (int k = 0; k < 8; k++)
+            if (w[i * 8 + k] != w[k]) return false;
+    }
+    for (int d = 0; d < npacked; d++) packed[d] = 0;
+    for (int k = 0; k < 8; k++) {
+        const int b = pre_zeros + k;
+        if (b / 4 >= npacked) return false;
+        packed[b / 4] |= (uint32_t)w[k] << (8 * (b % 4));
+    }
+    *shift = __builtin_ctz((uint32_t)sum);
+    return true;
+}
+
 int fgl_resolve(fgl_ctx *src, fgl_ctx *dst, int32_t filter) {
     if (check_ctx(src) || check_ctx(dst)) return FGL_E_INVALID;
     if (src->device != dst->device) {
@@ -1257,6 +1285,9 @@ int fgl_resolve(fgl_ctx *src, fgl_ctx *dst, int32_t filter) {
             // the host vectors go out of scope: finish the copies first
             HIPCHK(src, hipStreamSynchronize(src->stream));
             std::memcpy(src->rs_key, key, sizeof key);
+            ResolveTent4 t4{};
+            t4.ok = tent4_pack(wx, ox, flx, 2, t4.wx, 3, &t4.shx) && tent4_pack(wy, oy, fly, 0, t4.wy, 2, &t4.shy);
+            src->rs_t4 = t4;
         }
         HIPCHK(src, src->rs_tmp.ensure((size_t)dst->W * src->H * 4));
     }
@@ -1264,7 +1295,7 @@ int fgl_resolve(fgl_ctx *src, fgl_ctx *dst, int32_t filter) {
         return launch_resolve(src->color.as<uint32_t>(), src->W, src->H, src->rs_tmp.as<uint32_t>(),
                               dst->color.as<uint32_t>(), dst->W, dst->H, filter, src->rs_wx.as<int16_t>(),
                               src->rs_ox.as<int32_t>(), flx, src->rs_wy.as<int16_t>(), src->rs_oy.as<int32_t>(), fly,
-                              src->stream);
+                              src->rs_t4, src->stream);
     });
     if (r) return r;
     // later work on dst (reads, draws) runs after the resolve

@@ -133,9 +133,20 @@ FGL_TILE_SHAPES(FGL_DECLARE_TILE_OPS)
 hipError_t launch_copy_words(const uint32_t *src, uint32_t *dst, int32_t n, hipStream_t s);
 hipError_t launch_fill_color(uint32_t *color, int64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_fill_depth(double *depth, int64_t n, double v, hipStream_t s);
+// The 4x tent when every output row of both weight tables is the same eight
+// 8-bit weights with a power-of-two sum and offsets 4*i - 2 (the host checks
+// the tables): weights packed four to a dword for v_dot4_u32_u8.  wx holds
+// twelve taps from a 16-byte-aligned staged column (two zero weights each
+// side), wy eight.
+struct ResolveTent4 {
+    int32_t ok;
+    uint32_t wx[3], wy[2];
+    int32_t shx, shy;  // log2 of each table's weight sum
+};
+
 hipError_t launch_resolve(const uint32_t *src, int32_t sw, int32_t sh, uint32_t *tmp, uint32_t *dst, int32_t dw,
                           int32_t dh, int32_t filter, const int16_t *wx, const int32_t *ox, int32_t flx,
-                          const int16_t *wy, const int32_t *oy, int32_t fly, hipStream_t s);
+                          const int16_t *wy, const int32_t *oy, int32_t fly, const ResolveTent4 &t4, hipStream_t s);
 hipError_t launch_depth_minmax(const double *depth, int64_t n, double *partial, int32_t nblocks, hipStream_t s);
 hipError_t launch_depth_image(const double *depth, int64_t n, const double *partial, int32_t nblocks,
                               uint16_t *out, hipStream_t s);

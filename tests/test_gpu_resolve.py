@@ -44,12 +44,28 @@ def random_nrgba(h, w, seed, opaque=False):
 
 
 @pytest.mark.parametrize("filt", sorted(FILTERS))
-@pytest.mark.parametrize("f,dw,dh", [(1, 9, 4), (2, 17, 11), (3, 37, 23), (4, 64, 36), (4, 101, 7), (5, 13, 13)])
+@pytest.mark.parametrize("f,dw,dh", [(1, 9, 4), (2, 17, 11), (3, 37, 23), (4, 64, 36), (4, 101, 7), (4, 1, 1),
+                                     (4, 16, 15), (4, 33, 31), (5, 13, 13)])
 def test_resolve_random(filt, f, dw, dh):
     _say(f"resolve {filt} x{f} {dw}x{dh}")
     img = random_nrgba(dh * f, dw * f, seed=100 + f * 7 + dw)
     got = resolve_gpu(img, dw, dh, filt)
     ref = O.resolve(img, dw, dh, FILTERS[filt])
+    assert np.array_equal(got, ref), np.argwhere(got != ref)[:5]
+
+
+@pytest.mark.parametrize("dw,dh", [(250, 47), (17, 46)])
+def test_resolve_tent_4x_opaque_and_mixed_blocks(dw, dh):
+    """4x tent where most 4-pixel groups are opaque (premultiply skipped) and a
+    few scattered pixels are not, so one workgroup mixes both."""
+    _say(f"resolve tent x4 {dw}x{dh} mostly opaque")
+    img = random_nrgba(dh * 4, dw * 4, seed=7 + dw, opaque=True)
+    rng = np.random.Generator(np.random.PCG64(dw * 31 + dh))
+    ys = rng.integers(0, dh * 4, 40)
+    xs = rng.integers(0, dw * 4, 40)
+    img[ys, xs, 3] = rng.integers(0, 255, 40, dtype=np.uint8)
+    got = resolve_gpu(img, dw, dh, "tent")
+    ref = O.resolve(img, dw, dh, 1)
     assert np.array_equal(got, ref), np.argwhere(got != ref)[:5]
 
 
