@@ -37,10 +37,14 @@ FGL_D uint32_t pair_spans(const DrawArgs &a, const uint32_t *keys, const uint32_
 }
 
 // Spans per pair, summed per block of SPAN_THREADS pairs; also each tile's
-// range in the sorted pairs.
+// range in the sorted pairs, and the zeroed fragment counts per span block
+// (the whole capacity, grid-wide) that k_spans accumulates into.
 __global__ __launch_bounds__(SPAN_THREADS) void k_span_count(DrawArgs a) {
     __shared__ uint64_t wsum[SPAN_THREADS / 64];
     if (a.ctr->overflow) return;
+    const uint64_t nfb = ((uint64_t)a.span_cap + FRAG_BLOCK - 1) / FRAG_BLOCK;
+    for (uint64_t k = (uint64_t)blockIdx.x * SPAN_THREADS + threadIdx.x; k < nfb; k += (uint64_t)gridDim.x * SPAN_THREADS)
+        a.frag_blocksum[k] = 0;
     const uint32_t E = a.ctr->npairs;
     const uint32_t *keys = a.keys[a.sorted_buf];
     const uint32_t *vals = a.vals[a.sorted_buf];
@@ -72,10 +76,6 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_span_scan(DrawArgs a) {
     const uint32_t E = a.ctr->npairs;
     const int nb = (int)((E + SPAN_THREADS - 1) / SPAN_THREADS);
     const uint64_t total = scan_block_sums(a.span_blocksum, a.span_blockoff, nb, lds);
-    // k_spans adds each span block's fragment count into frag_blocksum
-    const uint64_t ns = total > a.span_cap ? a.span_cap : total;
-    const uint32_t nfb = (uint32_t)((ns + FRAG_BLOCK - 1) / FRAG_BLOCK);
-    for (uint32_t k = threadIdx.x; k < nfb; k += SCAN_THREADS) a.frag_blocksum[k] = 0;
     if (threadIdx.x == SCAN_THREADS - 1) {
         if (total > a.span_cap) atomicOr(&a.ctr->overflow, 16u);
         a.ctr->nspans = total > a.span_cap ? a.span_cap : (uint32_t)total;
