@@ -352,6 +352,7 @@ constexpr int T4_QW = T4_OW + 2;               // 18 dwords (72 columns) per sta
 constexpr int T4_QS = 20;                      // plane row stride in dwords (4 rows apart: half the banks)
 constexpr int T4_QUADS = T4_ROWS * T4_QW;      // 16-byte loads per workgroup
 constexpr int T4_LOADS = (T4_QUADS + 255) / 256;
+static_assert(T4_QS >= T4_QW, "a plane row holds the staged columns");
 static_assert(T4_OW * (T4_ROWS / 4) == 256 && T4_OW * T4_OH <= 256, "one horizontal and one vertical item a thread");
 
 FGL_D uint32_t t4_tap(uint32_t acc, int sh) {
