@@ -150,20 +150,41 @@ def Gray(x):
     return Color(float(x), float(x), float(x), 1.0)
 
 
+def _scan_hex(x, width, count):
+    """fmt.Sscanf(x, "%<width>x" * count): each field takes up to `width` hex
+    digits; the first field without a digit ends the scan, leaving it and
+    the rest unset (None), as the error Sscanf returns does."""
+    out, pos = [], 0
+    for _ in range(count):
+        j = pos
+        while j < len(x) and j - pos < width and x[j] in "0123456789abcdefABCDEF":
+            j += 1
+        if j == pos:
+            break
+        out.append(int(x[pos:j], 16))
+        pos = j
+    return out + [None] * (count - len(out))
+
+
 def HexColor(x):
-    """HexColor (color.go:31-54)"""
+    """HexColor (color.go:31-54): strings.Trim of '#', then by length (in
+    bytes) 3 or 4 one-digit fields each widened to (v << 4) | v, or 6 or 8
+    two-digit fields; any other length leaves R = G = B = 0, A = 255."""
     x = x.strip("#")
+    n = len(x.encode("utf-8"))
+    r = g = b = 0
     a = 255
-    if len(x) == 3:
-        r, g, b = (int(c, 16) * 17 for c in x)
-    elif len(x) == 4:
-        r, g, b, a = (int(c, 16) * 17 for c in x)
-    elif len(x) == 6:
-        r, g, b = int(x[0:2], 16), int(x[2:4], 16), int(x[4:6], 16)
-    elif len(x) == 8:
-        r, g, b, a = int(x[0:2], 16), int(x[2:4], 16), int(x[4:6], 16), int(x[6:8], 16)
+    if n in (3, 4):
+        f = _scan_hex(x, 1, n)
+        f = [None if v is None else (v << 4) | v for v in f]
+    elif n in (6, 8):
+        f = _scan_hex(x, 2, n // 2)
     else:
-        r = g = b = 0
+        f = []
+    if f:
+        r, g, b = (v if v is not None else 0 for v in f[:3])
+        if len(f) == 4 and f[3] is not None:
+            a = f[3]
     d = 255.0
     return Color(r / d, g / d, b / d, a / d)
 
@@ -416,11 +437,70 @@ def _soa(a, nv, k, n=None):
 
 
 class Mesh:
-    """Mesh{Triangles, Lines} with SoA soups."""
+    """Mesh{Triangles, Lines} with SoA soups, held on the host.
+
+    Context.DrawMesh uploads a host mesh's soups at every draw (the
+    reference reads the caller's triangles at every draw too), unless the
+    context was asked to keep it resident (Context.KeepResident).  A
+    resident copy is tagged with the mesh's generation, which dirty()
+    advances (mesh.go:31; Add, Transform and the other in-place methods
+    call it), and is uploaded again when the generation moved on.  Code
+    that edits the soups' arrays directly calls dirty() itself."""
 
     def __init__(self, triangles=None, lines=None):
         self.Triangles = triangles
         self.Lines = lines
+        self.generation = 0
+
+    def dirty(self):
+        """Mesh.dirty (mesh.go:31-33): the mesh changed in place."""
+        self.generation += 1
+
+    def Add(self, other):
+        """Mesh.Add (mesh.go:50-53): appends other's triangles and lines."""
+        def cat(a, b):
+            if a is None:
+                return b
+            if b is None:
+                return a
+            parts = {}
+            for k in ("position", "normal", "texture", "color"):
+                x, y = getattr(a, k), getattr(b, k)
+                if x is None and y is None:
+                    parts[k] = None
+                    continue
+                if x is None:
+                    x = np.zeros((len(a),) + y.shape[1:])
+                if y is None:
+                    y = np.zeros((len(b),) + x.shape[1:])
+                parts[k] = np.concatenate([x, y])
+            return Prims(parts["position"], parts["normal"], parts["texture"], parts["color"], nv=a.nv)
+        self.Triangles = cat(self.Triangles, other.Triangles)
+        self.Lines = cat(self.Lines, other.Lines)
+        self.dirty()
+
+    def Transform(self, matrix):
+        """Mesh.Transform (mesh.go:167-175): Triangle / Line.Transform of
+        every vertex: Position by MulPosition, Normal by MulDirection
+        (matrix.go:209-229), in Go's term order."""
+        m = Matrix(matrix)
+        for p in (self.Triangles, self.Lines):
+            if p is None:
+                continue
+            P = p.position
+            x, y, z = P[..., 0].copy(), P[..., 1].copy(), P[..., 2].copy()
+            for r in range(3):
+                P[..., r] = ((m[4 * r] * x + m[4 * r + 1] * y) + m[4 * r + 2] * z) + m[4 * r + 3]
+            if p.normal is None:
+                p.normal = np.zeros_like(P)
+            Nn = p.normal
+            x, y, z = Nn[..., 0].copy(), Nn[..., 1].copy(), Nn[..., 2].copy()
+            for r in range(3):
+                Nn[..., r] = (m[4 * r] * x + m[4 * r + 1] * y) + m[4 * r + 2] * z
+            with np.errstate(divide="ignore", invalid="ignore"):
+                rr = 1 / np.sqrt((Nn[..., 0] * Nn[..., 0] + Nn[..., 1] * Nn[..., 1]) + Nn[..., 2] * Nn[..., 2])
+                Nn *= rr[..., None]
+        self.dirty()
 
 
 def NewTriangleMesh(triangles):
@@ -636,6 +716,7 @@ class Context:
         self.LineWidth = 2.0
         self.DepthBias = 0.0
         self._textures = []
+        self._resident = {}  # id(host Mesh) -> [mesh, generation uploaded, DeviceMesh]
 
     def _ptr(self):
         if not self.h:
@@ -699,9 +780,29 @@ class Context:
             N.check(N.load().fgl_draw_mesh(self._ptr(), ctypes.byref(st), ctypes.byref(sh), mesh.h, info),
                     self._ptr())
             return RasterizeInfo(int(info[0]), int(info[1]))
+        ent = self._resident.get(id(mesh))
+        if ent is not None and ent[0] is mesh:
+            if ent[1] != mesh.generation or ent[2] is None:
+                # the mesh changed in place since its upload (Mesh.dirty)
+                if ent[2] is not None:
+                    ent[2].close()
+                ent[2] = DeviceMesh(self, mesh)
+                ent[1] = mesh.generation
+            return self.DrawMesh(ent[2])
         i1 = self.DrawTriangles(mesh.Triangles) if mesh.Triangles is not None else RasterizeInfo(0, 0)
         i2 = self.DrawLines(mesh.Lines) if mesh.Lines is not None else RasterizeInfo(0, 0)
         return RasterizeInfo(i1[0] + i2[0], i1[1] + i2[1])
+
+    def KeepResident(self, mesh, keep=True):
+        """Opts a host Mesh into residency on this context's device: DrawMesh
+        then draws an uploaded copy, uploaded again whenever the mesh's
+        generation (Mesh.dirty) moved on since.  keep=False frees the copy;
+        DrawMesh goes back to uploading the soups at every draw."""
+        ent = self._resident.pop(id(mesh), None)
+        if ent is not None and ent[2] is not None:
+            ent[2].close()
+        if keep:
+            self._resident[id(mesh)] = [mesh, None, None]
 
     def Upload(self, mesh):
         """Keeps mesh resident on this context's device for repeated DrawMesh."""

@@ -96,7 +96,9 @@ static int go_is_odd_int(double x) {
 
 /* Go math.Pow: special cases, then x**yi by repeated squaring of the
  * Frexp mantissa with the binary exponent tracked separately, times
- * x**yf (exp/log; libm here -- parity unpinned for non-integer y). */
+ * x**yf = exp(yf*log(x)) with the C library's exp and log.  For non-integer
+ * y the value is pinned against exact powers within a stated ulp bound;
+ * its last bits against Go's own Exp and Log are not. */
 static double go_pow(double x, double y) {
     if (y == 0 || x == 1) return 1;
     if (y == 1) return x;
@@ -135,9 +137,9 @@ static double go_pow(double x, double y) {
     if (yf != 0) {
         if (yf > 0.5) {
             yf--;
-            yi++;
+            if (ORC_MUTANT != 16) yi++;
         }
-        a1 = exp(yf * log(x));
+        a1 = ORC_MUTANT == 15 ? exp(yf * log10(x)) : exp(yf * log(x));
     }
     int xe_i;
     double x1 = frexp(x, &xe_i);

@@ -29,6 +29,8 @@ MUTANTS = {
     12: ("tent taps start one source pixel early", "resolve"),
     13: ("tent horizontal pass does not premultiply by alpha", "resolve"),
     14: ("box resolve truncates instead of rounding", "resolve"),
+    15: ("pow: the fractional part uses log10 for log", "pow"),
+    16: ("pow: the fold yf > 1/2 drops its yi + 1", "pow"),
 }
 
 

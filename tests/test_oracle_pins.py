@@ -110,6 +110,41 @@ def test_go_pow_integer_exponents_within_the_product_error_bound():
                 assert ulps(got, ref) <= 1
 
 
+def test_go_pow_fractional_exponents_against_exact_powers():
+    # Non-integer y (shader.go:91, Go's math.Pow): y = yi + yf with yf folded
+    # into (-1/2, 1/2], a1 = exp(yf * log x), times x**yi by binary powering.
+    # Reference value: x**y for the exact binary x and y at 60 decimal
+    # digits.  Error bound, u = 2**-53: log x and the product yf*log x are
+    # each within 1 ulp, so the argument t of exp is off by at most
+    # |t| * 2**-51 absolute, i.e. a relative error of |t| * 2**-51 in exp(t);
+    # exp adds 1 ulp, the integer chain yi ulps (as above) and the final
+    # reciprocal for y < 0 half an ulp.  In ulps of the result (an ulp is at
+    # least 2**-53 relative): 4|t| + 2*yi + 4.  log10 for log, or a fold
+    # that drops the yi + 1, is off by orders of magnitude more.  (The fold
+    # itself changes the value only by rounding: x**y = x**(yf-1) * x**(yi+1);
+    # which rounding Go's own Exp / Log give is not pinned here.)
+    import decimal
+    ctx = decimal.Context(prec=60)
+    rng = np.random.Generator(np.random.PCG64(11))
+    xs = [float(v) for v in rng.uniform(1e-3, 1.0, 120)] + [float(v) for v in rng.uniform(1.0, 40.0, 40)]
+    xs += [0.5, 0.999999, 1e-5, 0.3, 3.0, 1.0000001]
+    for y in (0.75, 2.5, 12.5, 33.3, -4.5, 1.25, -0.75, 7.6):
+        ay = abs(y)
+        yi, yf = math.floor(ay), ay - math.floor(ay)
+        if yf > 0.5:
+            yf, yi = yf - 1, yi + 1
+        for x in xs:
+            got = O.go_pow(x, y)
+            ref = ctx.power(decimal.Decimal(x), decimal.Decimal(y))
+            if not (2.2250738585072014e-308 < float(ref) < MAXF):
+                continue
+            t = abs(yf * math.log(x))
+            bound = 4 * t + 2 * yi + 4
+            err = abs(decimal.Decimal(got) - ref)
+            ulp = decimal.Decimal(math.ulp(float(ref)))
+            assert float(err / ulp) <= bound, (x, y, got, float(ref), float(err / ulp), bound)
+
+
 def test_go_pow_special_cases():
     assert O.go_pow(0.3, 0.0) == 1.0 and O.go_pow(1.0, 1e300) == 1.0 and O.go_pow(0.37, 1.0) == 0.37
     assert O.go_pow(0.25, 0.5) == 0.5 and O.go_pow(4.0, -0.5) == 0.5
