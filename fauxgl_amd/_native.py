@@ -38,7 +38,7 @@ EXPORTED = [
     "fgl_encode_png_bound", "fgl_encode_png",
     "fgl_batch_create", "fgl_batch_destroy", "fgl_batch_error", "fgl_batch_add", "fgl_batch_run",
     "fgl_tile_shapes", "fgl_ctx_set_tile", "fgl_ctx_get_tile", "fgl_autotune_mesh", "fgl_tune_save",
-    "fgl_tune_load", "fgl_tune_clear",
+    "fgl_tune_load", "fgl_tune_clear", "fgl_ctx_force_capacities",
 ]
 
 _d = ctypes.c_double
@@ -46,7 +46,8 @@ _d = ctypes.c_double
 
 class FglDrawStat(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in ("total_pixels", "updated_pixels", "clipped_triangles",
-                                                "side_triangles", "pairs", "spans", "fragments")]
+                                                "side_triangles", "pairs", "spans", "fragments",
+                                                "overflow_mask", "reruns")]
 
 
 class FglState(ctypes.Structure):
@@ -121,6 +122,7 @@ def load():
         "fgl_ctx_sync": (i32, [vp]),
         "fgl_profile_only": (i32, [vp, ctypes.c_char_p]),
         "fgl_draw_stats": (i32, [vp, ctypes.POINTER(FglDrawStat)]),
+        "fgl_ctx_force_capacities": (i32, [vp, ctypes.POINTER(ctypes.c_uint32)]),
         "fgl_mesh_counts": (i32, [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
         "fgl_mesh_read": (i32, [vp, i32, i32, ctypes.POINTER(_d), ctypes.c_int64]),
         "fgl_mesh_bounding_box": (i32, [vp, ctypes.POINTER(_d)]),

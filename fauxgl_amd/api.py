@@ -911,6 +911,13 @@ class Context:
     def Sync(self):
         N.check(N.load().fgl_ctx_sync(self._ptr()), self._ptr())
 
+    def ForceCapacities(self, side_triangles=0, clipped_vertices=0, pairs=0, spans=0, fragments=0):
+        """Failure injection: the next draw's first pass runs with at most
+        these capacities (0 = as computed), overflows, grows its buffers and
+        runs again; DrawStats() then reports the overflow bits and reruns."""
+        caps = (ctypes.c_uint32 * 5)(side_triangles, clipped_vertices, pairs, spans, fragments)
+        N.check(N.load().fgl_ctx_force_capacities(self._ptr(), caps), self._ptr())
+
     def DrawStats(self):
         """Work counts of the last draw (pairs, spans, fragments, ...); a
         diagnostic with no reference counterpart."""

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cfloat>
 #include <cmath>
@@ -33,16 +34,25 @@ void set_err(std::string &dst, const char *fmt, ...) {
     dst = buf;
 }
 
+// Advances whenever the library frees or replaces a device allocation
+// (buffers, textures).  A recorded graph bakes device addresses into its
+// kernel nodes; a batch replays its graph only while this is unchanged.
+std::atomic<uint64_t> g_dev_epoch{0};
+
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
     ~DevBuf() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            (void)hipFree(p);
+            g_dev_epoch++;
+        }
     }
     hipError_t ensure(size_t need) {
         if (need <= bytes && p) return hipSuccess;
         if (p) {
             (void)hipFree(p);
+            g_dev_epoch++;
             p = nullptr;
             bytes = 0;
         }
@@ -104,6 +114,14 @@ struct fgl_ctx {
 
     int32_t tile_w = 0, tile_h = 0;  // tile shape set for this context (0: from the tuning table)
 
+    // Failure injection: capacities (side records, side vertices, pairs,
+    // spans, fragments; 0 = as computed) that the next draw's first pass
+    // starts with.  Its kernels then overflow, set their bits and exit,
+    // and the draw grows the buffers and runs again, as for a real overflow.
+    uint32_t force_caps[5] = {0, 0, 0, 0, 0};
+    uint32_t last_overflow = 0;  // overflow bits seen by the last draw's passes
+    uint32_t last_reruns = 0;    // passes the last draw ran again after an overflow
+
     bool prof = false;
     std::string prof_only;  // time only launches of this kernel (empty: all)
     std::map<std::string, KStat> kstats;
@@ -112,7 +130,10 @@ struct fgl_ctx {
 
     ~fgl_ctx() {
         for (auto &kv : tex)
-            if (kv.second.d) (void)hipFree(kv.second.d);
+            if (kv.second.d) {
+                (void)hipFree(kv.second.d);
+                g_dev_epoch++;
+            }
         for (auto &p : pending) {
             (void)hipEventDestroy(p.a);
             (void)hipEventDestroy(p.b);
@@ -477,6 +498,10 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
     HIPCHK(c, c->tstart.ensure((size_t)a.ntiles * 4));
     HIPCHK(c, c->tend.ensure((size_t)a.ntiles * 4));
 
+    if (!capture_ctr) {
+        c->last_overflow = 0;
+        c->last_reruns = 0;
+    }
     for (int attempt = 0; attempt < 6; attempt++) {
         HIPCHK(c, c->side_rec.ensure((size_t)c->side_rec_cap * sizeof(PrimRec)));
         HIPCHK(c, c->side_vtx.ensure((size_t)c->side_vtx_cap * SIDE_VTX_DOUBLES * 8));
@@ -537,6 +562,18 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
         a.clear_depth = c->clear_depth;
         a.ctr = c->ctr.as<DrawCounters>();
         a.zeros = c->zeros.as<double>();
+        if (attempt == 0 && !capture_ctr) {
+            // failure injection: the kernels see smaller capacities than the
+            // buffers hold, so an overflow check that is off cannot write
+            // outside them
+            const uint32_t *f = c->force_caps;
+            if (f[0]) a.side_rec_cap = std::min(a.side_rec_cap, f[0]);
+            if (f[1]) a.side_vtx_cap = std::min(a.side_vtx_cap, f[1]);
+            if (f[2]) a.pair_cap = std::min(a.pair_cap, f[2]);
+            if (f[3]) a.span_cap = std::min(a.span_cap, f[3]);
+            if (f[4]) a.frag_cap = std::min(a.frag_cap, f[4]);
+            std::memset(c->force_caps, 0, sizeof c->force_caps);
+        }
 
         if ((r = launch(c, "zero", [&] { return T.zero(a, c->stream); }))) return r;
         if ((r = launch(c, "vertex", [&] { return T.vs(a, c->stream); }))) return r;
@@ -566,6 +603,8 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
         HIPCHK(c, hipStreamSynchronize(c->stream));
         if (c->prof) prof_collect(c);
         c->last = h;
+        c->last_overflow |= h.overflow;
+        if (h.overflow) c->last_reruns++;
         if (h.overflow == 0) {
             if (info) {
                 info[0] = h.total;
@@ -776,6 +815,7 @@ int fgl_texture_destroy(fgl_ctx *ctx, int64_t handle) {
     }
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(it->second.d);
+    g_dev_epoch++;
     ctx->tex.erase(it);
     return FGL_OK;
 }
@@ -1353,6 +1393,18 @@ int fgl_draw_stats(fgl_ctx *ctx, fgl_draw_stat *out) {
     out->pairs = (uint64_t)h.pairs_hi << 32 | h.pairs_lo;
     out->spans = (uint64_t)h.spans_hi << 32 | h.spans_lo;
     out->fragments = (uint64_t)h.frags_hi << 32 | h.frags_lo;
+    out->overflow_mask = ctx->last_overflow;
+    out->reruns = ctx->last_reruns;
+    return FGL_OK;
+}
+
+int fgl_ctx_force_capacities(fgl_ctx *ctx, const uint32_t caps[5]) {
+    if (check_ctx(ctx)) return FGL_E_INVALID;
+    if (!caps) {
+        ctx->err = "fgl_ctx_force_capacities: null capacities";
+        return FGL_E_INVALID;
+    }
+    std::memcpy(ctx->force_caps, caps, sizeof ctx->force_caps);
     return FGL_OK;
 }
 
@@ -2327,9 +2379,15 @@ int fgl_encode_png(fgl_ctx *ctx, uint8_t *out, int64_t cap, int64_t *len) {
 // view's clear + triangle + line pipelines on its context's stream (views
 // of different contexts are independent branches; views of one context run
 // in order), a kernel copying each pipeline's counters into one array, a
-// join back.  Later runs replay the graph and read the counters once; a
-// view whose frame outgrew its buffers is redrawn eagerly and the graph is
-// recorded again.
+// join back.  Later runs replay the graph and read the counters once.
+//
+// The graph bakes device addresses: every view's context buffers, mesh
+// arrays and textures.  It is replayed only while no library allocation has
+// been freed or replaced since it was recorded (g_dev_epoch); otherwise the
+// run starts over with the eager pass and records again.  A replay first
+// waits for the work already queued on every view's context.  When a view's
+// frame outgrew its buffers, that view and every later view on the same
+// context are redrawn eagerly, in order, and the graph is recorded again.
 
 struct fgl_batch {
     struct View {
@@ -2341,13 +2399,15 @@ struct fgl_batch {
         double clear_depth;
     };
     std::vector<View> views;
+    std::vector<fgl_ctx *> ctxs;  // the distinct contexts, in first-use order
     int device = -1;
     hipStream_t s0 = nullptr;
     hipEvent_t fork = nullptr;
     std::vector<hipEvent_t> joins;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    DevBuf ctrs;  // 2 DrawCounters per view (triangles, lines)
+    uint64_t epoch = 0;  // g_dev_epoch the graph's addresses are valid for
+    DevBuf ctrs;         // 2 DrawCounters per view (triangles, lines)
     std::vector<DrawCounters> host;
     std::string err;
     ~fgl_batch() {
@@ -2381,6 +2441,9 @@ void mesh_prims(const fgl_mesh *m, bool lines, DevPrims &p) {
         }                                                                                           \
     } while (0)
 
+// ClearColorBufferWith + ClearDepthBufferWith + DrawMesh of view i, drawn
+// directly (buffers grow as needed).  A view with an empty mesh still gets
+// its clears.
 int batch_eager(fgl_batch *b, size_t i, uint64_t info[2]) {
     fgl_batch::View &v = b->views[i];
     v.ctx->clear_color = v.clear_color;
@@ -2393,6 +2456,7 @@ int batch_eager(fgl_batch *b, size_t i, uint64_t info[2]) {
     uint64_t i1[2] = {0, 0}, i2[2] = {0, 0};
     int r = run_draw(v.ctx, &v.st, &v.sh, PRIM_TRI, t, i1);
     if (!r) r = run_draw(v.ctx, &v.st, &v.sh, PRIM_LINE, l, i2);
+    if (!r && t.n == 0 && l.n == 0) r = materialize(v.ctx);
     if (r) {
         set_err(b->err, "view %zu: %s", i, v.ctx->err.c_str());
         return r;
@@ -2402,14 +2466,15 @@ int batch_eager(fgl_batch *b, size_t i, uint64_t info[2]) {
     return FGL_OK;
 }
 
-int batch_capture(fgl_batch *b) {
+// Records every view into one graph.  Nothing is allocated while recording
+// unless a capacity moved since the eager pass; then the addresses of the
+// views recorded earlier may be stale, and the recording is made again.
+int batch_record(fgl_batch *b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
     b->graph = nullptr;
-    std::vector<fgl_ctx *> ctxs;
-    for (auto &v : b->views)
-        if (std::find(ctxs.begin(), ctxs.end(), v.ctx) == ctxs.end()) ctxs.push_back(v.ctx);
+    std::vector<fgl_ctx *> &ctxs = b->ctxs;
     while (b->joins.size() < ctxs.size()) {
         hipEvent_t e;
         BCHK(b, hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -2420,7 +2485,8 @@ int batch_capture(fgl_batch *b) {
         prof[k] = ctxs[k]->prof;
         ctxs[k]->prof = false;  // no timing events inside the graph
     }
-    BCHK(b, hipStreamBeginCapture(b->s0, hipStreamCaptureModeGlobal));
+    const uint64_t epoch0 = g_dev_epoch.load();
+    BCHK(b, hipStreamBeginCapture(b->s0, hipStreamCaptureModeThreadLocal));
     int r = FGL_OK;
     if (hipEventRecord(b->fork, b->s0) != hipSuccess) r = FGL_E_DEVICE;
     for (size_t k = 0; k < ctxs.size() && !r; k++)
@@ -2440,6 +2506,7 @@ int batch_capture(fgl_batch *b) {
         if (t.n > 0) v.ctx->pend_color = v.ctx->pend_depth = false;  // the triangle pass clears
         if (!r) r = run_draw(v.ctx, &v.st, &v.sh, PRIM_LINE, l, nullptr, dst + (2 * i + 1) * words);
         if (l.n > 0) v.ctx->pend_color = v.ctx->pend_depth = false;
+        if (!r && t.n == 0 && l.n == 0) r = materialize(v.ctx);  // fill kernels: the clear of an empty view
         if (r) set_err(b->err, "view %zu: %s", i, v.ctx->err.c_str());
     }
     for (size_t k = 0; k < ctxs.size() && !r; k++) {
@@ -2459,9 +2526,39 @@ int batch_capture(fgl_batch *b) {
         set_err(b->err, "hipStreamEndCapture failed: %s", hipGetErrorString(ee));
         return FGL_E_DEVICE;
     }
+    if (g_dev_epoch.load() != epoch0) {  // an address recorded above may be stale
+        (void)hipGraphDestroy(g);
+        return 1;
+    }
     b->graph = g;
     BCHK(b, hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    b->epoch = epoch0;
     return FGL_OK;
+}
+
+int batch_capture(fgl_batch *b) {
+    for (int attempt = 0; attempt < 3; attempt++) {
+        const int r = batch_record(b);
+        if (r <= 0) return r;
+    }
+    b->err = "recording the batch kept reallocating device buffers";
+    return FGL_E_DEVICE;
+}
+
+// The eager pass: every view drawn directly (sizing every context's
+// buffers), then the graph recorded.
+int batch_first_run(fgl_batch *b, uint64_t *out) {
+    const size_t n = b->views.size();
+    for (size_t i = 0; i < n; i++) {
+        int r = batch_eager(b, i, out + 2 * i);
+        if (r) return r;
+    }
+    if (b->ctrs.ensure(2 * n * sizeof(DrawCounters)) != hipSuccess) {
+        b->err = "fgl_batch_run: device allocation failed";
+        return FGL_E_NOMEM;
+    }
+    for (fgl_ctx *c : b->ctxs) BCHK(b, hipStreamSynchronize(c->stream));
+    return batch_capture(b);
 }
 
 }  // namespace
@@ -2503,6 +2600,7 @@ int fgl_batch_add(fgl_batch *b, fgl_ctx *ctx, const uint8_t clear_nrgba[4], doub
                     ((uint32_t)clear_nrgba[3] << 24);
     v.clear_depth = clear_depth;
     b->views.push_back(v);
+    if (std::find(b->ctxs.begin(), b->ctxs.end(), ctx) == b->ctxs.end()) b->ctxs.push_back(ctx);
     if (b->exec) {  // the recorded graph no longer covers every view
         (void)hipGraphExecDestroy(b->exec);
         b->exec = nullptr;
@@ -2519,48 +2617,49 @@ int fgl_batch_run(fgl_batch *b, uint64_t *info) {
         BCHK(b, hipStreamCreateWithFlags(&b->s0, hipStreamNonBlocking));
         BCHK(b, hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
     }
+    b->err.clear();
     std::vector<uint64_t> tmp(2 * n);
     uint64_t *out = info ? info : tmp.data();
-    if (!b->exec) {
-        // eager pass: every view drawn once, every context's buffers sized
-        for (size_t i = 0; i < n; i++) {
-            int r = batch_eager(b, i, out + 2 * i);
-            if (r) return r;
-        }
-        for (auto &v : b->views) BCHK(b, hipStreamSynchronize(v.ctx->stream));
-        if (b->ctrs.ensure(2 * n * sizeof(DrawCounters)) != hipSuccess) {
-            b->err = "fgl_batch_run: device allocation failed";
-            return FGL_E_NOMEM;
-        }
-        int r = batch_capture(b);
-        if (r) return r;
-        for (auto &v : b->views) {  // the eager pass left every frame drawn
-            v.ctx->pend_color = v.ctx->pend_depth = false;
-        }
-        return FGL_OK;
+    if (!b->exec || b->epoch != g_dev_epoch.load()) return batch_first_run(b, out);
+    // the replay follows whatever is already queued on the views' contexts
+    // (a resolve or readback of the previous frame, a clear's fill)
+    for (fgl_ctx *c : b->ctxs) {
+        if (!c->xev) BCHK(b, hipEventCreateWithFlags(&c->xev, hipEventDisableTiming));
+        BCHK(b, hipEventRecord(c->xev, c->stream));
+        BCHK(b, hipStreamWaitEvent(b->s0, c->xev, 0));
     }
     BCHK(b, hipGraphLaunch(b->exec, b->s0));
     b->host.resize(2 * n);
     BCHK(b, hipMemcpyAsync(b->host.data(), b->ctrs.p, 2 * n * sizeof(DrawCounters), hipMemcpyDeviceToHost, b->s0));
     BCHK(b, hipStreamSynchronize(b->s0));
-    bool regrow = false;
+    // every view cleared its context as it was drawn
+    for (fgl_ctx *c : b->ctxs) c->pend_color = c->pend_depth = false;
+    // the first view per context whose frame no longer fits the recorded
+    // capacities
+    std::map<fgl_ctx *, size_t> first_ovf;
     for (size_t i = 0; i < n; i++) {
         const DrawCounters &t = b->host[2 * i], &l = b->host[2 * i + 1];
         const bool has_t = b->views[i].mesh->ntri > 0, has_l = b->views[i].mesh->nline > 0;
-        if ((has_t && t.overflow) || (has_l && l.overflow)) {
-            // this frame no longer fits the recorded capacities: draw it
-            // eagerly (growing its buffers) and record the graph again
+        if ((has_t && t.overflow) || (has_l && l.overflow)) first_ovf.emplace(b->views[i].ctx, i);
+    }
+    for (size_t i = 0; i < n; i++) {
+        fgl_batch::View &v = b->views[i];
+        auto it = first_ovf.find(v.ctx);
+        if (it != first_ovf.end() && i >= it->second) {
+            // redrawn directly (growing its buffers), in view order, so the
+            // context ends with the last of its views
             int r = batch_eager(b, i, out + 2 * i);
             if (r) return r;
-            regrow = true;
             continue;
         }
+        const DrawCounters &t = b->host[2 * i], &l = b->host[2 * i + 1];
+        const bool has_t = v.mesh->ntri > 0, has_l = v.mesh->nline > 0;
         out[2 * i] = (has_t ? t.total : 0) + (has_l ? l.total : 0);
         out[2 * i + 1] = (has_t ? updated_pixels(t) : 0) + (has_l ? updated_pixels(l) : 0);
-        b->views[i].ctx->last = has_l ? l : t;
+        if (has_t || has_l) v.ctx->last = has_l ? l : t;
     }
-    if (regrow) {
-        for (auto &v : b->views) BCHK(b, hipStreamSynchronize(v.ctx->stream));
+    if (!first_ovf.empty()) {
+        for (fgl_ctx *c : b->ctxs) BCHK(b, hipStreamSynchronize(c->stream));
         return batch_capture(b);
     }
     return FGL_OK;

@@ -35,14 +35,16 @@ extern "C" {
 
 typedef enum fgl_status {
     FGL_OK = 0,
-    FGL_E_INVALID = -1,     /* bad argument (null pointer, size, enum value) */
+    FGL_E_INVALID = -1,     /* bad argument (null pointer, size, shader / texture / filter kind) */
     FGL_E_NOMEM = -2,       /* device or host allocation failed */
     FGL_E_DEVICE = -3,      /* HIP runtime error or no usable gfx950 device */
     FGL_E_UNSUPPORTED = -4, /* request outside what the device path implements */
     FGL_E_IO = -5           /* a file could not be opened, read or written */
 } fgl_status;
 
-/* context.go:11-26 */
+/* context.go:11-26: the same values as Go's iota enums (FaceCW = 1,
+ * FaceCCW = 2; CullNone = 1, CullFront = 2, CullBack = 3), so a Go caller
+ * passes dc.FrontFace and dc.Cull unchanged. */
 enum { FGL_FACE_CW = 1, FGL_FACE_CCW = 2 };
 enum { FGL_CULL_NONE = 1, FGL_CULL_FRONT = 2, FGL_CULL_BACK = 3 };
 
@@ -60,12 +62,16 @@ enum { FGL_TEX_NRGBA8 = 1, FGL_TEX_RGBA8 = 2, FGL_TEX_RGBA16 = 3 };
 enum { FGL_RESOLVE_TENT = 1, FGL_RESOLVE_BOX = 2 };
 
 /* Pipeline state = the public Context fields read at draw time
- * (context.go:47-55). */
+ * (context.go:47-55).  front_face and cull accept any int32, as the Go
+ * fields accept any int, and act as drawClippedTriangle's comparisons do
+ * (context.go:328-336): only FGL_FACE_CW reverses the winding; FGL_CULL_NONE
+ * keeps both faces, FGL_CULL_FRONT culls front faces, and every other value
+ * (0 included) culls back faces like FGL_CULL_BACK.  Nothing rejects them. */
 typedef struct fgl_state {
     uint8_t read_depth, write_depth, write_color, alpha_blend, wireframe;
     uint8_t pad[3];
-    int32_t front_face; /* FGL_FACE_* */
-    int32_t cull;       /* FGL_CULL_* */
+    int32_t front_face; /* FGL_FACE_* (any value, see above) */
+    int32_t cull;       /* FGL_CULL_* (any value, see above) */
     double line_width;
     double depth_bias;
 } fgl_state;
@@ -312,13 +318,26 @@ int fgl_ctx_sync(fgl_ctx *ctx);
  * input triangles that crossed a clip plane, side_triangles the raster
  * triangles written by clipping, wireframe and line expansion, pairs the
  * (tile, triangle) bin entries, spans the per-row coverage runs and
- * fragments the covered in-bounds pixels shaded. */
+ * fragments the covered in-bounds pixels shaded.  overflow_mask: the
+ * capacity bits the draw's passes hit (1 side triangles, 2 clipped
+ * vertices, 4 pairs, 16 spans, 32 fragments); reruns: how many passes ran
+ * again after growing the buffers. */
 typedef struct fgl_draw_stat {
     uint64_t total_pixels, updated_pixels;
     uint64_t clipped_triangles, side_triangles;
     uint64_t pairs, spans, fragments;
+    uint64_t overflow_mask, reruns;
 } fgl_draw_stat;
 int fgl_draw_stats(fgl_ctx *ctx, fgl_draw_stat *out);
+
+/* Failure injection (no reference counterpart; for tests of the capacity
+ * path).  The next triangle or line draw on ctx starts its first pass with
+ * at most caps[0] side triangles, caps[1] clipped vertices, caps[2] pairs,
+ * caps[3] spans and caps[4] fragments (0 = as computed).  A kernel that
+ * runs out sets its overflow bit and exits normally; the host then grows
+ * the buffers and runs the draw again, so the result is unchanged and
+ * fgl_draw_stats reports the bits and the reruns. */
+int fgl_ctx_force_capacities(fgl_ctx *ctx, const uint32_t caps[5]);
 
 /* ---- learned anti-aliasing (no reference counterpart: it replaces the
  * 16x SSAA render + resolve that the reference pipeline would run) ----

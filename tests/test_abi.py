@@ -58,7 +58,7 @@ STRUCTS = {
     "fgl_prims": (N.FglPrims, ["count", "position", "normal", "texture", "color"]),
     "fgl_kernel_time": (N.FglKernelTime, ["name", "launches", "total_ms"]),
     "fgl_draw_stat": (N.FglDrawStat, ["total_pixels", "updated_pixels", "clipped_triangles", "side_triangles",
-                                      "pairs", "spans", "fragments"]),
+                                      "pairs", "spans", "fragments", "overflow_mask", "reruns"]),
 }
 
 
