@@ -422,26 +422,11 @@ uint32_t cap_for(uint32_t lo, uint32_t hi) {
     return (uint32_t)std::min<uint64_t>(v, 0xfffff000ull);
 }
 
-int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, const DevPrims &in, uint64_t info[2],
-             uint32_t *capture_ctr = nullptr) {
-    if (info) info[0] = info[1] = 0;
-    if (in.n < 0 || in.n > kMaxPrims) {
-        set_err(c->err, "primitive count %lld outside [0, %lld]", (long long)in.n, (long long)kMaxPrims);
-        return FGL_E_INVALID;
-    }
-    DrawArgs a;
-    std::memset(&a, 0, sizeof a);
-    int r = fill_shader(c, sh, a.sh);
-    if (r) return r;
-    if (in.n == 0) return FGL_OK;
-    if (hipSetDevice(c->device) != hipSuccess) {
-        set_err(c->err, "hipSetDevice failed");
-        return FGL_E_DEVICE;
-    }
-
+// The draw-wide fields of a draw's kernel arguments: framebuffer, tiles,
+// state (context.go:47-55), shader and input; the buffers come later.
+void draw_setup(fgl_ctx *c, const fgl_state *st, const TileOps &T, int kind, const DevPrims &in, DrawArgs &a) {
     a.W = c->W;
     a.H = c->H;
-    const TileOps &T = *draw_ops(c, kind, in.n);
     a.ntx = (c->W + T.tw - 1) / T.tw;
     a.nty = (c->H + T.th - 1) / T.th;
     a.ntiles = a.ntx * a.nty;
@@ -462,12 +447,57 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
     screen_matrix(c->W, c->H, a.S);
     a.color = c->color.as<uint32_t>();
     a.depth = c->depth.as<double>();
-
-    const int64_t n = in.n;
-    a.nvsblocks = (int32_t)((n + VS_THREADS - 1) / VS_THREADS);
+    a.nvsblocks = (int32_t)((in.n + VS_THREADS - 1) / VS_THREADS);
     a.key_bits = key_bits_for(a.ntiles);
+    a.sorted_buf = ((a.key_bits + 7) / 8) & 1;
+    a.clear_color_pending = c->pend_color;
+    a.clear_depth_pending = c->pend_depth;
+    a.clear_color = c->clear_color;
+    a.clear_depth = c->clear_depth;
+}
+
+// One pass of a draw: the kernel sequence on the context's stream.
+int draw_pass(fgl_ctx *c, const TileOps &T, const DrawArgs &a, int kind) {
+    int r;
     const int npass = (a.key_bits + 7) / 8;
-    a.sorted_buf = npass & 1;
+    if ((r = launch(c, "zero", [&] { return T.zero(a, c->stream); }))) return r;
+    if ((r = launch(c, "vertex", [&] { return T.vs(a, c->stream); }))) return r;
+    if (kind == PRIM_TRI && !a.st.wireframe)
+        if ((r = launch(c, "vertex_clip", [&] { return T.vs_deferred(a, c->stream); }))) return r;
+    if ((r = launch(c, "scan", [&] { return T.scan_top(a, c->stream); }))) return r;
+    if ((r = launch(c, "emit", [&] { return T.emit(a, c->stream); }))) return r;
+    for (int p = 0; p < npass; p++) {
+        if ((r = launch(c, "radix_hist", [&] { return T.radix_hist(a, p, c->stream); }))) return r;
+        if ((r = launch(c, "radix_scan", [&] { return T.radix_scan(a, c->stream); }))) return r;
+        if ((r = launch(c, "radix_scatter", [&] { return T.radix_scatter(a, p, c->stream); }))) return r;
+    }
+    if ((r = launch(c, "span_count", [&] { return T.span_count(a, c->stream); }))) return r;
+    if ((r = launch(c, "span_scan", [&] { return T.span_scan(a, c->stream); }))) return r;
+    if ((r = launch(c, "spans", [&] { return T.spans(a, c->stream); }))) return r;
+    if ((r = launch(c, "frag_scan", [&] { return T.frag_scan(a, c->stream); }))) return r;
+    if ((r = launch(c, "shade", [&] { return T.shade(a, c->stream); }))) return r;
+    return launch(c, "rop", [&] { return T.rop(a, c->stream); });
+}
+
+int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, const DevPrims &in, uint64_t info[2],
+             uint32_t *capture_ctr = nullptr) {
+    if (info) info[0] = info[1] = 0;
+    if (in.n < 0 || in.n > kMaxPrims) {
+        set_err(c->err, "primitive count %lld outside [0, %lld]", (long long)in.n, (long long)kMaxPrims);
+        return FGL_E_INVALID;
+    }
+    DrawArgs a;
+    std::memset(&a, 0, sizeof a);
+    int r = fill_shader(c, sh, a.sh);
+    if (r) return r;
+    if (in.n == 0) return FGL_OK;
+    if (hipSetDevice(c->device) != hipSuccess) {
+        set_err(c->err, "hipSetDevice failed");
+        return FGL_E_DEVICE;
+    }
+    const TileOps &T = *draw_ops(c, kind, in.n);
+    draw_setup(c, st, T, kind, in, a);
+    const int64_t n = in.n;
 
     // starting capacities; grown on overflow
     uint64_t want_side = kind == PRIM_LINE ? (uint64_t)2 * n : (st->wireframe ? (uint64_t)6 * n : (uint64_t)n / 8);
@@ -556,10 +586,6 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
         a.deferred = c->deferred.as<uint32_t>();
         a.tstart = c->tstart.as<uint32_t>();
         a.tend = c->tend.as<uint32_t>();
-        a.clear_color_pending = c->pend_color;
-        a.clear_depth_pending = c->pend_depth;
-        a.clear_color = c->clear_color;
-        a.clear_depth = c->clear_depth;
         a.ctr = c->ctr.as<DrawCounters>();
         a.zeros = c->zeros.as<double>();
         if (attempt == 0 && !capture_ctr) {
@@ -575,23 +601,7 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
             std::memset(c->force_caps, 0, sizeof c->force_caps);
         }
 
-        if ((r = launch(c, "zero", [&] { return T.zero(a, c->stream); }))) return r;
-        if ((r = launch(c, "vertex", [&] { return T.vs(a, c->stream); }))) return r;
-        if (kind == PRIM_TRI && !a.st.wireframe)
-            if ((r = launch(c, "vertex_clip", [&] { return T.vs_deferred(a, c->stream); }))) return r;
-        if ((r = launch(c, "scan", [&] { return T.scan_top(a, c->stream); }))) return r;
-        if ((r = launch(c, "emit", [&] { return T.emit(a, c->stream); }))) return r;
-        for (int p = 0; p < npass; p++) {
-            if ((r = launch(c, "radix_hist", [&] { return T.radix_hist(a, p, c->stream); }))) return r;
-            if ((r = launch(c, "radix_scan", [&] { return T.radix_scan(a, c->stream); }))) return r;
-            if ((r = launch(c, "radix_scatter", [&] { return T.radix_scatter(a, p, c->stream); }))) return r;
-        }
-        if ((r = launch(c, "span_count", [&] { return T.span_count(a, c->stream); }))) return r;
-        if ((r = launch(c, "span_scan", [&] { return T.span_scan(a, c->stream); }))) return r;
-        if ((r = launch(c, "spans", [&] { return T.spans(a, c->stream); }))) return r;
-        if ((r = launch(c, "frag_scan", [&] { return T.frag_scan(a, c->stream); }))) return r;
-        if ((r = launch(c, "shade", [&] { return T.shade(a, c->stream); }))) return r;
-        if ((r = launch(c, "rop", [&] { return T.rop(a, c->stream); }))) return r;
+        if ((r = draw_pass(c, T, a, kind))) return r;
         if (capture_ctr)
             return launch(c, "counters", [&] {
                 return launch_copy_words(reinterpret_cast<const uint32_t *>(a.ctr), capture_ctr,
