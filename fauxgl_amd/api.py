@@ -725,6 +725,12 @@ class Context:
 
     def close(self):
         if getattr(self, "h", None):
+            # the resident copies refer to this context (and it to them), so
+            # the collector may reach either first: free them before it
+            for ent in getattr(self, "_resident", {}).values():
+                if ent[2] is not None:
+                    ent[2].close()
+            self._resident = {}
             N.load().fgl_ctx_destroy(self.h)
             self.h = None
 

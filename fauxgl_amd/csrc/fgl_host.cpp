@@ -144,7 +144,8 @@ struct fgl_ctx {
 };
 
 struct fgl_mesh {
-    fgl_ctx *ctx = nullptr;
+    fgl_ctx *ctx = nullptr;  // home context: its stream and error slot serve the mesh's own calls
+    int device = -1;         // the device the arrays live on (fgl_mesh_destroy needs no context)
     int64_t ntri = 0, nline = 0;
     DevBuf tpos, tnrm, tuv, tcol, lpos, lnrm, luv, lcol;
     bool t_has[4] = {false, false, false, false};
@@ -205,6 +206,9 @@ int launch(fgl_ctx *c, const char *name, F &&f) {
         b = ev_get(c);
         if (a) (void)hipEventRecord(a, c->stream);
     }
+    // an error left by an earlier call whose result was ignored (a destroy
+    // path, a timing event) is not this launch's
+    (void)hipGetLastError();
     hipError_t e = f();
     if (e != hipSuccess) {
         set_err(c->err, "launch of %s failed: %s", name, hipGetErrorString(e));
@@ -761,6 +765,7 @@ void fgl_ctx_destroy(fgl_ctx *ctx) {
     hipStream_t s = ctx->stream;
     delete ctx;
     (void)hipStreamDestroy(s);
+    (void)hipGetLastError();  // the calls above report nothing to the next launch
 }
 
 int fgl_ctx_size(const fgl_ctx *ctx, int32_t *width, int32_t *height) {
@@ -899,6 +904,7 @@ int fgl_mesh_create(fgl_ctx *ctx, const fgl_prims *tris, const fgl_prims *lines,
         return FGL_E_NOMEM;
     }
     m->ctx = ctx;
+    m->device = ctx->device;
     m->ntri = nt;
     m->nline = nl;
     int r = FGL_OK;
@@ -924,11 +930,16 @@ int fgl_mesh_create(fgl_ctx *ctx, const fgl_prims *tris, const fgl_prims *lines,
     return FGL_OK;
 }
 
+// The home context may already be gone (a host language's collector may
+// free a context before a mesh that refers to it), and any context on the
+// device may still draw the mesh: wait for the whole device, not for one
+// stream, and read nothing through mesh->ctx.
 void fgl_mesh_destroy(fgl_mesh *mesh) {
     if (!mesh) return;
-    (void)hipSetDevice(mesh->ctx->device);
-    (void)hipStreamSynchronize(mesh->ctx->stream);
+    (void)hipSetDevice(mesh->device);
+    (void)hipDeviceSynchronize();
     delete mesh;
+    (void)hipGetLastError();  // the calls above report nothing to the next launch
 }
 
 int fgl_draw_mesh(fgl_ctx *ctx, const fgl_state *state, const fgl_shader *shader, const fgl_mesh *mesh,
@@ -1435,6 +1446,7 @@ constexpr int32_t kNnRangeBlocks = 2048;  // depth min / max partials (enough bl
 
 struct fgl_nn {
     fgl_ctx *home = nullptr;  // device owner (parameters live with it)
+    int device = -1;
     int32_t cin[NN_LAYERS] = {NN_FEAT_C, NN_C, NN_C, NN_C, NN_C, NN_C};
     int32_t cout[NN_LAYERS] = {NN_C, NN_C, NN_C, NN_C, NN_C, NN_OUT_C};
     int64_t woff[NN_LAYERS] = {}, boff[NN_LAYERS] = {};
@@ -1640,6 +1652,7 @@ int fgl_nn_create(fgl_ctx *ctx, uint64_t seed, fgl_nn **out) {
     HIPCHK(ctx, hipSetDevice(ctx->device));
     fgl_nn *nn = new fgl_nn;
     nn->home = ctx;
+    nn->device = ctx->device;
     int64_t off = 0;
     for (int l = 0; l < NN_LAYERS; l++) {
         nn->woff[l] = off;
@@ -1696,8 +1709,11 @@ int fgl_nn_create(fgl_ctx *ctx, uint64_t seed, fgl_nn **out) {
 
 void fgl_nn_destroy(fgl_nn *nn) {
     if (!nn) return;
-    if (nn->home) (void)hipStreamSynchronize(nn->home->stream);
+    // as fgl_mesh_destroy: nothing read through a home context that may be gone
+    (void)hipSetDevice(nn->device);
+    (void)hipDeviceSynchronize();
     delete nn;
+    (void)hipGetLastError();
 }
 
 int fgl_nn_param_count(const fgl_nn *nn, int64_t *n) {
@@ -2115,6 +2131,7 @@ int fgl_mesh_load_stl(fgl_ctx *ctx, const uint8_t *data, int64_t size, fgl_mesh 
         fgl_mesh *m = new (std::nothrow) fgl_mesh;
         if (!m) return FGL_E_NOMEM;
         m->ctx = ctx;
+        m->device = ctx->device;
         m->ntri = count;
         DevBuf file;
         auto fail = [&](int code) {

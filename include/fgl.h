@@ -163,7 +163,11 @@ int fgl_draw_lines(fgl_ctx *ctx, const fgl_state *state, const fgl_shader *shade
                    uint64_t info[2]);
 
 /* Device-resident meshes: Mesh{Triangles, Lines} (mesh.go:9-13) uploaded
- * once, for callers that draw the same mesh many times. */
+ * once, for callers that draw the same mesh many times.  Any context on the
+ * same device may draw a mesh.  The mesh's own calls (fgl_mesh_read, the
+ * preparation calls below) run on the stream of the context it was created
+ * on; once that context is destroyed, fgl_mesh_destroy is the only call left
+ * (it waits for the device and needs no context). */
 int fgl_mesh_create(fgl_ctx *ctx, const fgl_prims *tris, const fgl_prims *lines, fgl_mesh **out);
 void fgl_mesh_destroy(fgl_mesh *mesh);
 /* DrawMesh (context.go:435-439): triangles, then lines. */

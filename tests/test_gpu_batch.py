@@ -205,8 +205,12 @@ def test_overflow_in_a_replay_on_a_shared_context():
     the context must end with its last view's frame."""
     _say("batch: replay overflow on a shared context")
     home = A.NewContext(8, 8)
-    a = prepare(home, M.noise_sphere_prims(ntri=20000, detail=5, amplitude=0.15, seed=5))
-    bm = prepare(home, M.noise_sphere_prims(ntri=3000, detail=4, amplitude=0.15, seed=6))
+    # whole detail-4 spheres (5,120 triangles), shrunk to radius 0.4 so that
+    # their first frames fit the capacities the first draws reserve
+    a = prepare(home, M.noise_sphere_prims(ntri=5120, detail=4, amplitude=0.15, seed=5))
+    bm = prepare(home, M.noise_sphere_prims(ntri=5120, detail=4, amplitude=0.15, seed=6))
+    a.Transform(A.Scale(A.V(0.4, 0.4, 0.4)))
+    bm.Transform(A.Scale(A.V(0.4, 0.4, 0.4)))
     eye0, eye1, eye2 = A.V(3.5, 0.3, 0.2), A.V(-0.4, 0.5, 3.4), A.V(0.2, 3.4, 0.5)
     S = A.NewContext(1024, 1024)
     T = A.NewContext(256, 256)
@@ -219,15 +223,16 @@ def test_overflow_in_a_replay_on_a_shared_context():
     b.Add(T, a, clear_color=M.BACKGROUND)
     b.Run()
     b.Run()
-    a.Transform(A.Scale(A.V(2.2, 2.2, 2.2)))  # in place: no reallocation
+    a.Transform(A.Scale(A.V(2.5, 2.5, 2.5)))  # in place (no reallocation): now fills the 1024x1024 view
     for rep in range(2):
         infos = b.Run()
         i0, _, _ = single(a, 1024, 1024, eye0)
         i1, img1, d1 = single(bm, 1024, 1024, eye1)
         i2, img2, d2 = single(a, 256, 256, eye2)
         # more fragments than the 1024x1024 context's fragment capacity after its
-        # first frames (2 * 3 * (2 * 20000 + 1024 tiles + 65536) = 639,360)
-        assert i0[0] > 700000, i0
+        # first frames: 2 * 3 * (2 * 5120 triangles + tiles + 65536), at most
+        # 466,944 with the smallest (32x16) tiles
+        assert i0[0] > 466944, i0
         assert infos == [i0, i1, i2], (rep, infos)
         assert np.array_equal(S.Image(), img1) and np.array_equal(S.DepthBuffer, d1), rep
         assert np.array_equal(T.Image(), img2) and np.array_equal(T.DepthBuffer, d2), rep

@@ -210,6 +210,7 @@ def test_draw_transform_draw(resident):
     W, H = 128, 96
     mesh = A.Mesh(rand_tris(400, 23, colors=False), None)
     sh = A.NewPhongShader(ORTHO, A.V(0.2, 0.4, 1).Normalize(), A.V(0, 0, 3))
+    sh.ObjectColor = A.HexColor("#468966")  # opaque: the vertices carry no color
     ctx = A.NewContext(W, H)
     ctx.Shader = sh
     if resident:
