@@ -29,7 +29,7 @@ EXPORTED = [
     "fgl_init", "fgl_last_error", "fgl_ctx_create", "fgl_ctx_destroy", "fgl_ctx_size", "fgl_clear_color",
     "fgl_clear_depth", "fgl_texture_create", "fgl_texture_destroy", "fgl_draw_triangles", "fgl_draw_lines",
     "fgl_mesh_create", "fgl_mesh_destroy", "fgl_draw_mesh", "fgl_read_color", "fgl_read_depth",
-    "fgl_write_color", "fgl_write_depth", "fgl_depth_image", "fgl_resolve", "fgl_profile_enable",
+    "fgl_write_color", "fgl_write_depth", "fgl_depth_image", "fgl_depth_image_noalloc_async", "fgl_resolve", "fgl_profile_enable",
     "fgl_profile_read", "fgl_profile_reset", "fgl_ctx_sync", "fgl_draw_stats", "fgl_profile_only",
     "fgl_nn_create", "fgl_nn_destroy", "fgl_nn_param_count", "fgl_nn_get_params", "fgl_nn_set_params",
     "fgl_nn_infer", "fgl_nn_residual", "fgl_nn_grad", "fgl_nn_train_step",
@@ -115,6 +115,7 @@ def load():
         "fgl_write_color": (i32, [vp, ctypes.POINTER(ctypes.c_uint8)]),
         "fgl_write_depth": (i32, [vp, ctypes.POINTER(_d)]),
         "fgl_depth_image": (i32, [vp, ctypes.POINTER(ctypes.c_uint16)]),
+        "fgl_depth_image_noalloc_async": (i32, [vp, vp]),
         "fgl_resolve": (i32, [vp, vp, i32]),
         "fgl_profile_enable": (i32, [vp, i32]),
         "fgl_profile_read": (i32, [vp, ctypes.POINTER(FglKernelTime), i32, ctypes.POINTER(i32)]),

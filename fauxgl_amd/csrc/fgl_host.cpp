@@ -1254,6 +1254,41 @@ int fgl_depth_image(fgl_ctx *ctx, uint16_t *gray16) {
     return FGL_OK;
 }
 
+// fgl_depth_image's kernels with the min/max partials in the library's
+// per-device scratch and the image written to the caller's device buffer:
+// no allocation, no event, no synchronisation.  A pending clear is applied
+// by the fill kernels first, as every reader of the buffers does.
+int fgl_depth_image_noalloc_async(fgl_ctx *ctx, uint16_t *gray16_device) {
+    if (check_ctx(ctx)) return FGL_E_INVALID;
+    if (!gray16_device) {
+        ctx->err = "fgl_depth_image_noalloc_async: null buffer";
+        return FGL_E_INVALID;
+    }
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    double *part = nullptr;
+    HIPCHK(ctx, depth_scratch(ctx->device, &part));
+    const int64_t n = (int64_t)ctx->W * ctx->H;
+    // one partial (min, max) per block of k_depth_minmax, which strides over
+    // any n: the scratch bounds the block count, not the image size
+    const int32_t nb = 512;
+    if (nb > kDepthScratchBlocks) {
+        set_err(ctx->err, "fgl_depth_image_noalloc_async: %d blocks exceed the %d-block scratch", nb,
+                kDepthScratchBlocks);
+        return FGL_E_UNSUPPORTED;
+    }
+    if (ctx->pend_color) {
+        HIPCHK(ctx, launch_fill_color(ctx->color.as<uint32_t>(), n, ctx->clear_color, ctx->stream));
+        ctx->pend_color = false;
+    }
+    if (ctx->pend_depth) {
+        HIPCHK(ctx, launch_fill_depth(ctx->depth.as<double>(), n, ctx->clear_depth, ctx->stream));
+        ctx->pend_depth = false;
+    }
+    HIPCHK(ctx, launch_depth_minmax(ctx->depth.as<double>(), n, part, nb, ctx->stream));
+    HIPCHK(ctx, launch_depth_image(ctx->depth.as<double>(), n, part, nb, gray16_device, ctx->stream));
+    return FGL_OK;
+}
+
 // Tent weights: the linear kernel widened by the scale factor, 8-bit fixed
 // point, as the Bilinear resize the reference pipeline applies after
 // rendering (README.md:109).

@@ -150,6 +150,11 @@ hipError_t launch_resolve(const uint32_t *src, int32_t sw, int32_t sh, uint32_t 
 hipError_t launch_depth_minmax(const double *depth, int64_t n, double *partial, int32_t nblocks, hipStream_t s);
 hipError_t launch_depth_image(const double *depth, int64_t n, const double *partial, int32_t nblocks,
                               uint16_t *out, hipStream_t s);
+// the allocation-free DepthImage's per-device scratch (fgl_misc.hip): room
+// for kDepthScratchBlocks blocks of (min, max); *ptr = this device's copy
+constexpr int32_t kDepthScratchBlocks = 512;
+constexpr int32_t kMaxDevices = 64;
+hipError_t depth_scratch(int32_t device, double **ptr);
 
 // learned anti-aliasing (fgl_nn.hip)
 constexpr int NN_C = 32;         // hidden channels (widest layer)

@@ -1,5 +1,7 @@
 // fgl_misc.hip -- framebuffer fills (deferred clears made concrete), the
 // SSAA resolve, and DepthImage.
+#include <mutex>
+
 #include "fgl_internal.h"
 
 namespace fgl {
@@ -533,6 +535,36 @@ hipError_t launch_depth_image(const double *depth, int64_t n, const double *part
                               hipStream_t s) {
     hipLaunchKernelGGL(k_depth_image, dim3(grid_for(n)), dim3(256), 0, s, depth, n, partial, nblocks, out);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free DepthImage: k_depth_minmax's per-block
+// (min, max) partials, kDepthScratchBlocks blocks (8 KiB).  Every GPU holds
+// its own copy from the library's load until the process ends, and every
+// call on that GPU shares it: two such calls in flight at once on one GPU
+// race on it.  Host code never passes the array's name to a kernel (on the
+// host it is only a stand-in for the symbol); it passes the device address
+// hipGetSymbolAddress returns, looked up once per device.
+
+static __device__ __align__(256) double scratch[2 * kDepthScratchBlocks];
+
+namespace {
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t depth_scratch(int32_t device, double **ptr) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        // the caller made `device` current: the address is that GPU's copy
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = static_cast<double *>(p);
+    }
+    *ptr = g_scratch_ptr[device];
+    return hipSuccess;
 }
 
 }  // namespace fgl

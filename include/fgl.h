@@ -288,6 +288,17 @@ int fgl_write_color(fgl_ctx *ctx, const uint8_t *nrgba);
 int fgl_write_depth(fgl_ctx *ctx, const double *depth);
 /* DepthImage() (context.go:87-117): W*H Gray16 values. */
 int fgl_depth_image(fgl_ctx *ctx, uint16_t *gray16);
+/* No counterpart in the reference: fgl_depth_image for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The same kernels and the same W*H Gray16 values,
+ * written to gray16_device, a device buffer of at least W*H uint16 on ctx's
+ * device.  Nothing is allocated: the min/max partials live in a fixed 8 KiB
+ * scratch array per GPU (512 blocks of min and max, enough for any image
+ * size), and the call only queues work on ctx's stream (fgl_ctx_sync waits
+ * for it).  Every call on one GPU shares that GPU's scratch, so two such
+ * calls in flight at once on one GPU (contexts on different streams or
+ * threads) race; serialise them. */
+int fgl_depth_image_noalloc_async(fgl_ctx *ctx, uint16_t *gray16_device);
 
 /* SSAA resolve: dst (w x h) color = src (w*f x h*f) color downsampled by the
  * integer factor f with `filter`.  FGL_RESOLVE_TENT is a separable 8-bit
