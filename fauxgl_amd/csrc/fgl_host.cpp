@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -1536,6 +1537,16 @@ int nn_buffers(fgl_nn *nn, int32_t H, int32_t W) {
 // layer writes the raw output (nn->out) or, with dst, the final color.
 // keep_feat: the features go through nn->feat (the weight gradient of
 // layer 0 reads them); otherwise layer 0 computes them as it stages tiles.
+// FGL_NN_PAIRS=1 runs inference as three launches of two fused layers each
+// (the earlier kernel), for comparison with the one-launch stream
+bool nn_pairs() {
+    static const bool v = [] {
+        const char *e = std::getenv("FGL_NN_PAIRS");
+        return e && e[0] == '1';
+    }();
+    return v;
+}
+
 int nn_forward(fgl_nn *nn, fgl_ctx *src, fgl_ctx *dst, bool keep_feat) {
     fgl_ctx *c = nn->home;
     int r = materialize(src);
@@ -1561,6 +1572,21 @@ int nn_forward(fgl_nn *nn, fgl_ctx *src, fgl_ctx *dst, bool keep_feat) {
                       })))
         return r;
     if (dst && (r = stream_after(c, dst))) return r;
+    if (!keep_feat && !nn_pairs()) {
+        // inference: all six layers in one launch, streamed down strips with
+        // every activation in LDS
+        const uint16_t *w[NN_LAYERS];
+        const float *b[NN_LAYERS];
+        for (int l = 0; l < NN_LAYERS; l++) {
+            w[l] = nn->wf[l].as<uint16_t>();
+            b[l] = nn->p.as<float>() + nn->boff[l];
+        }
+        return launch(c, "nn_stream", [&] {
+            return launch_nn_stream(dst ? 3 : 1, src->color.as<uint32_t>(), src->depth.as<double>(),
+                                    nn->dpart.as<double>(), kNnRangeBlocks, w, b, H, W,
+                                    dst ? dst->color.p : nn->out.p, c->stream);
+        });
+    }
     if (!keep_feat) {
         // inference: layers fused in pairs, the odd layers' inputs stay in LDS
         static const char *const kPairName[NN_LAYERS / 2] = {"nn_conv01", "nn_conv23", "nn_conv45"};

@@ -204,6 +204,11 @@ hipError_t launch_mesh_smooth(bool threshold_mode, const double *pos, const doub
 // epi_b (0: ReLU -> bf16, 1: f32, 3: final color); in == nullptr: layer A
 // reads the features computed from color, depth and range ({lo, hi}, or
 // nrange > 0 per-block {lo, hi} partials)
+// all six layers in one launch (inference): epi 1 raw output (H x W x 16
+// floats), 3 the final color; w / b the six layers' bf16 weights and biases
+hipError_t launch_nn_stream(int epi, const uint32_t *color, const double *depth, const double *range,
+                            int32_t nrange, const uint16_t *const *w, const float *const *b, int32_t H, int32_t W,
+                            void *out, hipStream_t s);
 hipError_t launch_nn_conv2(int cin_a, int cout_b, int epi_b, const uint16_t *in, const uint16_t *wa, const float *ba,
                            const uint16_t *wb, const float *bb, int32_t H, int32_t W, void *out,
                            const uint32_t *color, const double *depth, const double *range, int32_t nrange,

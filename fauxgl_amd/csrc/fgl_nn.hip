@@ -650,6 +650,336 @@ __global__ __launch_bounds__(512) void k_nn_conv2(const uint16_t *in, const uint
 }
 
 // ---------------------------------------------------------------------------
+// All six layers in one kernel (inference), streamed down vertical strips.
+//
+// A workgroup owns a strip of NS_OW output columns and a band of output
+// rows.  Every layer is computed on NS_CW = NS_OW + 12 columns (the six
+// layers' one-pixel halos; a layer's result is exact on one column fewer at
+// each side than its input), and the band is walked one row per step: at
+// step t the feature stage writes feature row y0 - 6 + t, layer l writes its
+// output row y0 - 8 - 2 l + t.  Layer l's input lives in a ring of four rows
+// in LDS: the three rows its taps read at step t were written at steps
+// before, and the fourth slot takes the row its producer writes in the same
+// step, so one barrier per step orders everything.  No activation goes to
+// HBM: the kernel reads the 1x color and depth and writes the final color
+// (or the raw output).
+//
+// Each wave owns one stage and keeps that layer's weight fragments in
+// registers: waves 0-3 layers 1-4 (a whole row: NS_NB blocks of 16 pixels,
+// both 16-channel output tiles), wave 4 layer 0, waves 5 and 7 half of
+// layer 5's blocks each, wave 6 the features.  Rows outside the image are
+// written as zeros and columns outside it are zeroed, which is the zero
+// padding every layer sees; the ring columns left and right of the strip
+// stay zero.  Layers 1-5 accumulate bias, then taps 0..8 (k_nn_conv's
+// order, the same bits).  Layer 0 has 8 used input channels: one MFMA
+// covers four taps (lane group kg holds tap 4 g + kg), three MFMAs a tile,
+// so its sums are grouped differently from k_nn_conv's (within rounding).
+//
+// Ring pixel p holds local column p - 1.  A row is four planes, one per
+// group of 8 channels, of 16 bytes per pixel: the 16 lanes that read (or
+// write) one channel group of 16 consecutive pixels touch 256 contiguous
+// bytes, whatever the pixel offset, so no access has a bank conflict.
+
+constexpr int NS_NB = 6, NS_CW = 16 * NS_NB, NS_OW = NS_CW - 12, NS_RW = NS_CW + 2, NS_ARW = NS_RW;
+constexpr int NS_ACT_ROW = NS_ARW * 64, NS_FEAT_ROW = NS_RW * 16;
+constexpr int NS_ACT_RING = 4 * NS_ACT_ROW, NS_FEAT_RING = 4 * NS_FEAT_ROW;
+constexpr int NS_COLOR_ROW = NS_CW * 4, NS_COLOR_RING = 16 * NS_COLOR_ROW;
+constexpr int NS_LDS = 5 * NS_ACT_RING + NS_FEAT_RING + NS_COLOR_RING;
+constexpr int NS_FEAT_PER_LANE = (NS_CW + 63) / 64;
+
+FGL_D int ns_chunk(int p, int kg) { return (kg * NS_ARW + p) * 16; }
+
+// The nine tap fragments of block b from the three ring rows (tap =
+// 3 dy + dx reads pixel 16 b + pm + dx of row dy).
+FGL_D void ns_load9(const uint8_t *const (&rows)[3], int b, int pm, int kg, bf16x8 (&f)[9]) {
+#pragma unroll
+    for (int dy = 0; dy < 3; dy++)
+#pragma unroll
+        for (int dx = 0; dx < 3; dx++)
+            f[3 * dy + dx] = *reinterpret_cast<const bf16x8 *>(rows[dy] + ns_chunk(16 * b + pm + dx, kg));
+}
+
+// Scheduling of one block: the next block's nine fragment loads spread
+// between this block's MFMAs (one load, then two MFMAs), so that their LDS
+// latency hides behind the matrix work.
+FGL_D void ns_interleave() {
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // DS read
+        __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);  // MFMA
+    }
+}
+
+template <int EPI>
+__global__ __launch_bounds__(512) void k_nn_stream(const uint32_t *color, const double *depth, const double *range,
+                                                   int32_t nrange, const uint16_t *w0, const uint16_t *w1,
+                                                   const uint16_t *w2, const uint16_t *w3, const uint16_t *w4,
+                                                   const uint16_t *w5, const float *b0, const float *b1,
+                                                   const float *b2, const float *b3, const float *b4, const float *b5,
+                                                   int32_t H, int32_t W, int32_t band_h, int32_t nstrips, void *out) {
+    static_assert(EPI == 1 || EPI == 3, "EPI");
+    static_assert(NS_LDS <= 160 * 1024, "rings fit the LDS");
+    __shared__ __attribute__((aligned(16))) uint8_t lds[NS_LDS];
+    // ring k (k = 1..5) is layer k's input; the feature ring is layer 0's
+    uint8_t *const feat_ring = lds + 5 * NS_ACT_RING;
+    uint8_t *const color_ring = feat_ring + NS_FEAT_RING;  // input colors, 16 rows (EPI 3)
+    auto act_ring = [&](int k) { return lds + (k - 1) * NS_ACT_RING; };
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int pm = lane & 15, kg = lane >> 4;
+    const int strip = blockIdx.x % nstrips, band = blockIdx.x / nstrips;
+    const int x0 = strip * NS_OW - 6;  // global x of local column 0
+    const int y0 = band * band_h, y1 = min(H, y0 + band_h);
+    if (y0 >= H) return;
+    const int nsteps = (y1 - y0) + 18;
+
+    for (int q = tid; q < NS_LDS / 16; q += 512) reinterpret_cast<uint4 *>(lds)[q] = make_uint4(0, 0, 0, 0);
+
+    // Every stage runs its own loop of nsteps steps with one barrier each, so
+    // that the registers of one stage (a layer's weights, the feature
+    // stage's prefetched rows) are not live in another's code.
+    const int layer = wave < 4 ? wave + 1 : (wave == 4 ? 0 : (wave == 6 ? -1 : 5));
+
+    if (layer == -1) {
+        // features: depth range, then the raw samples (color and five depths
+        // per pixel) of the rows three steps ahead, in four register sets used
+        // in rotation, so that three rows' loads are in flight while a fourth
+        // is turned into features
+        double lo = kMaxF64, hi = -kMaxF64;
+        for (int q = lane; q < nrange; q += 64) {
+            const double l = range[2 * q], h = range[2 * q + 1];
+            lo = l < lo ? l : lo;
+            hi = h > hi ? h : hi;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const double l = __shfl_xor(lo, off), h = __shfl_xor(hi, off);
+            lo = l < lo ? l : lo;
+            hi = h > hi ? h : hi;
+        }
+        const double rlo = lo, rhi = hi;
+        FeatIn raw0[NS_FEAT_PER_LANE], raw1[NS_FEAT_PER_LANE], raw2[NS_FEAT_PER_LANE], raw3[NS_FEAT_PER_LANE];
+        // unconditional (rows past the band load clamped samples nobody
+        // uses): a load that may be skipped would merge old and new values of
+        // a set, and the copies that merge costs wait for the loads
+        auto fetch = [&](int fy, FeatIn (&raw)[NS_FEAT_PER_LANE]) {
+#pragma unroll
+            for (int i = 0; i < NS_FEAT_PER_LANE; i++) {
+                const int gx = x0 + lane + 64 * i;
+                const int cy = fy < 0 ? 0 : (fy >= H ? H - 1 : fy), cx = gx < 0 ? 0 : (gx >= W ? W - 1 : gx);
+                raw[i] = feat_load(color, depth, H, W, cx, cy);
+            }
+        };
+        // row fy into the feature ring, its colors into the color ring (layer
+        // 5 reads them twelve steps later for the residual)
+        auto put = [&](int fy, const FeatIn (&raw)[NS_FEAT_PER_LANE]) {
+            if (fy >= y1 + 6) return;
+            uint8_t *row = feat_ring + (fy & 3) * NS_FEAT_ROW;
+            uint32_t *crow = reinterpret_cast<uint32_t *>(color_ring + (fy & 15) * NS_COLOR_ROW);
+#pragma unroll
+            for (int i = 0; i < NS_FEAT_PER_LANE; i++) {
+                const int c = lane + 64 * i;
+                if (c >= NS_CW) continue;
+                const int gx = x0 + c;
+                uint4 f = make_uint4(0, 0, 0, 0);
+                if (fy >= 0 && fy < H && gx >= 0 && gx < W) f = feat_eval(raw[i], rlo, rhi);
+                *reinterpret_cast<uint4 *>(row + (c + 1) * 16) = f;
+                if (EPI == 3) crow[c] = raw[i].c;
+            }
+        };
+        fetch(y0 - 6, raw0);
+        fetch(y0 - 5, raw1);
+        fetch(y0 - 4, raw2);
+        __syncthreads();  // rings zeroed
+        // four steps per iteration, each set in a fixed role: step t loads
+        // the row of step t + 3 into the set step t - 1 used
+        for (int t = 0; t < nsteps; t += 4) {
+            const int fy = y0 - 6 + t;
+            fetch(fy + 3, raw3);
+            put(fy, raw0);
+            __syncthreads();
+            fetch(fy + 4, raw0);
+            put(fy + 1, raw1);
+            if (t + 1 < nsteps) __syncthreads();
+            fetch(fy + 5, raw1);
+            put(fy + 2, raw2);
+            if (t + 2 < nsteps) __syncthreads();
+            fetch(fy + 6, raw2);
+            put(fy + 3, raw3);
+            if (t + 3 < nsteps) __syncthreads();
+        }
+        return;
+    }
+
+    if (layer == 0) {
+        // 8 used input channels: A[g][t] holds w[4 g + kg][co(pm, t)][0 .. 7]
+        bf16x8 A[3][2];
+        f32x4 bias[2];
+#pragma unroll
+        for (int g = 0; g < 3; g++)
+#pragma unroll
+            for (int t2 = 0; t2 < 2; t2++) {
+                const int co = 8 * (pm >> 2) + 4 * t2 + (pm & 3), tap = 4 * g + kg;
+                A[g][t2] = tap < 9 ? *reinterpret_cast<const bf16x8 *>(w0 + (tap * 32 + co) * 16)
+                                   : bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+            }
+#pragma unroll
+        for (int t2 = 0; t2 < 2; t2++)
+            bias[t2] = f32x4{b0[8 * kg + 4 * t2], b0[8 * kg + 4 * t2 + 1], b0[8 * kg + 4 * t2 + 2],
+                             b0[8 * kg + 4 * t2 + 3]};
+        __syncthreads();  // rings zeroed
+        for (int t = 0; t < nsteps; t++) {
+            const int r = y0 - 8 + t;
+            if (r >= y0 - 5 && r < y1 + 5) {
+                uint8_t *dst = act_ring(1) + (r & 3) * NS_ACT_ROW;
+                if (r < 0 || r >= H) {
+                    for (int q = lane; q < NS_CW * 4; q += 64)
+                        *reinterpret_cast<uint4 *>(dst + ns_chunk(1 + (q >> 2), q & 3)) = make_uint4(0, 0, 0, 0);
+                } else {
+#pragma unroll
+                    for (int b = 0; b < NS_NB; b++) {
+                        bf16x8 f[3];
+#pragma unroll
+                        for (int g = 0; g < 3; g++) {
+                            const int tap = 4 * g + kg;
+                            const int dy = tap / 3, dx = tap - 3 * (tap / 3);
+                            f[g] = tap < 9 ? *reinterpret_cast<const bf16x8 *>(
+                                                 feat_ring + ((r - 1 + dy) & 3) * NS_FEAT_ROW + (16 * b + pm + dx) * 16)
+                                           : bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+                        }
+                        f32x4 acc[2] = {bias[0], bias[1]};
+#pragma unroll
+                        for (int g = 0; g < 3; g++)
+#pragma unroll
+                            for (int t2 = 0; t2 < 2; t2++)
+                                acc[t2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[g][t2], f[g], acc[t2], 0, 0, 0);
+                        const int c = 16 * b + pm, gx = x0 + c;
+                        uint4 o = make_uint4(0, 0, 0, 0);
+                        if (gx >= 0 && gx < W)
+                            o = make_uint4(pack_bf2(relu(acc[0][0]), relu(acc[0][1])),
+                                           pack_bf2(relu(acc[0][2]), relu(acc[0][3])),
+                                           pack_bf2(relu(acc[1][0]), relu(acc[1][1])),
+                                           pack_bf2(relu(acc[1][2]), relu(acc[1][3])));
+                        *reinterpret_cast<uint4 *>(dst + ns_chunk(c + 1, kg)) = o;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        return;
+    }
+
+    if (layer <= 4) {
+        // layers 1-4: lane (pm, kg) holds w[tap][co(pm, t)][8 kg .. 8 kg + 7]
+        const uint16_t *wl = layer == 1 ? w1 : layer == 2 ? w2 : layer == 3 ? w3 : w4;
+        const float *bl = layer == 1 ? b1 : layer == 2 ? b2 : layer == 3 ? b3 : b4;
+        bf16x8 A[9][2];
+        f32x4 bias[2];
+#pragma unroll
+        for (int tap = 0; tap < 9; tap++)
+#pragma unroll
+            for (int t2 = 0; t2 < 2; t2++) {
+                const int co = 8 * (pm >> 2) + 4 * t2 + (pm & 3);
+                A[tap][t2] = *reinterpret_cast<const bf16x8 *>(wl + (tap * 32 + co) * 32 + 8 * kg);
+            }
+#pragma unroll
+        for (int t2 = 0; t2 < 2; t2++)
+            bias[t2] = f32x4{bl[8 * kg + 4 * t2], bl[8 * kg + 4 * t2 + 1], bl[8 * kg + 4 * t2 + 2],
+                             bl[8 * kg + 4 * t2 + 3]};
+        const uint8_t *src = act_ring(layer);
+        uint8_t *const dring = act_ring(layer + 1);
+        __syncthreads();  // rings zeroed
+        for (int t = 0; t < nsteps; t++) {
+            const int r = y0 - 8 - 2 * layer + t;
+            if (r >= y0 - (5 - layer) && r < y1 + (5 - layer)) {
+                uint8_t *dst = dring + (r & 3) * NS_ACT_ROW;
+                if (r < 0 || r >= H) {
+                    for (int q = lane; q < NS_CW * 4; q += 64)
+                        *reinterpret_cast<uint4 *>(dst + ns_chunk(1 + (q >> 2), q & 3)) = make_uint4(0, 0, 0, 0);
+                } else {
+                    const uint8_t *const rows[3] = {src + ((r - 1) & 3) * NS_ACT_ROW, src + (r & 3) * NS_ACT_ROW,
+                                                    src + ((r + 1) & 3) * NS_ACT_ROW};
+                    // double-buffered fragments: block b + 1's loads go out
+                    // between block b's MFMAs
+                    bf16x8 fa[9], fb[9];
+                    ns_load9(rows, 0, pm, kg, fa);
+#pragma unroll
+                    for (int b = 0; b < NS_NB; b++) {
+                        bf16x8(&cur)[9] = (b & 1) ? fb : fa;
+                        bf16x8(&nxt)[9] = (b & 1) ? fa : fb;
+                        if (b + 1 < NS_NB) ns_load9(rows, b + 1, pm, kg, nxt);
+                        f32x4 acc[2] = {bias[0], bias[1]};
+#pragma unroll
+                        for (int tap = 0; tap < 9; tap++)
+#pragma unroll
+                            for (int t2 = 0; t2 < 2; t2++)
+                                acc[t2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[tap][t2], cur[tap], acc[t2], 0, 0, 0);
+                        if (b + 1 < NS_NB) ns_interleave();
+                        const int c = 16 * b + pm, gx = x0 + c;
+                        uint4 o = make_uint4(0, 0, 0, 0);
+                        if (gx >= 0 && gx < W)
+                            o = make_uint4(pack_bf2(relu(acc[0][0]), relu(acc[0][1])),
+                                           pack_bf2(relu(acc[0][2]), relu(acc[0][3])),
+                                           pack_bf2(relu(acc[1][0]), relu(acc[1][1])),
+                                           pack_bf2(relu(acc[1][2]), relu(acc[1][3])));
+                        *reinterpret_cast<uint4 *>(dst + ns_chunk(c + 1, kg)) = o;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        return;
+    }
+
+    // layer 5 (waves 5 and 7: blocks blk0 .. blk0 + 2): 16 output channels,
+    // lane (pm, kg) holds w[tap][pm][8 kg .. 8 kg + 7]
+    const int blk0 = wave == 7 ? NS_NB / 2 : 0;
+    bf16x8 A[9];
+#pragma unroll
+    for (int tap = 0; tap < 9; tap++) A[tap] = *reinterpret_cast<const bf16x8 *>(w5 + (tap * 16 + pm) * 32 + 8 * kg);
+    const f32x4 bias = f32x4{b5[4 * kg], b5[4 * kg + 1], b5[4 * kg + 2], b5[4 * kg + 3]};
+    const uint8_t *src = act_ring(5);
+    __syncthreads();  // rings zeroed
+    for (int t = 0; t < nsteps; t++) {
+        const int r = y0 - 18 + t;
+        if (r >= y0 && r < y1) {
+            const uint8_t *const rows[3] = {src + ((r - 1) & 3) * NS_ACT_ROW, src + (r & 3) * NS_ACT_ROW,
+                                            src + ((r + 1) & 3) * NS_ACT_ROW};
+            const uint32_t *crow = reinterpret_cast<const uint32_t *>(color_ring + (r & 15) * NS_COLOR_ROW);
+            bf16x8 fa[9], fb[9];
+            ns_load9(rows, blk0, pm, kg, fa);
+#pragma unroll
+            for (int k = 0; k < NS_NB / 2; k++) {
+                const int b = blk0 + k;
+                bf16x8(&cur)[9] = (k & 1) ? fb : fa;
+                bf16x8(&nxt)[9] = (k & 1) ? fa : fb;
+                if (k + 1 < NS_NB / 2) ns_load9(rows, b + 1, pm, kg, nxt);
+                f32x4 acc = bias;
+#pragma unroll
+                for (int tap = 0; tap < 9; tap++) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[tap], cur[tap], acc, 0, 0, 0);
+                const int c = 16 * b + pm, x = x0 + c;
+                if (c < 6 || c >= 6 + NS_OW || x >= W) continue;
+                const int64_t px = (int64_t)r * W + x;
+                if (EPI == 1) {
+                    *reinterpret_cast<float4 *>(static_cast<float *>(out) + px * 16 + 4 * kg) =
+                        make_float4(acc[0], acc[1], acc[2], acc[3]);
+                } else if (kg == 0) {
+                    const uint32_t cc = crow[c];
+                    uint32_t o = cc & 0xff000000u;
+#pragma unroll
+                    for (int ch = 0; ch < 3; ch++) {
+                        float v = (float)((cc >> (8 * ch)) & 0xff) * (1.0f / 255.0f) + acc[ch];
+                        v = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+                        o |= (uint32_t)(v * 255.0f + 0.5f) << (8 * ch);
+                    }
+                    static_cast<uint32_t *>(out)[px] = o;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // weight gradient: dw[tap][co][ci] = sum over pixels p of
 // x[p + d(tap)][ci] * dy[p][co]; each wave sums NN_DW_ROWS rows of 16
 // pixels and writes its own partial (9 * Cout * Cin floats) plus the bias
@@ -845,6 +1175,35 @@ hipError_t launch_nn_conv2(int cin_a, int cout_b, int epi_b, const uint16_t *in,
     FGL_NN_CONV2(32, 16, 1, false)
 #undef FGL_NN_CONV2
     return hipErrorInvalidValue;
+}
+void nn_stream_grid(int32_t H, int32_t W, int32_t *band_h, int32_t *nstrips, int32_t *nbands) {
+    // one workgroup per CU (the rings take most of its LDS): about 256
+    // strips x bands, bands at least 16 rows so that the 18-step pipeline
+    // fill stays a minor part of each band
+    *nstrips = (W + NS_OW - 1) / NS_OW;
+    int32_t nb = 256 / *nstrips;
+    if (nb < 1) nb = 1;
+    int32_t bh = (H + nb - 1) / nb;
+    if (bh < 16) bh = 16;
+    *band_h = bh;
+    *nbands = (H + bh - 1) / bh;
+}
+hipError_t launch_nn_stream(int epi, const uint32_t *color, const double *depth, const double *range,
+                            int32_t nrange, const uint16_t *const *w, const float *const *b, int32_t H, int32_t W,
+                            void *out, hipStream_t s) {
+    int32_t band_h, nstrips, nbands;
+    nn_stream_grid(H, W, &band_h, &nstrips, &nbands);
+    const dim3 grid((unsigned)(nstrips * nbands));
+    if (epi == 1)
+        hipLaunchKernelGGL(k_nn_stream<1>, grid, dim3(512), 0, s, color, depth, range, nrange, w[0], w[1], w[2], w[3],
+                           w[4], w[5], b[0], b[1], b[2], b[3], b[4], b[5], H, W, band_h, nstrips, out);
+    else if (epi == 3)
+        hipLaunchKernelGGL(k_nn_stream<3>, grid, dim3(512), 0, s, color, depth, range, nrange, w[0], w[1], w[2], w[3],
+                           w[4], w[5], b[0], b[1], b[2], b[3], b[4], b[5], H, W, band_h, nstrips, out);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+    return hipGetLastError();
 }
 int64_t nn_dw_waves(int32_t H, int32_t W) { return (int64_t)((W + 15) / 16) * ((H + NN_DW_ROWS - 1) / NN_DW_ROWS); }
 hipError_t launch_nn_dw(const uint16_t *x, int32_t Cin, const uint16_t *dy, int32_t Cout, int32_t H, int32_t W,
