@@ -212,6 +212,145 @@ def batch_views(steps, warmup):
     }
 
 
+def scene_setup(kind):
+    """(W, H, Mesh, shader, state overrides, description) of the two other
+    single-frame workloads: config 1, NewSphere(5) + the offset half cube at
+    512x512 with Phong (the reference's own example scene); config 3, the
+    200,000-triangle textured soup at 1920x1080 with alpha blending and depth
+    bias 1e-4 (order-dependent blending)."""
+    from fauxgl_amd import meshes as M
+    if kind == "config1":
+        W = H = 512
+        return (W, H, M.sphere_and_cube_mesh(), M.phong_shader(W, H), {},
+                "NewSphere(5) (20,480 triangles) + NewCube scaled 0.5 at (0.8, 0, 0), Phong, 512x512")
+    W, H = 1920, 1080
+    mesh, shader = M.soup_scene(W, H)
+    return (W, H, mesh, shader, {"DepthBias": 1e-4},
+            "200,000-triangle random soup (seed 2), TextureShader on a 1024x1024 NRGBA value-noise texture "
+            "(alpha 0.2-0.8), alpha blending, depth bias 1e-4, 1920x1080")
+
+
+def scene_frames(kind, steps, warmup):
+    """GPU frames/s of config 1 or config 3: clear + DrawMesh of the resident
+    mesh per frame, with the kernel table of an untimed profiled pass and the
+    dominant kernel timed live in the timed frames."""
+    from fauxgl_amd import api as A
+    from fauxgl_amd import meshes as M
+    W, H, mesh, shader, state, desc = scene_setup(kind)
+    ctx = A.NewContext(W, H)
+    ctx.Shader = shader
+    for k, v in state.items():
+        setattr(ctx, k, v)
+    dm = ctx.Upload(mesh)
+    tile = ctx.Autotune(dm, reps=3)
+
+    def frame():
+        ctx.ClearColorBufferWith(M.BACKGROUND)
+        ctx.ClearDepthBuffer()
+        return ctx.DrawMesh(dm)
+
+    for _ in range(warmup):
+        info = frame()
+    ctx.Sync()
+    ctx.ProfileReset()
+    ctx.ProfileEnable(True)
+    for _ in range(PROFILE_FRAMES):
+        info = frame()
+    ctx.Sync()
+    table = ctx.ProfileRead()
+    dom = max(table, key=lambda k: table[k][1])
+    ctx.ProfileReset()
+    ctx.ProfileOnly(dom)
+    ctx.Sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        info = frame()
+    ctx.Sync()
+    ms = (time.perf_counter() - t0) * 1000.0 / steps
+    live = ctx.ProfileRead().get(dom, (0, 0.0))
+    ctx.ProfileEnable(False)
+    ctx.ProfileOnly(None)
+    stats = ctx.DrawStats()
+    dm.close()
+    ntri = len(mesh.Triangles)
+    # algorithmic bytes: the vertex payload the shader reads once (position +
+    # normal for Phong, position + texture coordinates for the texture shader)
+    # and 12 B per framebuffer pixel written once; plus, for the soup, the
+    # texture read once
+    per_vertex = 48 if kind == "config1" else 40
+    nbytes = 3 * ntri * per_vertex + 12 * W * H + (1024 * 1024 * 4 if kind == "config3" else 0)
+    dom_ms = live[1] / max(live[0], 1)
+    kr = kernel_roofline(dom, dom_ms, stats, W, H, 1, "phong" if kind == "config1" else "texture")
+    return {
+        "workload": desc, "triangles": ntri, "width": W, "height": H, "tile": f"{tile[0]}x{tile[1]} (autotuned)",
+        "ms_per_frame": round(ms, 4), "frames_per_s": round(1000.0 / ms, 2),
+        "rasterize_info": list(info), "draw_stats": stats,
+        "kernels": {k: {"launches": v[0] // PROFILE_FRAMES, "ms_per_frame": round(v[1] / PROFILE_FRAMES, 4)}
+                    for k, v in sorted(table.items(), key=lambda kv: -kv[1][1])},
+        "dominant_kernel": dom, "dominant_kernel_avg_ms": round(dom_ms, 4), "dominant_kernel_roofline": kr,
+        "roofline_frame": {"bytes": nbytes, "t_roof_ms": round(nbytes / (HBM_PEAK_GBS * 1e9) * 1000, 4),
+                           "frac": round(nbytes / (HBM_PEAK_GBS * 1e9) * 1000 / ms, 4)},
+    }
+
+
+def cpu_scene_frames(kind, frames, threads):
+    """The oracle's all-core mirror of DrawMesh on config 1 or 3 (median of
+    `frames` frames after one warm-up)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from fauxgl_amd import api as A
+    from fauxgl_amd import meshes as M
+    W, H, mesh, shader, state, _ = scene_setup(kind)
+    ctx = O.Context(W, H)
+    ctx.set_state(O.make_state(depth_bias=state.get("DepthBias", 0.0)))
+    if isinstance(shader, A.TextureShader):
+        tex = O.Texture(shader.Texture.Image, O.TEX_NRGBA8)
+        osh = O.make_shader(O.SHADER_TEXTURE, shader.Matrix, texture=tex)
+    else:
+        osh = O.make_shader(O.SHADER_PHONG, shader.Matrix, light_dir=shader.LightDirection,
+                            camera_pos=shader.CameraPosition, object_color=shader.ObjectColor)
+    ctx.set_shader(osh)
+    om = O.Mesh(tris=mesh.Triangles.arrays())
+    times = []
+    info = None
+    for i in range(frames + 1):
+        t = time.perf_counter()
+        ctx.clear_color(tuple(M.BACKGROUND))
+        ctx.clear_depth(1.7976931348623157e308)
+        info = ctx.draw_mesh(om, threads)
+        if i > 0:
+            times.append(time.perf_counter() - t)
+    return float(np.median(times)), info
+
+
+def readback_frames(mesh, W, H, steps, warmup):
+    """The 16x frame as a caller that wants the pixels on the host sees it:
+    clear, draw, tent resolve to 1920x1080 and Image() (8.3 MB over PCIe)."""
+    from fauxgl_amd import api as A
+    from fauxgl_amd import meshes as M
+    big = A.NewContext(4 * W, 4 * H)
+    out = A.NewContext(W, H)
+    big.Shader = M.phong_shader(W, H)
+    dm = big.Upload(mesh)
+    big.Autotune(dm, reps=2)
+
+    def frame():
+        big.ClearColorBufferWith(M.BACKGROUND)
+        big.ClearDepthBuffer()
+        big.DrawMesh(dm)
+        big.Resolve(out, "tent")
+        return out.Image()
+
+    for _ in range(warmup):
+        frame()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        img = frame()
+    ms = (time.perf_counter() - t0) * 1000.0 / steps
+    dm.close()
+    return ms, img
+
+
 def psnr(a, b):
     d = a[..., :3].astype(np.float64) - b[..., :3].astype(np.float64)
     mse = float(np.mean(d * d))
@@ -268,15 +407,25 @@ def algorithmic(W, H, ssaa, ntri):
     return 3 * ntri * 48 + 12 * npix
 
 
-def kernel_roofline(name, avg_ms, stats, W, H, ssaa):
+def kernel_roofline(name, avg_ms, stats, W, H, ssaa, shader="phong"):
     """Roofline of one kernel from its algorithmic work per launch."""
     s = int(round(ssaa ** 0.5))
     npix = W * s * H * s
     t = avg_ms / 1000.0
+    if name == "shade" and shader == "texture":
+        # the texture shader's FP64 work per fragment is a few dozen
+        # operations; its bound is taken as the bytes it must move: the spans
+        # read and the 16-byte fragments written
+        nbytes = 32 * stats["spans"] + 16 * stats["fragments"]
+        ach = nbytes / t / 1e9
+        return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4),
+                "work": f"32 B x {stats['spans']} spans + 16 B x {stats['fragments']} fragments = {nbytes} B per launch"}
     if name == "shade":
         flops = PHONG_OPS_PER_FRAGMENT * stats["fragments"]
         ach = flops / t / 1e12
-        return {"bound": "mfma", "pipe": "FP64 VALU (no matrix-shaped work; FP64 peak)", "achieved": round(ach, 3),
+        return {"bound": "fp64-valu", "pipe": "FP64 VALU (no matrix-shaped work; the FP64 vector peak)",
+                "achieved": round(ach, 3),
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / FP64_PEAK_TFLOPS, 4),
                 "work": f"{PHONG_OPS_PER_FRAGMENT} FP64 ops x {stats['fragments']} fragments = {flops} per launch"}
     if name == "rop":
@@ -318,13 +467,21 @@ def main():
     ap.add_argument("--cpu-frames-1t", type=int, default=3, help="timed single-thread CPU frames at 1x (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = CPUs in this process's affinity mask")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched multi-view line")
-    ap.add_argument("--only", choices=["16x", "1x", "nn"], default=None,
-                    help="time one leg only (16x, 1x, or 1x + neural AA) and print its kernel table (for profiler runs)")
+    ap.add_argument("--only", choices=["16x", "1x", "nn", "batch", "config1", "config3"], default=None,
+                    help="time one leg only (16x, 1x, 1x + neural AA, or the batched views) and print its "
+                         "kernel table (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.only == "batch":
+        print(json.dumps({"workload": "batched views", **batch_views(args.steps, args.warmup)}), flush=True)
+        return
+    if args.only in ("config1", "config3"):
+        print(json.dumps(scene_frames(args.only, args.steps, args.warmup)), flush=True)
+        return
 
     W, H, SSAA = 1920, 1080, 16
     mesh = build_scene()
@@ -374,6 +531,17 @@ def main():
         log(f"[bench] batched views: {batched['ms_per_step_graph']} ms per 256 views as a graph, "
             f"{batched['ms_per_step_view_by_view']} ms view by view")
 
+    msr, imgr = readback_frames(mesh, W, H, args.steps, args.warmup)
+    log(f"[bench] 16x SSAA with readback: {msr:.3f} ms/frame")
+    readback = {"ms_per_frame": round(msr, 4), "frames_per_s": round(1000.0 / msr, 3),
+                "same_image": bool(np.array_equal(imgr, img16)),
+                "note": "clear + draw + tent resolve + Image() of the 1920x1080 NRGBA frame to host memory"}
+    others = {}
+    for kind in ("config1", "config3"):
+        others[kind] = scene_frames(kind, args.steps, args.warmup)
+        log(f"[bench] {kind}: {others[kind]['ms_per_frame']:.3f} ms/frame, dominant "
+            f"{others[kind]['dominant_kernel']}")
+
     # dominant kernel of the headline step and its roofline
     per_kernel = {k: {"launches": v[0], "avg_ms": v[1] / max(v[0], 1), "ms_per_frame": v[1] / PROFILE_FRAMES}
                   for k, v in prof16.items()}
@@ -409,6 +577,15 @@ def main():
             s1, _ = cpu_frames(mesh, W, H, 1, args.cpu_frames_1t, 1)
             log(f"[bench] CPU 1x single thread: {s1 * 1000:.1f} ms/frame")
             cpu["ms_per_frame_1x_single_thread"] = round(s1 * 1000, 2)
+        for kind in ("config1", "config3"):
+            c, cinfo = cpu_scene_frames(kind, args.cpu_frames, threads)
+            log(f"[bench] CPU {kind}: {c * 1000:.1f} ms/frame on {threads} threads")
+            others[kind]["cpu_baseline"] = {
+                "value": round(1.0 / c, 3), "unit": "frames/s", "cores": threads, "kind": "port",
+                "sample": f"median of {args.cpu_frames} frames after 1 warm-up, the oracle's all-core mirror",
+                "ms_per_frame": round(c * 1000, 2),
+                "total_pixels_match_gpu": cinfo[0] == others[kind]["rasterize_info"][0]}
+            others[kind]["speedup_vs_cpu"] = round(c * 1000 / others[kind]["ms_per_frame"], 1)
 
     fps16 = 1000.0 / ms16
     line = {
@@ -442,6 +619,9 @@ def main():
         "cpu_baseline": cpu,
         "neural_aa": neural_aa,
         "batched_views": batched,
+        "readback_16x": readback,
+        "config1": others["config1"],
+        "config3": others["config3"],
     }
     if cpu is not None:
         line["speedup_vs_cpu_16x"] = round(fps16 / cpu["value"], 1)
