@@ -94,6 +94,11 @@ struct fgl_ctx {
     bool pend_color = false, pend_depth = false;
     uint32_t clear_color = 0;
     double clear_depth = 0;
+    // blocks whose pending clear a draw left unwritten (CB_COLOR / CB_DEPTH
+    // per CB_W x CB_H block); cblk_any: some block may be marked
+    DevBuf cblk;
+    int32_t cbw = 0, cbh = 0;
+    bool cblk_any = false;
     std::string err;
     std::map<int64_t, Tex> tex;
     int64_t next_tex = 1;
@@ -270,19 +275,39 @@ int stream_after(fgl_ctx *waiter, fgl_ctx *signaller) {
     return FGL_OK;
 }
 
-int materialize(fgl_ctx *c) {
+// Writes the pending clears and the blocks a draw left at the clear values
+// into the channels `chans` (CB_COLOR / CB_DEPTH) of the framebuffer, for a
+// reader of those channels.
+int materialize(fgl_ctx *c, uint8_t chans = CB_COLOR | CB_DEPTH) {
     int64_t n = (int64_t)c->W * c->H;
-    if (c->pend_color) {
+    if (c->pend_color && (chans & CB_COLOR)) {
         int r = launch(c, "fill_color", [&] { return launch_fill_color(c->color.as<uint32_t>(), n, c->clear_color, c->stream); });
         if (r) return r;
         c->pend_color = false;
     }
-    if (c->pend_depth) {
+    if (c->pend_depth && (chans & CB_DEPTH)) {
         int r = launch(c, "fill_depth", [&] { return launch_fill_depth(c->depth.as<double>(), n, c->clear_depth, c->stream); });
         if (r) return r;
         c->pend_depth = false;
     }
+    if (c->cblk_any) {  // the blocks a draw left at their clear values
+        int r = launch(c, "fill_blocks", [&] {
+            return launch_cblk_fill(c->color.as<uint32_t>(), c->depth.as<double>(), c->W, c->H, c->cblk.as<uint8_t>(),
+                                    c->cbw, c->cbh, c->clear_color, c->clear_depth, chans, c->stream);
+        });
+        if (r) return r;
+        if (chans == (CB_COLOR | CB_DEPTH)) c->cblk_any = false;  // else marks of the other channel may remain
+    }
     return FGL_OK;
+}
+
+// The channel `bits` of the whole framebuffer was just overwritten (a host
+// write, a resolve or network output): no block of it holds a deferred clear.
+int cblk_drop(fgl_ctx *c, uint8_t bits) {
+    if (!c->cblk_any) return FGL_OK;
+    return launch(c, "drop_blocks", [&] {
+        return launch_cblk_drop(c->cblk.as<uint8_t>(), (int64_t)c->cbw * c->cbh, bits, c->stream);
+    });
 }
 
 int key_bits_for(int32_t ntiles) {
@@ -459,6 +484,8 @@ void draw_setup(fgl_ctx *c, const fgl_state *st, const TileOps &T, int kind, con
     a.clear_depth_pending = c->pend_depth;
     a.clear_color = c->clear_color;
     a.clear_depth = c->clear_depth;
+    a.cblk = c->cblk.as<uint8_t>();
+    a.cbw = c->cbw;
 }
 
 // One pass of a draw: the kernel sequence on the context's stream.
@@ -625,6 +652,9 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
                 info[0] = h.total;
                 info[1] = updated_pixels(h);
             }
+            // the ROP marked the blocks of fragment-free tiles with the
+            // clears it consumed
+            if (c->pend_color || c->pend_depth) c->cblk_any = true;
             c->pend_color = false;
             c->pend_depth = false;
             return FGL_OK;
@@ -739,15 +769,18 @@ int fgl_ctx_create(int32_t width, int32_t height, fgl_ctx **out) {
     c->H = height;
     (void)hipGetDevice(&c->device);
     size_t npix = (size_t)width * height;
+    c->cbw = (width + CB_W - 1) / CB_W;
+    c->cbh = (height + CB_H - 1) / CB_H;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess || c->color.ensure(npix * 4) ||
-        c->depth.ensure(npix * 8)) {
+        c->depth.ensure(npix * 8) || c->cblk.ensure((size_t)c->cbw * c->cbh)) {
         set_err(g_err, "device allocation of a %d x %d framebuffer failed", width, height);
         if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;
         return FGL_E_NOMEM;
     }
     // image.NewNRGBA starts zeroed; NewContext clears depth to MaxFloat64.
-    if (hipMemsetAsync(c->color.p, 0, npix * 4, c->stream) != hipSuccess) {
+    if (hipMemsetAsync(c->color.p, 0, npix * 4, c->stream) != hipSuccess ||
+        hipMemsetAsync(c->cblk.p, 0, (size_t)c->cbw * c->cbh, c->stream) != hipSuccess) {
         g_err = "hipMemsetAsync failed";
         (void)hipStreamDestroy(c->stream);
         delete c;
@@ -1040,7 +1073,11 @@ int fgl_autotune_mesh(fgl_ctx *ctx, const fgl_state *state, const fgl_shader *sh
     HIPCHK(ctx, keep_depth.ensure(npix * 8));
     HIPCHK(ctx, hipMemcpyAsync(keep_color.p, ctx->color.p, npix * 4, hipMemcpyDeviceToDevice, ctx->stream));
     HIPCHK(ctx, hipMemcpyAsync(keep_depth.p, ctx->depth.p, npix * 8, hipMemcpyDeviceToDevice, ctx->stream));
-    const bool pc = ctx->pend_color, pd = ctx->pend_depth;
+    DevBuf keep_cblk;
+    const size_t ncb = (size_t)ctx->cbw * ctx->cbh;
+    HIPCHK(ctx, keep_cblk.ensure(ncb));
+    HIPCHK(ctx, hipMemcpyAsync(keep_cblk.p, ctx->cblk.p, ncb, hipMemcpyDeviceToDevice, ctx->stream));
+    const bool pc = ctx->pend_color, pd = ctx->pend_depth, cba = ctx->cblk_any;
     const int32_t sw = ctx->tile_w, sh = ctx->tile_h;
     const DrawCounters last = ctx->last;
     const bool prof = ctx->prof;
@@ -1079,9 +1116,11 @@ int fgl_autotune_mesh(fgl_ctx *ctx, const fgl_state *state, const fgl_shader *sh
     ctx->prof = prof;
     HIPCHK(ctx, hipMemcpyAsync(ctx->color.p, keep_color.p, npix * 4, hipMemcpyDeviceToDevice, ctx->stream));
     HIPCHK(ctx, hipMemcpyAsync(ctx->depth.p, keep_depth.p, npix * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->cblk.p, keep_cblk.p, ncb, hipMemcpyDeviceToDevice, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     ctx->pend_color = pc;
     ctx->pend_depth = pd;
+    ctx->cblk_any = cba;
     ctx->last = last;
     if (r) return r;
     {
@@ -1183,7 +1222,7 @@ int fgl_read_color(fgl_ctx *ctx, uint8_t *nrgba) {
         return FGL_E_INVALID;
     }
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    int r = materialize(ctx);
+    int r = materialize(ctx, CB_COLOR);
     if (r) return r;
     HIPCHK(ctx, hipMemcpyAsync(nrgba, ctx->color.p, (size_t)ctx->W * ctx->H * 4, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -1198,7 +1237,7 @@ int fgl_read_depth(fgl_ctx *ctx, double *depth) {
         return FGL_E_INVALID;
     }
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    int r = materialize(ctx);
+    int r = materialize(ctx, CB_DEPTH);
     if (r) return r;
     HIPCHK(ctx, hipMemcpyAsync(depth, ctx->depth.p, (size_t)ctx->W * ctx->H * 8, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -1214,6 +1253,8 @@ int fgl_write_color(fgl_ctx *ctx, const uint8_t *nrgba) {
     }
     HIPCHK(ctx, hipSetDevice(ctx->device));
     HIPCHK(ctx, hipMemcpyAsync(ctx->color.p, nrgba, (size_t)ctx->W * ctx->H * 4, hipMemcpyHostToDevice, ctx->stream));
+    int r = cblk_drop(ctx, CB_COLOR);
+    if (r) return r;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     ctx->pend_color = false;
     return FGL_OK;
@@ -1227,6 +1268,8 @@ int fgl_write_depth(fgl_ctx *ctx, const double *depth) {
     }
     HIPCHK(ctx, hipSetDevice(ctx->device));
     HIPCHK(ctx, hipMemcpyAsync(ctx->depth.p, depth, (size_t)ctx->W * ctx->H * 8, hipMemcpyHostToDevice, ctx->stream));
+    int r = cblk_drop(ctx, CB_DEPTH);
+    if (r) return r;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     ctx->pend_depth = false;
     return FGL_OK;
@@ -1239,7 +1282,7 @@ int fgl_depth_image(fgl_ctx *ctx, uint16_t *gray16) {
         return FGL_E_INVALID;
     }
     HIPCHK(ctx, hipSetDevice(ctx->device));
-    int r = materialize(ctx);
+    int r = materialize(ctx, CB_DEPTH);
     if (r) return r;
     const int64_t n = (int64_t)ctx->W * ctx->H;
     const int32_t nb = 512;
@@ -1284,6 +1327,12 @@ int fgl_depth_image_noalloc_async(fgl_ctx *ctx, uint16_t *gray16_device) {
     if (ctx->pend_depth) {
         HIPCHK(ctx, launch_fill_depth(ctx->depth.as<double>(), n, ctx->clear_depth, ctx->stream));
         ctx->pend_depth = false;
+    }
+    if (ctx->cblk_any) {
+        HIPCHK(ctx, launch_cblk_fill(ctx->color.as<uint32_t>(), ctx->depth.as<double>(), ctx->W, ctx->H,
+                                     ctx->cblk.as<uint8_t>(), ctx->cbw, ctx->cbh, ctx->clear_color, ctx->clear_depth,
+                                     CB_COLOR | CB_DEPTH, ctx->stream));
+        ctx->cblk_any = false;
     }
     HIPCHK(ctx, launch_depth_minmax(ctx->depth.as<double>(), n, part, nb, ctx->stream));
     HIPCHK(ctx, launch_depth_image(ctx->depth.as<double>(), n, part, nb, gray16_device, ctx->stream));
@@ -1356,8 +1405,7 @@ int fgl_resolve(fgl_ctx *src, fgl_ctx *dst, int32_t filter) {
         return FGL_E_INVALID;
     }
     HIPCHK(src, hipSetDevice(src->device));
-    int r = materialize(src);
-    if (r) return r;
+    int r = FGL_OK;
     // dst's earlier work finishes before the resolve overwrites its color
     if ((r = stream_after(src, dst))) return r;
     int32_t flx = 0, fly = 0;
@@ -1388,15 +1436,23 @@ int fgl_resolve(fgl_ctx *src, fgl_ctx *dst, int32_t filter) {
         }
         HIPCHK(src, src->rs_tmp.ensure((size_t)dst->W * src->H * 4));
     }
+    // the 4x tent reads the source's marks of deferred clears itself; the
+    // other filters read a materialised color buffer
+    const bool marks = resolve_reads_marks(filter, src->W, src->H, dst->W, dst->H, src->rs_t4);
+    if (marks && src->pend_color) r = materialize(src, CB_COLOR);
+    if (!marks) r = materialize(src, CB_COLOR);
+    if (r) return r;
+    const uint8_t *cb = marks && src->cblk_any ? src->cblk.as<uint8_t>() : nullptr;
     r = launch(src, "resolve", [&] {
         return launch_resolve(src->color.as<uint32_t>(), src->W, src->H, src->rs_tmp.as<uint32_t>(),
                               dst->color.as<uint32_t>(), dst->W, dst->H, filter, src->rs_wx.as<int16_t>(),
                               src->rs_ox.as<int32_t>(), flx, src->rs_wy.as<int16_t>(), src->rs_oy.as<int32_t>(), fly,
-                              src->rs_t4, src->stream);
+                              src->rs_t4, cb, src->cbw, src->clear_color, src->stream);
     });
     if (r) return r;
     // later work on dst (reads, draws) runs after the resolve
     if ((r = stream_after(dst, src))) return r;
+    if ((r = cblk_drop(dst, CB_COLOR))) return r;
     if (src->prof) prof_collect(src);
     dst->pend_color = false;
     return FGL_OK;
@@ -1628,7 +1684,7 @@ int nn_forward(fgl_nn *nn, fgl_ctx *src, fgl_ctx *dst, bool keep_feat) {
 // loss against target's color, gradient of every parameter into nn->g
 int nn_backward(fgl_nn *nn, fgl_ctx *src, fgl_ctx *target, float *loss) {
     fgl_ctx *c = nn->home;
-    int r = materialize(target);
+    int r = materialize(target, CB_COLOR);
     if (r) return r;
     if ((r = stream_after(c, target))) return r;
     const int32_t H = nn->H, W = nn->W;
@@ -1817,6 +1873,7 @@ int fgl_nn_infer(fgl_nn *nn, fgl_ctx *src, fgl_ctx *dst) {
     if ((r = nn_forward(nn, src, dst, false))) return r;
     if ((r = stream_after(dst, c))) return r;
     if ((r = stream_after(src, c))) return r;
+    if ((r = cblk_drop(dst, CB_COLOR))) return r;
     dst->pend_color = false;
     return FGL_OK;
 }
@@ -2371,7 +2428,7 @@ int fgl_encode_png(fgl_ctx *ctx, uint8_t *out, int64_t cap, int64_t *len) {
     }
     fgl_ctx *c = ctx;
     HIPCHK(c, hipSetDevice(c->device));
-    int r = materialize(c);
+    int r = materialize(c, CB_COLOR);
     if (r) return r;
     const int32_t W = c->W, H = c->H;
     DevBuf flag, stream, pieces, sizes, adl, offs, packed;
@@ -2711,17 +2768,31 @@ int fgl_batch_run(fgl_batch *b, uint64_t *info) {
     if (!b->exec || b->epoch != g_dev_epoch.load()) return batch_first_run(b, out);
     // the replay follows whatever is already queued on the views' contexts
     // (a resolve or readback of the previous frame, a clear's fill)
+    // (a stream with nothing pending, the usual case, needs no event)
     for (fgl_ctx *c : b->ctxs) {
+        const hipError_t q = hipStreamQuery(c->stream);
+        if (q == hipSuccess) continue;
+        if (q != hipErrorNotReady) BCHK(b, q);
         if (!c->xev) BCHK(b, hipEventCreateWithFlags(&c->xev, hipEventDisableTiming));
         BCHK(b, hipEventRecord(c->xev, c->stream));
         BCHK(b, hipStreamWaitEvent(b->s0, c->xev, 0));
     }
+    (void)hipGetLastError();  // a not-ready query is no error of a later launch
     BCHK(b, hipGraphLaunch(b->exec, b->s0));
     b->host.resize(2 * n);
     BCHK(b, hipMemcpyAsync(b->host.data(), b->ctrs.p, 2 * n * sizeof(DrawCounters), hipMemcpyDeviceToHost, b->s0));
     BCHK(b, hipStreamSynchronize(b->s0));
-    // every view cleared its context as it was drawn
-    for (fgl_ctx *c : b->ctxs) c->pend_color = c->pend_depth = false;
+    // every view cleared its context as it was drawn (its ROP may have left
+    // blocks at the clear values): a context's clear values are now its last
+    // view's, whatever was cleared on it between runs
+    for (fgl_ctx *c : b->ctxs) {
+        c->pend_color = c->pend_depth = false;
+        c->cblk_any = true;
+    }
+    for (const fgl_batch::View &v : b->views) {
+        v.ctx->clear_color = v.clear_color;
+        v.ctx->clear_depth = v.clear_depth;
+    }
     // the first view per context whose frame no longer fits the recorded
     // capacities
     std::map<fgl_ctx *, size_t> first_ovf;

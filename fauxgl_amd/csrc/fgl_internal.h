@@ -78,6 +78,10 @@ struct DrawArgs {
     int32_t clear_color_pending, clear_depth_pending;
     uint32_t clear_color;
     double clear_depth;
+    // per CB_W x CB_H pixel block: CB_COLOR / CB_DEPTH set while the block's
+    // color / depth is the context's clear value and has not been written
+    uint8_t *cblk;
+    int32_t cbw;           // blocks per row
 
     DrawCounters *ctr;
 };
@@ -131,6 +135,19 @@ FGL_TILE_SHAPES(FGL_DECLARE_TILE_OPS)
 #undef FGL_DECLARE_TILE_OPS
 
 hipError_t launch_copy_words(const uint32_t *src, uint32_t *dst, int32_t n, hipStream_t s);
+// Deferred clears per block.  A draw whose clear is still pending leaves the
+// blocks of the tiles no primitive reaches unwritten and marks them here; the
+// ROP starts a later tile's marked blocks from the clear value; a reader of
+// the whole buffer first fills the marked blocks (launch_cblk_fill).  Every
+// tile shape is a whole number of blocks.
+constexpr int CB_W = 32, CB_H = 16;
+constexpr uint8_t CB_COLOR = 1, CB_DEPTH = 2;
+// fills the marked blocks' channels in `chans` with the clear values and
+// unmarks them
+hipError_t launch_cblk_fill(uint32_t *color, double *depth, int32_t W, int32_t H, uint8_t *cblk, int32_t cbw,
+                            int32_t cbh, uint32_t clear_color, double clear_depth, uint8_t chans, hipStream_t s);
+// unmarks `bits` in every block (the channel was overwritten as a whole)
+hipError_t launch_cblk_drop(uint8_t *cblk, int64_t n, uint8_t bits, hipStream_t s);
 hipError_t launch_fill_color(uint32_t *color, int64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_fill_depth(double *depth, int64_t n, double v, hipStream_t s);
 // The 4x tent when every output row of both weight tables is the same eight
@@ -144,9 +161,14 @@ struct ResolveTent4 {
     int32_t shx, shy;  // log2 of each table's weight sum
 };
 
+// The 4x tent path reads the source's deferred-clear marks (cblk, or null
+// when there are none) and takes the clear color for marked blocks; the other
+// paths need the source's color materialised.
+bool resolve_reads_marks(int32_t filter, int32_t sw, int32_t sh, int32_t dw, int32_t dh, const ResolveTent4 &t4);
 hipError_t launch_resolve(const uint32_t *src, int32_t sw, int32_t sh, uint32_t *tmp, uint32_t *dst, int32_t dw,
                           int32_t dh, int32_t filter, const int16_t *wx, const int32_t *ox, int32_t flx,
-                          const int16_t *wy, const int32_t *oy, int32_t fly, const ResolveTent4 &t4, hipStream_t s);
+                          const int16_t *wy, const int32_t *oy, int32_t fly, const ResolveTent4 &t4,
+                          const uint8_t *cblk, int32_t cbw, uint32_t clear_color, hipStream_t s);
 hipError_t launch_depth_minmax(const double *depth, int64_t n, double *partial, int32_t nblocks, hipStream_t s);
 hipError_t launch_depth_image(const double *depth, int64_t n, const double *partial, int32_t nblocks,
                               uint16_t *out, hipStream_t s);

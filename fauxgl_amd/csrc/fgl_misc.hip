@@ -34,6 +34,38 @@ hipError_t launch_copy_words(const uint32_t *src, uint32_t *dst, int32_t n, hipS
     return hipGetLastError();
 }
 
+// one workgroup per block: its marked channels get the clear values
+__global__ __launch_bounds__(256) void k_cblk_fill(uint32_t *color, double *depth, int32_t W, int32_t H,
+                                                  uint8_t *cblk, int32_t cbw, uint32_t cc, double cd, uint8_t chans) {
+    const int b = blockIdx.x;
+    const uint8_t m = cblk[b] & chans;
+    if (!m) return;
+    const int x0 = (b % cbw) * CB_W, y0 = (b / cbw) * CB_H;
+    for (int p = threadIdx.x; p < CB_W * CB_H; p += 256) {
+        const int x = x0 + p % CB_W, y = y0 + p / CB_W;
+        if (x >= W || y >= H) continue;
+        const int64_t i = (int64_t)y * W + x;
+        if (m & CB_COLOR) color[i] = cc;
+        if (m & CB_DEPTH) depth[i] = cd;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) cblk[b] &= (uint8_t)~m;
+}
+hipError_t launch_cblk_fill(uint32_t *color, double *depth, int32_t W, int32_t H, uint8_t *cblk, int32_t cbw,
+                            int32_t cbh, uint32_t clear_color, double clear_depth, uint8_t chans, hipStream_t s) {
+    hipLaunchKernelGGL(k_cblk_fill, dim3((unsigned)(cbw * cbh)), dim3(256), 0, s, color, depth, W, H, cblk, cbw,
+                       clear_color, clear_depth, chans);
+    return hipGetLastError();
+}
+__global__ void k_cblk_drop(uint8_t *cblk, int64_t n, uint8_t bits) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        cblk[i] &= (uint8_t)~bits;
+}
+hipError_t launch_cblk_drop(uint8_t *cblk, int64_t n, uint8_t bits, hipStream_t s) {
+    hipLaunchKernelGGL(k_cblk_drop, dim3(grid_for(n)), dim3(256), 0, s, cblk, n, bits);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_color(uint32_t *color, int64_t n, uint32_t v, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_color, dim3(grid_for(n)), dim3(256), 0, s, color, n, v);
     return hipGetLastError();
@@ -363,21 +395,45 @@ FGL_D uint32_t t4_tap(uint32_t acc, int sh) {
 }
 
 __global__ __launch_bounds__(256) void k_resolve_tent4(const uint32_t *src, int32_t sw, int32_t sh, uint32_t *dst,
-                                                       int32_t dw, int32_t dh, ResolveTent4 t) {
+                                                       int32_t dw, int32_t dh, ResolveTent4 t, const uint8_t *cblk,
+                                                       int32_t cbw, uint32_t clear_color) {
     __shared__ uint32_t pl[4][T4_ROWS][T4_QS];
     __shared__ uint32_t hp[4][T4_OW][T4_ROWS / 4 + 1];
     const int tid = threadIdx.x;
     const int x0 = blockIdx.x * T4_OW, y0 = blockIdx.y * T4_OH;
     const int ca = 4 * x0 - 4, ra = 4 * y0 - 2;
     uint4 v[T4_LOADS];
+    // source blocks a draw left at the clear color (cblk: the marks of
+    // deferred clears, null when there are none) are not read: a quad of an
+    // interior load lies in one 32-pixel-wide block
+    uint8_t mk[T4_LOADS];
 #pragma unroll
     for (int k = 0; k < T4_LOADS; k++) {
         const int q = min(k * 256 + tid, T4_QUADS - 1);
         const int j = q / T4_QW, m = q % T4_QW;
-        const uint32_t *row = src + (int64_t)min(max(ra + j, 0), sh - 1) * sw;
+        const int ry = min(max(ra + j, 0), sh - 1), c = ca + 4 * m;
+        mk[k] = cblk && c >= 0 && c + 3 < sw ? cblk[(ry / CB_H) * cbw + c / CB_W] : (uint8_t)0;
+    }
+#pragma unroll
+    for (int k = 0; k < T4_LOADS; k++) {
+        const int q = min(k * 256 + tid, T4_QUADS - 1);
+        const int j = q / T4_QW, m = q % T4_QW;
+        const int ry = min(max(ra + j, 0), sh - 1);
+        const uint32_t *row = src + (int64_t)ry * sw;
         const int c = ca + 4 * m;
-        if (c >= 0 && c + 3 < sw) {
+        if (mk[k] & CB_COLOR) {
+            v[k] = make_uint4(clear_color, clear_color, clear_color, clear_color);
+        } else if (c >= 0 && c + 3 < sw) {
             v[k] = *reinterpret_cast<const uint4 *>(row + c);
+        } else if (cblk) {
+            // an edge quad (clamped columns): each pixel by its own block
+            const int c0 = min(max(c, 0), sw - 1), c1 = min(max(c + 1, 0), sw - 1);
+            const int c2 = min(max(c + 2, 0), sw - 1), c3 = min(max(c + 3, 0), sw - 1);
+            const uint8_t *mrow = cblk + (ry / CB_H) * cbw;
+            v[k].x = (mrow[c0 / CB_W] & CB_COLOR) ? clear_color : row[c0];
+            v[k].y = (mrow[c1 / CB_W] & CB_COLOR) ? clear_color : row[c1];
+            v[k].z = (mrow[c2 / CB_W] & CB_COLOR) ? clear_color : row[c2];
+            v[k].w = (mrow[c3 / CB_W] & CB_COLOR) ? clear_color : row[c3];
         } else {
             v[k].x = row[min(max(c, 0), sw - 1)];
             v[k].y = row[min(max(c + 1, 0), sw - 1)];
@@ -441,9 +497,14 @@ __global__ __launch_bounds__(256) void k_resolve_tent4(const uint32_t *src, int3
     }
 }
 
+bool resolve_reads_marks(int32_t filter, int32_t sw, int32_t sh, int32_t dw, int32_t dh, const ResolveTent4 &t4) {
+    return filter != 2 && t4.ok && sw == 4 * dw && sh == 4 * dh && sw % 4 == 0;
+}
+
 hipError_t launch_resolve(const uint32_t *src, int32_t sw, int32_t sh, uint32_t *tmp, uint32_t *dst, int32_t dw,
                           int32_t dh, int32_t filter, const int16_t *wx, const int32_t *ox, int32_t flx,
-                          const int16_t *wy, const int32_t *oy, int32_t fly, const ResolveTent4 &t4, hipStream_t s) {
+                          const int16_t *wy, const int32_t *oy, int32_t fly, const ResolveTent4 &t4,
+                          const uint8_t *cblk, int32_t cbw, uint32_t clear_color, hipStream_t s) {
     if (filter == 2) {
         hipLaunchKernelGGL(k_resolve_box, dim3(grid_for((int64_t)dw * dh)), dim3(256), 0, s, src, sw, dst, dw, dh,
                            sw / dw);
@@ -451,9 +512,10 @@ hipError_t launch_resolve(const uint32_t *src, int32_t sw, int32_t sh, uint32_t 
     }
     // the 4x tent with uniform weights (checked with the tables on the host);
     // rows of 16-byte multiples keep the staged loads aligned
-    if (t4.ok && sw == 4 * dw && sh == 4 * dh && sw % 4 == 0) {
+    if (resolve_reads_marks(filter, sw, sh, dw, dh, t4)) {
         dim3 grid((dw + T4_OW - 1) / T4_OW, (dh + T4_OH - 1) / T4_OH);
-        hipLaunchKernelGGL(k_resolve_tent4, grid, dim3(256), 0, s, src, sw, sh, dst, dw, dh, t4);
+        hipLaunchKernelGGL(k_resolve_tent4, grid, dim3(256), 0, s, src, sw, sh, dst, dw, dh, t4, cblk, cbw,
+                           clear_color);
         return hipGetLastError();
     }
     // integer factor up to RS_F: the source window fits the LDS stage

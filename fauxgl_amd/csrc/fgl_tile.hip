@@ -707,34 +707,47 @@ __global__ __launch_bounds__(TILE_THREADS, FGL_ROP_WAVES) void k_rop(DrawArgs a)
     const uint32_t ps = a.tstart[t], pe = a.tend[t];
     const uint32_t f0 = pe > ps ? a.foff[a.poff[ps]] : 0u;
     const uint32_t f1 = pe > ps ? a.foff[a.poff[pe]] : 0u;
+    // the tile's blocks of deferred clears (CB_W x CB_H pixels each)
+    constexpr int TBW = TILE_W / CB_W, TBH = TILE_H / CB_H;
+    static_assert(TILE_W % CB_W == 0 && TILE_H % CB_H == 0, "tiles are whole clear blocks");
+    const int bx0 = cx0 / CB_W, by0 = cy0 / CB_H;
+    const int nbx = (cw + CB_W - 1) / CB_W, nby = (chh + CB_H - 1) / CB_H;
     if (f1 <= f0) {
-        if (a.clear_color_pending || a.clear_depth_pending) {
-            for (int p = tid; p < TILE_PIX; p += TILE_THREADS) {
-                int lx = p % TILE_W, ly = p / TILE_W;
-                if (lx < cw && ly < chh) {
-                    int64_t gi = (int64_t)(cy0 + ly) * a.W + cx0 + lx;
-                    if (a.clear_color_pending) a.color[gi] = a.clear_color;
-                    if (a.clear_depth_pending) a.depth[gi] = a.clear_depth;
-                }
-            }
+        // no fragment: a pending clear stays deferred, marked on the blocks
+        const uint8_t bits = (a.clear_color_pending ? CB_COLOR : 0) | (a.clear_depth_pending ? CB_DEPTH : 0);
+        if (bits && tid < TBW * TBH) {
+            const int bx = tid % TBW, by = tid / TBW;
+            if (bx < nbx && by < nby) a.cblk[(by0 + by) * a.cbw + bx0 + bx] |= bits;
         }
         return;
     }
     {
-        // the tile's current color and depth (or the pending clear values);
-        // loads are unconditional (clamped to the tile origin outside the
-        // framebuffer) so they issue back to back
+        // the tile's current color and depth: the pending clear value, the
+        // clear value of a block still marked, or memory; loads are
+        // unconditional (clamped to the tile origin outside the framebuffer)
+        // so they issue back to back
         constexpr int PPT = TILE_PIX / TILE_THREADS;
         uint32_t cv[PPT];
         double dv[PPT];
+        uint8_t mk[PPT];
         const int64_t g0 = (int64_t)cy0 * a.W + cx0;
+        const bool both_pending = a.clear_color_pending && a.clear_depth_pending;
+#pragma unroll
+        for (int k = 0; k < PPT; k++) {
+            const int p = k * TILE_THREADS + tid;
+            const int lx = p % TILE_W, ly = p / TILE_W;
+            const bool in = lx < cw && ly < chh;
+            mk[k] = in && !both_pending ? a.cblk[(by0 + ly / CB_H) * a.cbw + bx0 + lx / CB_W] : (uint8_t)0;
+        }
 #pragma unroll
         for (int k = 0; k < PPT; k++) {
             const int p = k * TILE_THREADS + tid;
             const int lx = p % TILE_W, ly = p / TILE_W;
             const int64_t gi = (lx < cw && ly < chh) ? (int64_t)(cy0 + ly) * a.W + cx0 + lx : g0;
-            cv[k] = a.clear_color_pending ? a.clear_color : a.color[gi];
-            dv[k] = a.clear_depth_pending ? a.clear_depth : a.depth[gi];
+            cv[k] = a.clear_color;
+            dv[k] = a.clear_depth;
+            if (!a.clear_color_pending && !(mk[k] & CB_COLOR)) cv[k] = a.color[gi];
+            if (!a.clear_depth_pending && !(mk[k] & CB_DEPTH)) dv[k] = a.depth[gi];
         }
 #pragma unroll
         for (int k = 0; k < PPT; k++) {
@@ -797,6 +810,12 @@ __global__ __launch_bounds__(TILE_THREADS, FGL_ROP_WAVES) void k_rop(DrawArgs a)
             a.color[gi] = col[p];
             a.depth[gi] = dep[p];
         }
+    }
+    // the whole tile is written: its blocks hold no deferred clear (every
+    // thread read its marks before the first barrier of the fragment loop)
+    if (tid < TBW * TBH) {
+        const int bx = tid % TBW, by = tid / TBW;
+        if (bx < nbx && by < nby) a.cblk[(by0 + by) * a.cbw + bx0 + bx] = 0;
     }
     // UpdatedPixels of this tile, into one of the partial sums
     uint32_t uu = my_updated;
