@@ -69,6 +69,51 @@ struct DevBuf {
     template <class T> T *as() const { return static_cast<T *>(p); }
 };
 
+// Page-locked host memory (hipHostMalloc), so a device-to-host copy into it
+// needs no staging.
+struct HostBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~HostBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    hipError_t ensure(size_t need) {
+        if (need <= bytes && p) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipHostMalloc(&p, need, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        bytes = need;
+        return hipSuccess;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Waits for a stream by polling it.  A blocking wait spins only briefly and
+// then sleeps until an interrupt, which wakes the host tens of microseconds
+// after the GPU has finished: a large share of a draw that takes a few
+// hundred.  Polling gives up after kPollMs and blocks from then on, so a
+// long wait does not hold a core.
+constexpr int kPollMs = 20;
+hipError_t wait_stream(hipStream_t s) {
+    const auto t0 = std::chrono::steady_clock::now();
+    hipError_t q;
+    for (;;) {
+        q = hipStreamQuery(s);
+        if (q != hipErrorNotReady) break;
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(kPollMs)) {
+            q = hipStreamSynchronize(s);
+            break;
+        }
+    }
+    (void)hipGetLastError();  // a not-ready query is no error of a later call
+    return q;
+}
+
 struct Tex {
     void *d = nullptr;
     int32_t w = 0, h = 0, kind = 0;
@@ -104,6 +149,7 @@ struct fgl_ctx {
     int64_t next_tex = 1;
 
     DevBuf ctr, ntl, nrast, sstart, blocksum, blockoff, rec_main, side_rec, side_vtx;
+    HostBuf hctr;  // the draw counters as the host reads them back
     DevBuf keys0, vals0, keys1, vals1, hist, dtot, tstart, tend, deferred;
     DevBuf pspans, span_bsum, span_boff, poff, spans;
     DevBuf frag_bsum, frag_boff, frag_gsum, foff, frags;
@@ -640,9 +686,10 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
                                          (int32_t)(sizeof(DrawCounters) / 4), c->stream);
             });
 
-        DrawCounters h;
-        HIPCHK(c, hipMemcpyAsync(&h, a.ctr, sizeof h, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, c->hctr.ensure(sizeof(DrawCounters)));
+        HIPCHK(c, hipMemcpyAsync(c->hctr.p, a.ctr, sizeof(DrawCounters), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, wait_stream(c->stream));
+        const DrawCounters h = *c->hctr.as<DrawCounters>();
         if (c->prof) prof_collect(c);
         c->last = h;
         c->last_overflow |= h.overflow;
@@ -1523,7 +1570,7 @@ int fgl_ctx_force_capacities(fgl_ctx *ctx, const uint32_t caps[5]) {
 
 int fgl_ctx_sync(fgl_ctx *ctx) {
     if (check_ctx(ctx)) return FGL_E_INVALID;
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, wait_stream(ctx->stream));
     if (ctx->prof) prof_collect(ctx);
     return FGL_OK;
 }
