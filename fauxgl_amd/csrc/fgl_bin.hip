@@ -7,6 +7,7 @@
 #include "fgl_internal.h"
 #include "fgl_scan.h"
 #include "fgl_tiles.h"
+#include "fgl_draws.h"
 
 namespace fgl {
 namespace FGL_TILE_NS {
@@ -14,30 +15,40 @@ namespace FGL_TILE_NS {
 // ---------------------------------------------------------------------------
 // per-draw reset
 
-__global__ void k_zero(DrawArgs a) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+template <class G>
+FGL_D void zero_body(const DrawArgs &a, const G &g) {
+    int64_t i = (int64_t)g.bid() * blockDim.x + threadIdx.x;
     if (i == 0) {
         DrawCounters z = {};
         *a.ctr = z;
     }
-    for (int64_t t = i; t < a.ntiles; t += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t t = i; t < a.ntiles; t += (int64_t)g.nblk() * blockDim.x) {
         a.tstart[t] = 0;
         a.tend[t] = 0;
     }
 }
+FGL_DRAW_KERNELS(k_zero, zero_body, __launch_bounds__(256))
 
-hipError_t launch_zero(const DrawArgs &a, hipStream_t s) {
+uint32_t blocks_zero(const DrawArgs &a) {
     int blocks = (a.ntiles + 255) / 256;
     if (blocks < 1) blocks = 1;
     if (blocks > 1024) blocks = 1024;
-    hipLaunchKernelGGL(k_zero, dim3(blocks), dim3(256), 0, s, a);
+    return (uint32_t)blocks;
+}
+hipError_t launch_zero(const DrawArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero, dim3(blocks_zero(a)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_zero_multi(const MultiArgs &m, uint32_t nb, const DrawArgs &, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_m, dim3(nb), dim3(256), 0, s, m);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
 // exclusive scan of the vertex-stage block sums (one workgroup)
 
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(DrawArgs a) {
+template <class G>
+FGL_D void scan_top_body(const DrawArgs &a, const G &) {
     __shared__ uint64_t lds[SCAN_THREADS / 64];
     const int tid = threadIdx.x;
     const uint64_t total = scan_block_sums(a.blocksum, a.blockoff, a.nvsblocks, lds);
@@ -50,19 +61,25 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_top(DrawArgs a) {
         a.ctr->overflow = ovf;
     }
 }
+FGL_DRAW_KERNELS(k_scan_top, scan_top_body, __launch_bounds__(SCAN_THREADS))
 
 hipError_t launch_scan_top(const DrawArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_THREADS), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_scan_top_multi(const MultiArgs &m, uint32_t nb, const DrawArgs &, hipStream_t s) {
+    hipLaunchKernelGGL(k_scan_top_m, dim3(nb), dim3(SCAN_THREADS), 0, s, m);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
 // pair emission
 
-__global__ __launch_bounds__(VS_THREADS) void k_emit(DrawArgs a) {
+template <class G>
+FGL_D void emit_body(const DrawArgs &a, const G &g) {
     __shared__ uint64_t wsum[VS_THREADS / 64];
     if (a.ctr->overflow) return;
-    int64_t i = (int64_t)blockIdx.x * VS_THREADS + threadIdx.x;
+    int64_t i = (int64_t)g.bid() * VS_THREADS + threadIdx.x;
     uint32_t nt = i < a.in.n ? a.ntl[i] : 0u;
     // block-exclusive scan of nt
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -75,7 +92,7 @@ __global__ __launch_bounds__(VS_THREADS) void k_emit(DrawArgs a) {
     __syncthreads();
     uint64_t wbase = 0;
     for (int w = 0; w < wave; w++) wbase += wsum[w];
-    uint64_t pos = a.blockoff[blockIdx.x] + wbase + v - nt;
+    uint64_t pos = a.blockoff[g.bid()] + wbase + v - nt;
     if (i >= a.in.n || nt == 0) return;
     uint32_t nr = a.nrast[i];
     bool side = (nr & REF_SIDE) != 0;
@@ -103,15 +120,22 @@ __global__ __launch_bounds__(VS_THREADS) void k_emit(DrawArgs a) {
     }
 }
 
+FGL_DRAW_KERNELS(k_emit, emit_body, __launch_bounds__(VS_THREADS))
+
 hipError_t launch_emit(const DrawArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_emit, dim3(a.nvsblocks), dim3(VS_THREADS), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_emit_multi(const MultiArgs &m, uint32_t nb, const DrawArgs &, hipStream_t s) {
+    hipLaunchKernelGGL(k_emit_m, dim3(nb), dim3(VS_THREADS), 0, s, m);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
 // stable LSD radix sort of (tile id, ref) pairs, 8 bits per pass
 
-__global__ __launch_bounds__(RS_THREADS) void k_radix_hist(DrawArgs a, int pass) {
+template <class G>
+FGL_D void radix_hist_body(const DrawArgs &a, const G &g, int pass) {
     __shared__ uint32_t h[256];
     if (a.ctr->overflow) return;
     const uint32_t E = a.ctr->npairs;
@@ -119,10 +143,10 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(DrawArgs a, int pass)
     const int shift = 8 * pass;
     h[threadIdx.x] = 0;
     __syncthreads();
-    const uint64_t base = (uint64_t)blockIdx.x * RS_BLOCK;
+    const uint64_t base = (uint64_t)g.bid() * RS_BLOCK;
     if (base >= E) {
         __syncthreads();
-        a.hist[(uint64_t)threadIdx.x * a.nrsblocks + blockIdx.x] = 0;
+        a.hist[(uint64_t)threadIdx.x * a.nrsblocks + g.bid()] = 0;
         return;
     }
     // unconditional loads (clamped to the last pair) issue back to back
@@ -138,8 +162,9 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(DrawArgs a, int pass)
         if (idx < E) atomicAdd(&h[(k[it] >> shift) & 255u], 1u);
     }
     __syncthreads();
-    a.hist[(uint64_t)threadIdx.x * a.nrsblocks + blockIdx.x] = h[threadIdx.x];
+    a.hist[(uint64_t)threadIdx.x * a.nrsblocks + g.bid()] = h[threadIdx.x];
 }
+FGL_DRAW_KERNELS_PASS(k_radix_hist, radix_hist_body, __launch_bounds__(RS_THREADS))
 
 // Block-wide exclusive scan of one value per thread (256 threads); returns
 // the exclusive prefix and writes the block total to *total.
@@ -164,10 +189,11 @@ __device__ uint32_t block_exclusive_scan_256(uint32_t v, uint32_t *wsum, uint32_
 
 // One workgroup per digit: exclusive scan of that digit's per-block counts
 // (hist is digit-major) and the digit total.
-__global__ __launch_bounds__(RS_THREADS) void k_radix_scan(DrawArgs a) {
+template <class G>
+FGL_D void radix_scan_body(const DrawArgs &a, const G &g) {
     __shared__ uint32_t wsum[4];
     if (a.ctr->overflow) return;
-    const int d = blockIdx.x;
+    const int d = (int)g.bid();
     uint32_t *row = a.hist + (uint64_t)d * a.nrsblocks;
     uint32_t carry = 0;
     for (int base = 0; base < a.nrsblocks; base += RS_THREADS) {
@@ -180,8 +206,10 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scan(DrawArgs a) {
     }
     if (threadIdx.x == 0) a.dtot[d] = carry;
 }
+FGL_DRAW_KERNELS(k_radix_scan, radix_scan_body, __launch_bounds__(RS_THREADS))
 
-__global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(DrawArgs a, int pass) {
+template <class G>
+FGL_D void radix_scatter_body(const DrawArgs &a, const G &g, int pass) {
     __shared__ uint32_t run[256];
     __shared__ uint32_t wcnt[RS_THREADS / 64][256];
     if (a.ctr->overflow) return;
@@ -198,9 +226,9 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(DrawArgs a, int pa
         __shared__ uint32_t wsum[4];
         uint32_t tot;
         uint32_t dbase = block_exclusive_scan_256(a.dtot[tid], wsum, &tot);
-        run[tid] = dbase + a.hist[(uint64_t)tid * a.nrsblocks + blockIdx.x];
+        run[tid] = dbase + a.hist[(uint64_t)tid * a.nrsblocks + g.bid()];
     }
-    const uint64_t base = (uint64_t)blockIdx.x * RS_BLOCK;
+    const uint64_t base = (uint64_t)g.bid() * RS_BLOCK;
     if (base >= E) return;
     // all of this thread's pairs, loaded up front (clamped, unconditional)
     uint32_t kk[RS_ITEMS], vv[RS_ITEMS];
@@ -249,6 +277,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(DrawArgs a, int pa
     }
 }
 
+FGL_DRAW_KERNELS_PASS(k_radix_scatter, radix_scatter_body, __launch_bounds__(RS_THREADS))
+
 hipError_t launch_radix_hist(const DrawArgs &a, int pass, hipStream_t s) {
     hipLaunchKernelGGL(k_radix_hist, dim3(a.nrsblocks), dim3(RS_THREADS), 0, s, a, pass);
     return hipGetLastError();
@@ -259,6 +289,18 @@ hipError_t launch_radix_scan(const DrawArgs &a, hipStream_t s) {
 }
 hipError_t launch_radix_scatter(const DrawArgs &a, int pass, hipStream_t s) {
     hipLaunchKernelGGL(k_radix_scatter, dim3(a.nrsblocks), dim3(RS_THREADS), 0, s, a, pass);
+    return hipGetLastError();
+}
+hipError_t launch_radix_hist_multi(const MultiArgs &m, uint32_t nb, const DrawArgs &, hipStream_t s) {
+    hipLaunchKernelGGL(k_radix_hist_m, dim3(nb), dim3(RS_THREADS), 0, s, m);
+    return hipGetLastError();
+}
+hipError_t launch_radix_scan_multi(const MultiArgs &m, uint32_t nb, const DrawArgs &, hipStream_t s) {
+    hipLaunchKernelGGL(k_radix_scan_m, dim3(nb), dim3(RS_THREADS), 0, s, m);
+    return hipGetLastError();
+}
+hipError_t launch_radix_scatter_multi(const MultiArgs &m, uint32_t nb, const DrawArgs &, hipStream_t s) {
+    hipLaunchKernelGGL(k_radix_scatter_m, dim3(nb), dim3(RS_THREADS), 0, s, m);
     return hipGetLastError();
 }
 

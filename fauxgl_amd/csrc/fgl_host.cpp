@@ -14,6 +14,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/fgl.h"
@@ -557,14 +558,15 @@ int draw_pass(fgl_ctx *c, const TileOps &T, const DrawArgs &a, int kind) {
     return launch(c, "rop", [&] { return T.rop(a, c->stream); });
 }
 
-int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, const DevPrims &in, uint64_t info[2],
-             uint32_t *capture_ctr = nullptr) {
-    if (info) info[0] = info[1] = 0;
+// The first part of a draw: checks, the shader, the framebuffer and state
+// fields, the starting capacities and the buffers whose size does not depend
+// on them.  An empty draw returns FGL_OK with a.in.n == 0 and T unset.
+int draw_begin(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, const DevPrims &in, const TileOps *&T,
+               DrawArgs &a) {
     if (in.n < 0 || in.n > kMaxPrims) {
         set_err(c->err, "primitive count %lld outside [0, %lld]", (long long)in.n, (long long)kMaxPrims);
         return FGL_E_INVALID;
     }
-    DrawArgs a;
     std::memset(&a, 0, sizeof a);
     int r = fill_shader(c, sh, a.sh);
     if (r) return r;
@@ -573,8 +575,8 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
         set_err(c->err, "hipSetDevice failed");
         return FGL_E_DEVICE;
     }
-    const TileOps &T = *draw_ops(c, kind, in.n);
-    draw_setup(c, st, T, kind, in, a);
+    T = draw_ops(c, kind, in.n);
+    draw_setup(c, st, *T, kind, in, a);
     const int64_t n = in.n;
 
     // starting capacities; grown on overflow
@@ -605,67 +607,85 @@ int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, co
     HIPCHK(c, c->dtot.ensure(256 * 4));
     HIPCHK(c, c->tstart.ensure((size_t)a.ntiles * 4));
     HIPCHK(c, c->tend.ensure((size_t)a.ntiles * 4));
+    return FGL_OK;
+}
+
+// Sizes the buffers that depend on the capacities (again after they grow)
+// and points a at every buffer of the context.
+int draw_buffers(fgl_ctx *c, DrawArgs &a) {
+    HIPCHK(c, c->side_rec.ensure((size_t)c->side_rec_cap * sizeof(PrimRec)));
+    HIPCHK(c, c->side_vtx.ensure((size_t)c->side_vtx_cap * SIDE_VTX_DOUBLES * 8));
+    HIPCHK(c, c->keys0.ensure((size_t)c->pair_cap * 4));
+    HIPCHK(c, c->vals0.ensure((size_t)c->pair_cap * 4));
+    HIPCHK(c, c->keys1.ensure((size_t)c->pair_cap * 4));
+    HIPCHK(c, c->vals1.ensure((size_t)c->pair_cap * 4));
+    a.nrsblocks = (int32_t)(((uint64_t)c->pair_cap + RS_BLOCK - 1) / RS_BLOCK);
+    HIPCHK(c, c->hist.ensure((size_t)256 * a.nrsblocks * 4));
+    const size_t nsb = ((size_t)c->pair_cap + SPAN_THREADS - 1) / SPAN_THREADS + 1;
+    HIPCHK(c, c->pspans.ensure((size_t)c->pair_cap * 4));
+    HIPCHK(c, c->span_bsum.ensure(nsb * 8));
+    HIPCHK(c, c->span_boff.ensure(nsb * 8));
+    HIPCHK(c, c->poff.ensure(((size_t)c->pair_cap + 1) * 4));
+    HIPCHK(c, c->spans.ensure((size_t)c->span_cap * sizeof(Span)));
+    a.pspans = c->pspans.as<uint32_t>();
+    a.span_blocksum = c->span_bsum.as<uint64_t>();
+    a.span_blockoff = c->span_boff.as<uint64_t>();
+    a.poff = c->poff.as<uint32_t>();
+    a.spans = c->spans.as<Span>();
+    a.span_cap = c->span_cap;
+    const size_t nfb = ((size_t)c->span_cap + FRAG_BLOCK - 1) / FRAG_BLOCK + 1;
+    HIPCHK(c, c->frag_bsum.ensure(nfb * 8));
+    HIPCHK(c, c->frag_boff.ensure(nfb * 8));
+    HIPCHK(c, c->frag_gsum.ensure((nfb / 1024 + 2) * 8));
+    HIPCHK(c, c->foff.ensure(((size_t)c->span_cap + 1) * 4));
+    HIPCHK(c, c->frags.ensure((size_t)c->frag_cap * sizeof(Frag)));
+    a.frag_blocksum = c->frag_bsum.as<uint64_t>();
+    a.frag_blockoff = c->frag_boff.as<uint64_t>();
+    a.frag_gsum = c->frag_gsum.as<uint64_t>();
+    a.foff = c->foff.as<uint32_t>();
+    a.frags = c->frags.as<Frag>();
+    a.frag_cap = c->frag_cap;
+
+    a.rec_main = c->rec_main.as<PrimRec>();
+    a.side_rec = c->side_rec.as<PrimRec>();
+    a.side_rec_cap = c->side_rec_cap;
+    a.side_vtx = c->side_vtx.as<double>();
+    a.side_vtx_cap = c->side_vtx_cap;
+    a.ntl = c->ntl.as<uint32_t>();
+    a.nrast = c->nrast.as<uint32_t>();
+    a.sstart = c->sstart.as<uint32_t>();
+    a.blocksum = c->blocksum.as<uint64_t>();
+    a.blockoff = c->blockoff.as<uint64_t>();
+    a.keys[0] = c->keys0.as<uint32_t>();
+    a.vals[0] = c->vals0.as<uint32_t>();
+    a.keys[1] = c->keys1.as<uint32_t>();
+    a.vals[1] = c->vals1.as<uint32_t>();
+    a.pair_cap = c->pair_cap;
+    a.hist = c->hist.as<uint32_t>();
+    a.dtot = c->dtot.as<uint32_t>();
+    a.deferred = c->deferred.as<uint32_t>();
+    a.tstart = c->tstart.as<uint32_t>();
+    a.tend = c->tend.as<uint32_t>();
+    a.ctr = c->ctr.as<DrawCounters>();
+    a.zeros = c->zeros.as<double>();
+    return FGL_OK;
+}
+
+int run_draw(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, const DevPrims &in, uint64_t info[2],
+             uint32_t *capture_ctr = nullptr) {
+    if (info) info[0] = info[1] = 0;
+    DrawArgs a;
+    const TileOps *Tp = nullptr;
+    int r = draw_begin(c, st, sh, kind, in, Tp, a);
+    if (r || in.n == 0) return r;
+    const TileOps &T = *Tp;
 
     if (!capture_ctr) {
         c->last_overflow = 0;
         c->last_reruns = 0;
     }
     for (int attempt = 0; attempt < 6; attempt++) {
-        HIPCHK(c, c->side_rec.ensure((size_t)c->side_rec_cap * sizeof(PrimRec)));
-        HIPCHK(c, c->side_vtx.ensure((size_t)c->side_vtx_cap * SIDE_VTX_DOUBLES * 8));
-        HIPCHK(c, c->keys0.ensure((size_t)c->pair_cap * 4));
-        HIPCHK(c, c->vals0.ensure((size_t)c->pair_cap * 4));
-        HIPCHK(c, c->keys1.ensure((size_t)c->pair_cap * 4));
-        HIPCHK(c, c->vals1.ensure((size_t)c->pair_cap * 4));
-        a.nrsblocks = (int32_t)(((uint64_t)c->pair_cap + RS_BLOCK - 1) / RS_BLOCK);
-        HIPCHK(c, c->hist.ensure((size_t)256 * a.nrsblocks * 4));
-        const size_t nsb = ((size_t)c->pair_cap + SPAN_THREADS - 1) / SPAN_THREADS + 1;
-        HIPCHK(c, c->pspans.ensure((size_t)c->pair_cap * 4));
-        HIPCHK(c, c->span_bsum.ensure(nsb * 8));
-        HIPCHK(c, c->span_boff.ensure(nsb * 8));
-        HIPCHK(c, c->poff.ensure(((size_t)c->pair_cap + 1) * 4));
-        HIPCHK(c, c->spans.ensure((size_t)c->span_cap * sizeof(Span)));
-        a.pspans = c->pspans.as<uint32_t>();
-        a.span_blocksum = c->span_bsum.as<uint64_t>();
-        a.span_blockoff = c->span_boff.as<uint64_t>();
-        a.poff = c->poff.as<uint32_t>();
-        a.spans = c->spans.as<Span>();
-        a.span_cap = c->span_cap;
-        const size_t nfb = ((size_t)c->span_cap + FRAG_BLOCK - 1) / FRAG_BLOCK + 1;
-        HIPCHK(c, c->frag_bsum.ensure(nfb * 8));
-        HIPCHK(c, c->frag_boff.ensure(nfb * 8));
-        HIPCHK(c, c->frag_gsum.ensure((nfb / 1024 + 2) * 8));
-        HIPCHK(c, c->foff.ensure(((size_t)c->span_cap + 1) * 4));
-        HIPCHK(c, c->frags.ensure((size_t)c->frag_cap * sizeof(Frag)));
-        a.frag_blocksum = c->frag_bsum.as<uint64_t>();
-        a.frag_blockoff = c->frag_boff.as<uint64_t>();
-        a.frag_gsum = c->frag_gsum.as<uint64_t>();
-        a.foff = c->foff.as<uint32_t>();
-        a.frags = c->frags.as<Frag>();
-        a.frag_cap = c->frag_cap;
-
-        a.rec_main = c->rec_main.as<PrimRec>();
-        a.side_rec = c->side_rec.as<PrimRec>();
-        a.side_rec_cap = c->side_rec_cap;
-        a.side_vtx = c->side_vtx.as<double>();
-        a.side_vtx_cap = c->side_vtx_cap;
-        a.ntl = c->ntl.as<uint32_t>();
-        a.nrast = c->nrast.as<uint32_t>();
-        a.sstart = c->sstart.as<uint32_t>();
-        a.blocksum = c->blocksum.as<uint64_t>();
-        a.blockoff = c->blockoff.as<uint64_t>();
-        a.keys[0] = c->keys0.as<uint32_t>();
-        a.vals[0] = c->vals0.as<uint32_t>();
-        a.keys[1] = c->keys1.as<uint32_t>();
-        a.vals[1] = c->vals1.as<uint32_t>();
-        a.pair_cap = c->pair_cap;
-        a.hist = c->hist.as<uint32_t>();
-        a.dtot = c->dtot.as<uint32_t>();
-        a.deferred = c->deferred.as<uint32_t>();
-        a.tstart = c->tstart.as<uint32_t>();
-        a.tend = c->tend.as<uint32_t>();
-        a.ctr = c->ctr.as<DrawCounters>();
-        a.zeros = c->zeros.as<double>();
+        if ((r = draw_buffers(c, a))) return r;
         if (attempt == 0 && !capture_ctr) {
             // failure injection: the kernels see smaller capacities than the
             // buffers hold, so an overflow check that is off cannot write
@@ -2598,11 +2618,41 @@ struct fgl_batch {
     std::vector<hipEvent_t> joins;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    uint64_t epoch = 0;  // g_dev_epoch the graph's addresses are valid for
+    uint64_t epoch = 0;  // g_dev_epoch the recorded addresses are valid for
     DevBuf ctrs;         // 2 DrawCounters per view (triangles, lines)
-    std::vector<DrawCounters> host;
+    HostBuf host;        // the same, read back
     std::string err;
+
+    // Replays launch each stage once for many views instead of replaying a
+    // graph with every view's launches.  The views' draws are split into
+    // rounds (a context's draws go to successive rounds, in view order) and,
+    // within a round, into groups of one tile shape and kernel variant; a
+    // group's launches take a device table of its draws' arguments and, per
+    // launch, the split of its workgroups among them.
+    struct MLaunch {
+        int stage, pass;
+        uint32_t nblocks;
+        size_t view_of, first;  // byte offsets in mtab
+    };
+    struct MGroup {
+        const TileOps *T;
+        DrawArgs rep;  // one of the group's draws (the kernel variant)
+        size_t draws;  // byte offset of the DrawArgs table in mtab
+        std::vector<MLaunch> launches;
+    };
+    // After each round, its draws' counters are copied to their views' slots
+    // of ctrs (a context's counters are reset by its next draw).
+    struct MGather {
+        size_t groups_end;  // the round's last group + 1
+        int32_t count;
+        size_t src, slot;   // byte offsets in mtab
+    };
+    bool multi = false;  // replays use mgroups (else the graph)
+    std::vector<MGroup> mgroups;
+    std::vector<MGather> mgathers;
+    DevBuf mtab;
     ~fgl_batch() {
+        if (s0) (void)hipStreamSynchronize(s0);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         for (hipEvent_t e : joins) (void)hipEventDestroy(e);
@@ -2728,7 +2778,197 @@ int batch_record(fgl_batch *b) {
     return FGL_OK;
 }
 
+// Prepares every view's draws as a replay launches them (its clears
+// pending on its first draw) and builds the groups' tables.  Returns 1 when
+// a buffer moved while preparing (prepare again), 2 when the batch has a
+// view with an empty mesh (its clear is left to the graph).
+int batch_record_multi(fgl_batch *b) {
+    struct Item {
+        size_t view;
+        int kind;
+        const TileOps *T;
+        DrawArgs a;
+        int round;
+    };
+    const uint64_t epoch0 = g_dev_epoch.load();
+    std::vector<Item> items;
+    std::map<fgl_ctx *, int> used;  // draws per context so far
+    for (size_t i = 0; i < b->views.size(); i++) {
+        fgl_batch::View &v = b->views[i];
+        if (v.mesh->ntri == 0 && v.mesh->nline == 0) return 2;
+        v.ctx->clear_color = v.clear_color;
+        v.ctx->pend_color = true;
+        v.ctx->clear_depth = v.clear_depth;
+        v.ctx->pend_depth = true;
+        for (int kind : {(int)PRIM_TRI, (int)PRIM_LINE}) {
+            DevPrims p;
+            mesh_prims(v.mesh, kind == PRIM_LINE, p);
+            if (p.n == 0) continue;
+            Item it;
+            it.view = i;
+            it.kind = kind;
+            int r = draw_begin(v.ctx, &v.st, &v.sh, kind, p, it.T, it.a);
+            if (!r) r = draw_buffers(v.ctx, it.a);
+            if (r) {
+                set_err(b->err, "view %zu: %s", i, v.ctx->err.c_str());
+                return r;
+            }
+            it.round = used[v.ctx]++;
+            items.push_back(it);
+            v.ctx->pend_color = v.ctx->pend_depth = false;  // consumed by this draw
+        }
+    }
+    if (g_dev_epoch.load() != epoch0) return 1;
+
+    // groups: round, then tile shape and kernel variant, in order of first use
+    struct Key {
+        int round;
+        const TileOps *T;
+        int kind, wire, shk, tex;
+        bool operator<(const Key &o) const {
+            return std::tie(round, T, kind, wire, shk, tex) < std::tie(o.round, o.T, o.kind, o.wire, o.shk, o.tex);
+        }
+    };
+    std::map<Key, size_t> gid;
+    std::vector<std::vector<size_t>> members;
+    std::vector<Key> keys;
+    std::vector<size_t> round_end;  // groups of rounds 0..rd
+    int nround = 0;
+    for (const Item &it : items) nround = std::max(nround, it.round + 1);
+    for (int rd = 0; rd < nround; rd++, round_end.push_back(members.size()))
+        for (size_t k = 0; k < items.size(); k++) {
+            const Item &it = items[k];
+            if (it.round != rd) continue;
+            const Key key{rd, it.T, it.kind, it.a.st.wireframe ? 1 : 0, it.a.sh.kind, it.a.sh.has_tex ? 1 : 0};
+            auto f = gid.find(key);
+            if (f == gid.end()) {
+                f = gid.emplace(key, members.size()).first;
+                members.emplace_back();
+                keys.push_back(key);
+            }
+            members[f->second].push_back(k);
+        }
+
+    // the tables, each 256-byte aligned in one buffer
+    std::vector<uint8_t> tab;
+    auto put = [&](const void *src, size_t bytes) {
+        const size_t off = (tab.size() + 255) & ~(size_t)255;
+        tab.resize(off + bytes);
+        if (bytes) std::memcpy(tab.data() + off, src, bytes);
+        return off;
+    };
+    std::vector<fgl_batch::MGroup> groups;
+    for (size_t g = 0; g < members.size(); g++) {
+        const std::vector<size_t> &mem = members[g];
+        fgl_batch::MGroup G;
+        G.T = keys[g].T;
+        G.rep = items[mem[0]].a;
+        std::vector<DrawArgs> draws;
+        int maxpass = 0;
+        for (size_t k : mem) {
+            draws.push_back(items[k].a);
+            maxpass = std::max(maxpass, (items[k].a.key_bits + 7) / 8);
+        }
+        G.draws = put(draws.data(), draws.size() * sizeof(DrawArgs));
+        auto add = [&](int stage, int pass) {
+            std::vector<uint32_t> first(draws.size() + 1, 0), view_of;
+            for (size_t v = 0; v < draws.size(); v++) {
+                const DrawArgs &a = draws[v];
+                const bool in_pass = (a.key_bits + 7) / 8 > pass;
+                const bool radix = stage == ST_RADIX_HIST || stage == ST_RADIX_SCAN || stage == ST_RADIX_SCATTER;
+                const uint32_t nb = (!radix || in_pass) ? G.T->stage_blocks(stage, a, true) : 0u;
+                first[v] = (uint32_t)view_of.size();
+                view_of.insert(view_of.end(), nb, (uint32_t)v);
+            }
+            first[draws.size()] = (uint32_t)view_of.size();
+            fgl_batch::MLaunch L;
+            L.stage = stage;
+            L.pass = pass;
+            L.nblocks = (uint32_t)view_of.size();
+            L.first = put(first.data(), first.size() * 4);
+            L.view_of = put(view_of.data(), view_of.size() * 4);
+            G.launches.push_back(L);
+        };
+        // the stages of draw_pass, in its order
+        add(ST_ZERO, 0);
+        add(ST_VS, 0);
+        if (keys[g].kind == PRIM_TRI && !keys[g].wire) add(ST_VS_DEFERRED, 0);
+        add(ST_SCAN_TOP, 0);
+        add(ST_EMIT, 0);
+        for (int p = 0; p < maxpass; p++) {
+            add(ST_RADIX_HIST, p);
+            add(ST_RADIX_SCAN, p);
+            add(ST_RADIX_SCATTER, p);
+        }
+        add(ST_SPAN_COUNT, 0);
+        add(ST_SPAN_SCAN, 0);
+        add(ST_SPANS, 0);
+        add(ST_FRAG_SCAN, 0);
+        add(ST_SHADE, 0);
+        add(ST_ROP, 0);
+        groups.push_back(std::move(G));
+    }
+    // each round's counters, gathered into the per-view slots
+    std::vector<fgl_batch::MGather> gathers;
+    for (int rd = 0; rd < nround; rd++) {
+        std::vector<uint64_t> src;
+        std::vector<uint32_t> slot;
+        for (const Item &it : items) {
+            if (it.round != rd) continue;
+            src.push_back((uint64_t)(uintptr_t)it.a.ctr);
+            slot.push_back((uint32_t)(2 * it.view + (it.kind == PRIM_LINE ? 1 : 0)));
+        }
+        fgl_batch::MGather G;
+        G.groups_end = round_end[(size_t)rd];
+        G.count = (int32_t)src.size();
+        G.src = put(src.data(), src.size() * 8);
+        G.slot = put(slot.data(), slot.size() * 4);
+        gathers.push_back(G);
+    }
+
+    BCHK(b, b->mtab.ensure(tab.size()));
+    if (g_dev_epoch.load() != epoch0) return 1;  // the table itself may have moved a buffer's address
+    BCHK(b, hipMemcpy(b->mtab.p, tab.data(), tab.size(), hipMemcpyHostToDevice));
+    b->mgroups = std::move(groups);
+    b->mgathers = std::move(gathers);
+    b->multi = true;
+    b->epoch = g_dev_epoch.load();
+    return FGL_OK;
+}
+
+// One replay: the groups' launches, each round followed by its counters
+// gather, on s0.
+int batch_replay_multi(fgl_batch *b) {
+    uint8_t *base = b->mtab.as<uint8_t>();
+    size_t g = 0;
+    for (const fgl_batch::MGather &R : b->mgathers) {
+        for (; g < R.groups_end; g++) {
+            const fgl_batch::MGroup &G = b->mgroups[g];
+            for (const fgl_batch::MLaunch &L : G.launches) {
+                MultiArgs m;
+                m.draws = reinterpret_cast<const DrawArgs *>(base + G.draws);
+                m.view_of = reinterpret_cast<const uint32_t *>(base + L.view_of);
+                m.first = reinterpret_cast<const uint32_t *>(base + L.first);
+                m.pass = L.pass;
+                BCHK(b, G.T->multi(L.stage, m, L.nblocks, G.rep, b->s0));
+            }
+        }
+        BCHK(b, launch_gather_words(reinterpret_cast<const uint32_t *const *>(base + R.src),
+                                    reinterpret_cast<const uint32_t *>(base + R.slot), b->ctrs.as<uint32_t>(),
+                                    (int32_t)(sizeof(DrawCounters) / 4), R.count, b->s0));
+    }
+    return FGL_OK;
+}
+
 int batch_capture(fgl_batch *b) {
+    b->multi = false;
+    const char *env = std::getenv("FGL_BATCH_GRAPH");
+    const bool graph_only = env && env[0] == '1';
+    for (int attempt = 0; attempt < 3 && !graph_only; attempt++) {
+        const int r = batch_record_multi(b);
+        if (r == 2) break;  // a view with an empty mesh: the graph records its clear
+        if (r <= 0) return r;
+    }
     for (int attempt = 0; attempt < 3; attempt++) {
         const int r = batch_record(b);
         if (r <= 0) return r;
@@ -2797,6 +3037,7 @@ int fgl_batch_add(fgl_batch *b, fgl_ctx *ctx, const uint8_t clear_nrgba[4], doub
         (void)hipGraphExecDestroy(b->exec);
         b->exec = nullptr;
     }
+    b->multi = false;
     return FGL_OK;
 }
 
@@ -2812,7 +3053,7 @@ int fgl_batch_run(fgl_batch *b, uint64_t *info) {
     b->err.clear();
     std::vector<uint64_t> tmp(2 * n);
     uint64_t *out = info ? info : tmp.data();
-    if (!b->exec || b->epoch != g_dev_epoch.load()) return batch_first_run(b, out);
+    if (!(b->multi || b->exec) || b->epoch != g_dev_epoch.load()) return batch_first_run(b, out);
     // the replay follows whatever is already queued on the views' contexts
     // (a resolve or readback of the previous frame, a clear's fill)
     // (a stream with nothing pending, the usual case, needs no event)
@@ -2825,10 +3066,16 @@ int fgl_batch_run(fgl_batch *b, uint64_t *info) {
         BCHK(b, hipStreamWaitEvent(b->s0, c->xev, 0));
     }
     (void)hipGetLastError();  // a not-ready query is no error of a later launch
-    BCHK(b, hipGraphLaunch(b->exec, b->s0));
-    b->host.resize(2 * n);
-    BCHK(b, hipMemcpyAsync(b->host.data(), b->ctrs.p, 2 * n * sizeof(DrawCounters), hipMemcpyDeviceToHost, b->s0));
-    BCHK(b, hipStreamSynchronize(b->s0));
+    if (b->multi) {
+        const int r = batch_replay_multi(b);
+        if (r) return r;
+    } else {
+        BCHK(b, hipGraphLaunch(b->exec, b->s0));
+    }
+    BCHK(b, b->host.ensure(2 * n * sizeof(DrawCounters)));
+    BCHK(b, hipMemcpyAsync(b->host.p, b->ctrs.p, 2 * n * sizeof(DrawCounters), hipMemcpyDeviceToHost, b->s0));
+    BCHK(b, wait_stream(b->s0));
+    const DrawCounters *hc = b->host.as<DrawCounters>();
     // every view cleared its context as it was drawn (its ROP may have left
     // blocks at the clear values): a context's clear values are now its last
     // view's, whatever was cleared on it between runs
@@ -2844,7 +3091,7 @@ int fgl_batch_run(fgl_batch *b, uint64_t *info) {
     // capacities
     std::map<fgl_ctx *, size_t> first_ovf;
     for (size_t i = 0; i < n; i++) {
-        const DrawCounters &t = b->host[2 * i], &l = b->host[2 * i + 1];
+        const DrawCounters &t = hc[2 * i], &l = hc[2 * i + 1];
         const bool has_t = b->views[i].mesh->ntri > 0, has_l = b->views[i].mesh->nline > 0;
         if ((has_t && t.overflow) || (has_l && l.overflow)) first_ovf.emplace(b->views[i].ctx, i);
     }
@@ -2858,7 +3105,7 @@ int fgl_batch_run(fgl_batch *b, uint64_t *info) {
             if (r) return r;
             continue;
         }
-        const DrawCounters &t = b->host[2 * i], &l = b->host[2 * i + 1];
+        const DrawCounters &t = hc[2 * i], &l = hc[2 * i + 1];
         const bool has_t = v.mesh->ntri > 0, has_l = v.mesh->nline > 0;
         out[2 * i] = (has_t ? t.total : 0) + (has_l ? l.total : 0);
         out[2 * i + 1] = (has_t ? updated_pixels(t) : 0) + (has_l ? updated_pixels(l) : 0);

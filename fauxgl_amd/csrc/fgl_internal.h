@@ -95,6 +95,37 @@ constexpr int RS_THREADS = 256;
 constexpr int RS_ITEMS = 16;
 constexpr int RS_BLOCK = RS_THREADS * RS_ITEMS;
 
+// The launches of one draw, in order (the three radix stages once per pass).
+enum DrawStage : int {
+    ST_ZERO,
+    ST_VS,
+    ST_VS_DEFERRED,
+    ST_SCAN_TOP,
+    ST_EMIT,
+    ST_RADIX_HIST,
+    ST_RADIX_SCAN,
+    ST_RADIX_SCATTER,
+    ST_SPAN_COUNT,
+    ST_SPAN_SCAN,
+    ST_SPANS,
+    ST_FRAG_SCAN,
+    ST_SHADE,
+    ST_ROP,
+    ST_COUNT
+};
+
+// One launch of a stage for many draws (the views of a batch), each with its
+// own buffers: launch block b runs block b - first[v] of draw v = view_of[b]
+// (first: one entry per draw, then the total).  The draws of one launch use
+// the same tile shape and kernel variant (primitive kind, wireframe, shader
+// kind and texture); `pass` is the radix pass of the radix stages.
+struct MultiArgs {
+    const DrawArgs *draws;
+    const uint32_t *view_of;
+    const uint32_t *first;
+    int32_t pass;
+};
+
 // Kernel launchers (stream-ordered).  Each returns the hipError_t of the launch.
 
 // The draw pipeline's launchers, one set per tile shape (see FGL_TILE_NS).
@@ -113,6 +144,28 @@ hipError_t launch_spans(const DrawArgs &a, hipStream_t s);
 hipError_t launch_frag_scan(const DrawArgs &a, hipStream_t s);
 hipError_t launch_shade(const DrawArgs &a, hipStream_t s);
 hipError_t launch_tile(const DrawArgs &a, hipStream_t s);
+// workgroups of one stage's launch for draw a (radix stages: per pass);
+// `many`: the launch covers many draws
+uint32_t stage_blocks(int stage, const DrawArgs &a, bool many);
+// one launch of a stage over nblocks workgroups of many draws; rep is one
+// of the draws (it selects the kernel variant)
+hipError_t launch_multi(int stage, const MultiArgs &m, uint32_t nblocks, const DrawArgs &rep, hipStream_t s);
+// the pieces the two above are made of, one per stage
+uint32_t blocks_zero(const DrawArgs &a);
+uint32_t blocks_vs_deferred(const DrawArgs &a, bool many);
+uint32_t span_blocks(const DrawArgs &a);
+using MultiLaunch = hipError_t (*)(const MultiArgs &, uint32_t, const DrawArgs &, hipStream_t);
+hipError_t launch_zero_multi(const MultiArgs &m, uint32_t nb, const DrawArgs &rep, hipStream_t s);
+hipError_t launch_vs_multi(const MultiArgs &m, uint32_t nb, const DrawArgs &rep, hipStream_t s);
+hipError_t launch_vs_deferred_multi(const MultiArgs &m, uint32_t nb, const DrawArgs &rep, hipStream_t s);
+hipError_t launch_scan_top_multi(const MultiArgs &m, uint32_t nb, const DrawArgs &rep, hipStream_t s);
+hipError_t launch_emit_multi(const MultiArgs &m, uint32_t nb, const DrawArgs &rep, hipStream_t s);
+hipError_t launch_radix_hist_multi(const MultiArgs &m, uint32_t nb, const DrawArgs &rep, hipStream_t s);
+hipError_t launch_radix_scan_multi(const MultiArgs &m, uint32_t nb, const DrawArgs &rep, hipStream_t s);
+hipError_t launch_radix_scatter_multi(const MultiArgs &m, uint32_t nb, const DrawArgs &rep, hipStream_t s);
+hipError_t launch_span_count_multi(const MultiArgs &m, uint32_t nb, const DrawArgs &rep, hipStream_t s);
+hipError_t launch_span_scan_multi(const MultiArgs &m, uint32_t nb, const DrawArgs &rep, hipStream_t s);
+hipError_t launch_spans_multi(const MultiArgs &m, uint32_t nb, const DrawArgs &rep, hipStream_t s);
 }  // namespace FGL_TILE_NS
 
 using DrawLaunch = hipError_t (*)(const DrawArgs &, hipStream_t);
@@ -126,6 +179,8 @@ struct TileOps {
     DrawLaunch radix_scan;
     PassLaunch radix_scatter;
     DrawLaunch span_count, span_scan, spans, frag_scan, shade, rop;
+    uint32_t (*stage_blocks)(int stage, const DrawArgs &a, bool many);
+    hipError_t (*multi)(int stage, const MultiArgs &m, uint32_t nblocks, const DrawArgs &rep, hipStream_t s);
 };
 
 // The compiled shapes (width, height); 32 x 32 is the default.
@@ -135,6 +190,9 @@ FGL_TILE_SHAPES(FGL_DECLARE_TILE_OPS)
 #undef FGL_DECLARE_TILE_OPS
 
 hipError_t launch_copy_words(const uint32_t *src, uint32_t *dst, int32_t n, hipStream_t s);
+// count records of n words: record k from src[k] to dst + slot[k] * n
+hipError_t launch_gather_words(const uint32_t *const *src, const uint32_t *slot, uint32_t *dst, int32_t n,
+                               int32_t count, hipStream_t s);
 // Deferred clears per block.  A draw whose clear is still pending leaves the
 // blocks of the tiles no primitive reaches unwritten and marks them here; the
 // ROP starts a later tile's marked blocks from the clear value; a reader of

@@ -34,6 +34,19 @@ hipError_t launch_copy_words(const uint32_t *src, uint32_t *dst, int32_t n, hipS
     return hipGetLastError();
 }
 
+// one workgroup per record: record k's n words from src[k] to dst + slot[k] * n
+__global__ void k_gather_words(const uint32_t *const *src, const uint32_t *slot, uint32_t *dst, int32_t n) {
+    const uint32_t *s = src[blockIdx.x];
+    uint32_t *d = dst + (uint64_t)slot[blockIdx.x] * (uint64_t)n;
+    for (int32_t i = threadIdx.x; i < n; i += blockDim.x) d[i] = s[i];
+}
+hipError_t launch_gather_words(const uint32_t *const *src, const uint32_t *slot, uint32_t *dst, int32_t n,
+                               int32_t count, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_words, dim3((unsigned)count), dim3(64), 0, s, src, slot, dst, n);
+    return hipGetLastError();
+}
+
 // one workgroup per block: its marked channels get the clear values
 __global__ __launch_bounds__(256) void k_cblk_fill(uint32_t *color, double *depth, int32_t W, int32_t H,
                                                   uint8_t *cblk, int32_t cbw, uint32_t cc, double cd, uint8_t chans) {

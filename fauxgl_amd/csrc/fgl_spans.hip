@@ -11,6 +11,7 @@
 #include "fgl_internal.h"
 #include "fgl_scan.h"
 #include "fgl_tiles.h"
+#include "fgl_draws.h"
 
 namespace fgl {
 namespace FGL_TILE_NS {
@@ -39,16 +40,17 @@ FGL_D uint32_t pair_spans(const DrawArgs &a, const uint32_t *keys, const uint32_
 // Spans per pair, summed per block of SPAN_THREADS pairs; also each tile's
 // range in the sorted pairs, and the zeroed fragment counts per span block
 // (the whole capacity, grid-wide) that k_spans accumulates into.
-__global__ __launch_bounds__(SPAN_THREADS) void k_span_count(DrawArgs a) {
+template <class G>
+FGL_D void span_count_body(const DrawArgs &a, const G &g) {
     __shared__ uint64_t wsum[SPAN_THREADS / 64];
     if (a.ctr->overflow) return;
     const uint64_t nfb = ((uint64_t)a.span_cap + FRAG_BLOCK - 1) / FRAG_BLOCK;
-    for (uint64_t k = (uint64_t)blockIdx.x * SPAN_THREADS + threadIdx.x; k < nfb; k += (uint64_t)gridDim.x * SPAN_THREADS)
+    for (uint64_t k = (uint64_t)g.bid() * SPAN_THREADS + threadIdx.x; k < nfb; k += (uint64_t)g.nblk() * SPAN_THREADS)
         a.frag_blocksum[k] = 0;
     const uint32_t E = a.ctr->npairs;
     const uint32_t *keys = a.keys[a.sorted_buf];
     const uint32_t *vals = a.vals[a.sorted_buf];
-    uint32_t p = blockIdx.x * SPAN_THREADS + threadIdx.x;
+    uint32_t p = g.bid() * SPAN_THREADS + threadIdx.x;
     uint32_t n = p < E ? pair_spans(a, keys, vals, p) : 0u;
     if (p < E) {
         a.pspans[p] = n;
@@ -64,13 +66,15 @@ __global__ __launch_bounds__(SPAN_THREADS) void k_span_count(DrawArgs a) {
     if (threadIdx.x == 0) {
         uint64_t s = 0;
         for (int w = 0; w < SPAN_THREADS / 64; w++) s += wsum[w];
-        a.span_blocksum[blockIdx.x] = s;
+        a.span_blocksum[g.bid()] = s;
     }
 }
+FGL_DRAW_KERNELS(k_span_count, span_count_body, __launch_bounds__(SPAN_THREADS))
 
 // Exclusive scan of the per-block span sums (one workgroup), total and
 // capacity check.
-__global__ __launch_bounds__(SCAN_THREADS) void k_span_scan(DrawArgs a) {
+template <class G>
+FGL_D void span_scan_body(const DrawArgs &a, const G &) {
     __shared__ uint64_t lds[SCAN_THREADS / 64];
     if (a.ctr->overflow) return;
     const uint32_t E = a.ctr->npairs;
@@ -84,6 +88,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_span_scan(DrawArgs a) {
         a.poff[E] = total > a.span_cap ? a.span_cap : (uint32_t)total;
     }
 }
+FGL_DRAW_KERNELS(k_span_scan, span_scan_body, __launch_bounds__(SCAN_THREADS))
 
 // edge(a, b, c) (context.go:147-149)
 FGL_D double edge(double ax, double ay, double bx, double by, double cx, double cy) {
@@ -238,7 +243,8 @@ FGL_D Span pair_span(const PairSetup &q, uint32_t m, int32_t W, uint32_t &cnt_ou
 // position, a max-scan carries the mark) and reads the pair's setup from
 // LDS.  Also adds the covered pixels into the fragment count of each span
 // block it writes.
-__global__ __launch_bounds__(SPAN_THREADS) void k_spans(DrawArgs a) {
+template <class G>
+FGL_D void spans_body(const DrawArgs &a, const G &g) {
     __shared__ uint64_t wsum[SPAN_THREADS / 64];
     __shared__ uint32_t lfb[FB_SLOTS];
     __shared__ uint32_t ostart[SPAN_THREADS / 64][64];
@@ -247,7 +253,7 @@ __global__ __launch_bounds__(SPAN_THREADS) void k_spans(DrawArgs a) {
     const uint32_t E = a.ctr->npairs;
     const uint32_t *keys = a.keys[a.sorted_buf];
     const uint32_t *vals = a.vals[a.sorted_buf];
-    const uint32_t p = blockIdx.x * SPAN_THREADS + threadIdx.x;
+    const uint32_t p = g.bid() * SPAN_THREADS + threadIdx.x;
     const uint32_t n = p < E ? a.pspans[p] : 0u;
     for (uint32_t k = threadIdx.x; k < FB_SLOTS; k += SPAN_THREADS) lfb[k] = 0;
     // block-exclusive scan of n
@@ -259,7 +265,7 @@ __global__ __launch_bounds__(SPAN_THREADS) void k_spans(DrawArgs a) {
     }
     if (lane == 63) wsum[wave] = v;
     __syncthreads();
-    const uint64_t bbase = a.span_blockoff[blockIdx.x];
+    const uint64_t bbase = a.span_blockoff[g.bid()];
     const uint32_t blk0 = (uint32_t)(bbase / FRAG_BLOCK);
     uint64_t wbase = bbase;
     for (int w = 0; w < wave; w++) wbase += wsum[w];
@@ -296,9 +302,10 @@ __global__ __launch_bounds__(SPAN_THREADS) void k_spans(DrawArgs a) {
     for (uint32_t k = threadIdx.x; k < FB_SLOTS; k += SPAN_THREADS)
         if (lfb[k]) atomicAdd((unsigned long long *)&a.frag_blocksum[blk0 + k], (unsigned long long)lfb[k]);
 }
+FGL_DRAW_KERNELS(k_spans, spans_body, __launch_bounds__(SPAN_THREADS))
 
-static unsigned span_blocks(const DrawArgs &a) {
-    return (unsigned)(((uint64_t)a.pair_cap + SPAN_THREADS - 1) / SPAN_THREADS);
+uint32_t span_blocks(const DrawArgs &a) {
+    return (uint32_t)(((uint64_t)a.pair_cap + SPAN_THREADS - 1) / SPAN_THREADS);
 }
 
 hipError_t launch_span_count(const DrawArgs &a, hipStream_t s) {
@@ -311,6 +318,18 @@ hipError_t launch_span_scan(const DrawArgs &a, hipStream_t s) {
 }
 hipError_t launch_spans(const DrawArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_spans, dim3(span_blocks(a)), dim3(SPAN_THREADS), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_span_count_multi(const MultiArgs &m, uint32_t nb, const DrawArgs &, hipStream_t s) {
+    hipLaunchKernelGGL(k_span_count_m, dim3(nb), dim3(SPAN_THREADS), 0, s, m);
+    return hipGetLastError();
+}
+hipError_t launch_span_scan_multi(const MultiArgs &m, uint32_t nb, const DrawArgs &, hipStream_t s) {
+    hipLaunchKernelGGL(k_span_scan_m, dim3(nb), dim3(SCAN_THREADS), 0, s, m);
+    return hipGetLastError();
+}
+hipError_t launch_spans_multi(const MultiArgs &m, uint32_t nb, const DrawArgs &, hipStream_t s) {
+    hipLaunchKernelGGL(k_spans_m, dim3(nb), dim3(SPAN_THREADS), 0, s, m);
     return hipGetLastError();
 }
 

@@ -19,6 +19,7 @@
 #include "fgl_internal.h"
 #include "fgl_scan.h"
 #include "fgl_tiles.h"
+#include "fgl_draws.h"
 
 namespace fgl {
 namespace FGL_TILE_NS {
@@ -342,16 +343,6 @@ FGL_D C4 eval(const DevShader &sh, const ShadeIn &in, V4 b) {
     }
 }
 
-// The kernel's arguments read again from the kernarg segment.  The empty
-// asm makes the pointer opaque, so the loads through it stay at their uses
-// inside the shading loop (scalar loads each pass) instead of being hoisted
-// in front of it into scalar registers that the loop then spills.
-FGL_D const DrawArgs &args_reload() {
-    auto p = (const __attribute__((address_space(4))) DrawArgs *)__builtin_amdgcn_kernarg_segment_ptr();
-    asm volatile("" : "+s"(p));
-    return *(const DrawArgs *)p;
-}
-
 FGL_D const PrimRec *prim_of(const DrawArgs &a, uint32_t ref) {
     return (ref & REF_SIDE) ? a.side_rec + (ref & ~REF_SIDE) : a.rec_main + ref;
 }
@@ -372,12 +363,13 @@ constexpr int FRAG_GROUP = 1024;
 // FRAG_GTAB * FRAG_GROUP * FRAG_BLOCK spans, add their sums one by one).
 constexpr int FRAG_GTAB = 256;
 
-__global__ __launch_bounds__(FRAG_GROUP) void k_frag_scan(DrawArgs a) {
+template <class G>
+FGL_D void frag_scan_body(const DrawArgs &a, const G &grid) {
     __shared__ uint64_t wsum[FRAG_GROUP / 64];
     if (a.ctr->overflow) return;
     const uint32_t NS = a.ctr->nspans;
     const uint32_t nb = (NS + FRAG_BLOCK - 1) / FRAG_BLOCK;
-    const uint32_t g = blockIdx.x;
+    const uint32_t g = grid.bid();
     if ((uint64_t)g * FRAG_GROUP >= nb) return;
     const uint32_t b = g * FRAG_GROUP + threadIdx.x;
     const uint64_t v = b < nb ? a.frag_blocksum[b] : 0;
@@ -398,6 +390,7 @@ __global__ __launch_bounds__(FRAG_GROUP) void k_frag_scan(DrawArgs a) {
     if (b < nb) a.frag_blockoff[b] = base + incl - v;
     if (threadIdx.x == 0) a.frag_gsum[g] = total;
 }
+FGL_DRAW_KERNELS(k_frag_scan, frag_scan_body, __launch_bounds__(FRAG_GROUP))
 
 // ---------------------------------------------------------------------------
 // shading
@@ -534,8 +527,8 @@ FGL_D void load_extra(const DevShader &sh, const AttrSrc &src, uint64_t attr, Sh
     }
 }
 
-template <int SH>
-__global__ __launch_bounds__(SPAN_THREADS, SH == 3 ? FGL_SHADE_WAVES : 1) void k_shade(DrawArgs a) {
+template <int SH, class G>
+FGL_D void shade_body(const DrawArgs &a, const G &grid) {
     using L = SlotLayout<SH>;
     __shared__ uint32_t ostart[SPAN_THREADS / 64][64];
     __shared__ double slots[SPAN_THREADS / 64][SHADE_SLOTS][L::N];
@@ -568,7 +561,7 @@ __global__ __launch_bounds__(SPAN_THREADS, SH == 3 ? FGL_SHADE_WAVES : 1) void k
         total += gt;
         __syncthreads();
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (grid.bid() == 0 && threadIdx.x == 0) {
         if (total > a.frag_cap) atomicOr(&a.ctr->overflow, 32u);
         a.ctr->nfrags = total > a.frag_cap ? a.frag_cap : (uint32_t)total;
         a.ctr->frags_lo = (uint32_t)total;
@@ -582,7 +575,7 @@ __global__ __launch_bounds__(SPAN_THREADS, SH == 3 ? FGL_SHADE_WAVES : 1) void k
     // in the loop, so a wave with few fragments does not wait for one with
     // many
     constexpr uint32_t WAVES = SPAN_THREADS / 64;
-    for (uint32_t blk = blockIdx.x * WAVES + (uint32_t)wave; blk < nb; blk += gridDim.x * WAVES) {
+    for (uint32_t blk = grid.bid() * WAVES + (uint32_t)wave; blk < nb; blk += grid.nblk() * WAVES) {
         const uint32_t i = blk * FRAG_BLOCK + (uint32_t)lane;
         const Span sp = i < NS ? a.spans[i] : Span{{0, 0, 0}, 0, 0, 0};
         const uint32_t n = sp.n & SPAN_N;
@@ -616,7 +609,7 @@ __global__ __launch_bounds__(SPAN_THREADS, SH == 3 ? FGL_SHADE_WAVES : 1) void k
         const bool staged = T >= 2u * (uint32_t)__popcll(lm);
         const uint32_t myslot = staged ? before + (lead ? 1u : 0u) - 1u : ~0u;  // lanes >= NS own no fragments
         if (staged && lead && before < (uint32_t)SHADE_SLOTS) {
-            const DrawArgs &ka = args_reload();
+            const DrawArgs &ka = grid.args();
             const PrimRec *pr = prim_of(ka, sp.ref);
             TriIn t;
             tri_from_rec(pr, t);
@@ -647,7 +640,7 @@ __global__ __launch_bounds__(SPAN_THREADS, SH == 3 ? FGL_SHADE_WAVES : 1) void k
             double w1 = __shfl(sp.w[1], ow);
             double w2 = __shfl(sp.w[2], ow);
             if (p < T) {
-                const DrawArgs &ka = args_reload();
+                const DrawArgs &ka = grid.args();
                 const AttrSrc src = attr_src(ka);
                 const uint32_t j = p - oexcl;
                 TriIn t;
@@ -683,6 +676,15 @@ __global__ __launch_bounds__(SPAN_THREADS, SH == 3 ? FGL_SHADE_WAVES : 1) void k
         }
     }
 }
+template <int SH>
+__global__ __launch_bounds__(SPAN_THREADS, SH == 3 ? FGL_SHADE_WAVES : 1) void k_shade(DrawArgs a) {
+    shade_body<SH>(a, OneDraw{});
+}
+template <int SH>
+__global__ __launch_bounds__(SPAN_THREADS, SH == 3 ? FGL_SHADE_WAVES : 1) void k_shade_m(MultiArgs m) {
+    const ManyDraws g = many_draws(m);
+    shade_body<SH>(draw_of(g), g);
+}
 
 // ---------------------------------------------------------------------------
 // the ordered framebuffer update, one workgroup per tile
@@ -691,13 +693,14 @@ __global__ __launch_bounds__(SPAN_THREADS, SH == 3 ? FGL_SHADE_WAVES : 1) void k
 #define FGL_ROP_WAVES 1
 #endif
 
-__global__ __launch_bounds__(TILE_THREADS, FGL_ROP_WAVES) void k_rop(DrawArgs a) {
+template <class G>
+FGL_D void rop_body(const DrawArgs &a, const G &g) {
     __shared__ uint32_t col[TILE_PIX];
     __shared__ double dep[TILE_PIX];
     __shared__ uint32_t owner[TILE_PIX];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int t = blockIdx.x;
+    const int t = (int)g.bid();
     if (a.ctr->overflow) return;  // the host reruns the whole draw
     __shared__ uint32_t upd[TILE_THREADS / 64];
     const int tx = t % a.ntx, ty = t / a.ntx;
@@ -828,6 +831,7 @@ __global__ __launch_bounds__(TILE_THREADS, FGL_ROP_WAVES) void k_rop(DrawArgs a)
         if (s) atomicAdd(&a.ctr->upd[t % UPD_PARTS], s);
     }
 }
+FGL_DRAW_KERNELS(k_rop, rop_body, __launch_bounds__(TILE_THREADS, FGL_ROP_WAVES))
 
 
 // ---------------------------------------------------------------------------
@@ -836,30 +840,84 @@ static unsigned span_grid(const DrawArgs &a) {
     const uint64_t nb = ((uint64_t)a.span_cap + SPAN_THREADS - 1) / SPAN_THREADS;  // workgroups of 4 waves
     return (unsigned)(nb < SPAN_GRID_MAX ? (nb ? nb : 1) : SPAN_GRID_MAX);
 }
-
-hipError_t launch_frag_scan(const DrawArgs &a, hipStream_t s) {
+static unsigned frag_groups(const DrawArgs &a) {
     const uint64_t nb = ((uint64_t)a.span_cap + FRAG_BLOCK - 1) / FRAG_BLOCK;
     const unsigned groups = (unsigned)((nb + FRAG_GROUP - 1) / FRAG_GROUP);
-    hipLaunchKernelGGL(k_frag_scan, dim3(groups ? groups : 1), dim3(FRAG_GROUP), 0, s, a);
+    return groups ? groups : 1;
+}
+
+hipError_t launch_frag_scan(const DrawArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_frag_scan, dim3(frag_groups(a)), dim3(FRAG_GROUP), 0, s, a);
     return hipGetLastError();
 }
 hipError_t launch_shade(const DrawArgs &a, hipStream_t s) {
     dim3 grid(span_grid(a)), block(SPAN_THREADS);
-#define K_SHADE k_shade
     switch (a.sh.kind) {
-        case 1: hipLaunchKernelGGL(K_SHADE<1>, grid, block, 0, s, a); break;
-        case 2: hipLaunchKernelGGL(K_SHADE<2>, grid, block, 0, s, a); break;
+        case 1: hipLaunchKernelGGL(k_shade<1>, grid, block, 0, s, a); break;
+        case 2: hipLaunchKernelGGL(k_shade<2>, grid, block, 0, s, a); break;
         default:
-            if (a.sh.has_tex) hipLaunchKernelGGL(K_SHADE<4>, grid, block, 0, s, a);
-            else hipLaunchKernelGGL(K_SHADE<3>, grid, block, 0, s, a);
+            if (a.sh.has_tex) hipLaunchKernelGGL(k_shade<4>, grid, block, 0, s, a);
+            else hipLaunchKernelGGL(k_shade<3>, grid, block, 0, s, a);
             break;
     }
-#undef K_SHADE
     return hipGetLastError();
 }
 hipError_t launch_tile(const DrawArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_rop, dim3(a.ntiles), dim3(TILE_THREADS), 0, s, a);
     return hipGetLastError();
+}
+
+static hipError_t launch_frag_scan_multi(const MultiArgs &m, uint32_t nb, const DrawArgs &, hipStream_t s) {
+    hipLaunchKernelGGL(k_frag_scan_m, dim3(nb), dim3(FRAG_GROUP), 0, s, m);
+    return hipGetLastError();
+}
+static hipError_t launch_shade_multi(const MultiArgs &m, uint32_t nb, const DrawArgs &rep, hipStream_t s) {
+    dim3 grid(nb), block(SPAN_THREADS);
+    switch (rep.sh.kind) {
+        case 1: hipLaunchKernelGGL(k_shade_m<1>, grid, block, 0, s, m); break;
+        case 2: hipLaunchKernelGGL(k_shade_m<2>, grid, block, 0, s, m); break;
+        default:
+            if (rep.sh.has_tex) hipLaunchKernelGGL(k_shade_m<4>, grid, block, 0, s, m);
+            else hipLaunchKernelGGL(k_shade_m<3>, grid, block, 0, s, m);
+            break;
+    }
+    return hipGetLastError();
+}
+static hipError_t launch_rop_multi(const MultiArgs &m, uint32_t nb, const DrawArgs &, hipStream_t s) {
+    hipLaunchKernelGGL(k_rop_m, dim3(nb), dim3(TILE_THREADS), 0, s, m);
+    return hipGetLastError();
+}
+
+uint32_t stage_blocks(int stage, const DrawArgs &a, bool many) {
+    switch (stage) {
+        case ST_ZERO: return blocks_zero(a);
+        case ST_VS: return (uint32_t)a.nvsblocks;
+        case ST_VS_DEFERRED: return blocks_vs_deferred(a, many);
+        case ST_SCAN_TOP: return 1;
+        case ST_EMIT: return (uint32_t)a.nvsblocks;
+        case ST_RADIX_HIST: return (uint32_t)a.nrsblocks;
+        case ST_RADIX_SCAN: return 256;
+        case ST_RADIX_SCATTER: return (uint32_t)a.nrsblocks;
+        case ST_SPAN_COUNT: return span_blocks(a);
+        case ST_SPAN_SCAN: return 1;
+        case ST_SPANS: return span_blocks(a);
+        case ST_FRAG_SCAN: return frag_groups(a);
+        case ST_SHADE: return span_grid(a);
+        case ST_ROP: return (uint32_t)a.ntiles;
+        default: return 0;
+    }
+}
+
+hipError_t launch_multi(int stage, const MultiArgs &m, uint32_t nblocks, const DrawArgs &rep, hipStream_t s) {
+    static const MultiLaunch table[ST_COUNT] = {
+        launch_zero_multi,       launch_vs_multi,         launch_vs_deferred_multi, launch_scan_top_multi,
+        launch_emit_multi,       launch_radix_hist_multi, launch_radix_scan_multi,  launch_radix_scatter_multi,
+        launch_span_count_multi, launch_span_scan_multi,  launch_spans_multi,       launch_frag_scan_multi,
+        launch_shade_multi,      launch_rop_multi,
+    };
+    if (stage < 0 || stage >= ST_COUNT) return hipErrorInvalidValue;
+    if (nblocks == 0) return hipSuccess;
+    return table[stage](m, nblocks, rep, s);
 }
 
 }  // namespace FGL_TILE_NS
@@ -886,6 +944,8 @@ const TileOps &FGL_OPS1(FGL_TILE_W, FGL_TILE_H)() {
         FGL_TILE_NS::launch_frag_scan,
         FGL_TILE_NS::launch_shade,
         FGL_TILE_NS::launch_tile,
+        FGL_TILE_NS::stage_blocks,
+        FGL_TILE_NS::launch_multi,
     };
     return ops;
 }

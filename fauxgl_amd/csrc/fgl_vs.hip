@@ -7,6 +7,7 @@
 // order; the file is compiled with -ffp-contract=off.
 #include "fgl_internal.h"
 #include "fgl_tiles.h"
+#include "fgl_draws.h"
 
 namespace fgl {
 namespace FGL_TILE_NS {
@@ -379,10 +380,11 @@ FGL_D void vs_triangle(const DrawArgs &a, int64_t i, uint32_t &nr, uint32_t &nt,
 // Filled triangles without clipping in one lean pass; triangles with a
 // vertex outside the view volume are queued for k_vs_deferred, which fills
 // in their counts afterwards.
-__global__ __launch_bounds__(VS_THREADS) void k_vs_fast(DrawArgs a) {
+template <class G>
+FGL_D void vs_fast_body(const DrawArgs &a, const G &g) {
     __shared__ uint64_t wsum[VS_THREADS / 64];
     const int lane = threadIdx.x & 63;
-    int64_t i = (int64_t)blockIdx.x * VS_THREADS + threadIdx.x;
+    int64_t i = (int64_t)g.bid() * VS_THREADS + threadIdx.x;
     uint32_t nr = 0, nt = 0;
     bool defer = false;
     if (i < a.in.n) {
@@ -430,9 +432,10 @@ __global__ __launch_bounds__(VS_THREADS) void k_vs_fast(DrawArgs a) {
     if (threadIdx.x == 0) {
         uint64_t s = 0;
         for (int w = 0; w < VS_THREADS / 64; w++) s += wsum[w];
-        a.blocksum[blockIdx.x] = s;
+        a.blocksum[g.bid()] = s;
     }
 }
+FGL_DRAW_KERNELS(k_vs_fast, vs_fast_body, __launch_bounds__(VS_THREADS))
 
 // Exclusive position of this lane's `want` items in a block of the counter
 // reserved for the whole wave (one atomic per wave).  All 64 lanes call it.
@@ -453,11 +456,12 @@ FGL_D uint32_t wave_reserve(uint32_t *counter, uint32_t want) {
 // The triangles k_vs_fast queued, one per lane: clip (polygon in LDS),
 // reserve the side records and vertices of the whole wave at once, then
 // build and emit the fans.
-__global__ __launch_bounds__(64) void k_vs_deferred(DrawArgs a) {
+template <class G>
+FGL_D void vs_deferred_body(const DrawArgs &a, const G &g) {
     __shared__ V4 clipbuf[64][2][10];
     const uint32_t n = a.ctr->ndeferred;
     V4(*buf)[10] = clipbuf[threadIdx.x];
-    for (uint32_t q0 = blockIdx.x * 64; q0 < n; q0 += gridDim.x * 64) {
+    for (uint32_t q0 = g.bid() * 64; q0 < n; q0 += g.nblk() * 64) {
         const uint32_t q = q0 + threadIdx.x;
         const bool act = q < n;
         const int64_t i = act ? (int64_t)a.deferred[q] : 0;
@@ -511,6 +515,7 @@ __global__ __launch_bounds__(64) void k_vs_deferred(DrawArgs a) {
         }
     }
 }
+FGL_DRAW_KERNELS(k_vs_deferred, vs_deferred_body, __launch_bounds__(64))
 
 // One input line (DrawLine, context.go:351-368).
 FGL_D void vs_line(const DrawArgs &a, int64_t i, uint32_t &nr, uint32_t &nt, uint32_t &ss) {
@@ -537,10 +542,10 @@ FGL_D void vs_line(const DrawArgs &a, int64_t i, uint32_t &nr, uint32_t &nt, uin
     ss = rb;
 }
 
-template <int KIND>
-__global__ __launch_bounds__(VS_THREADS) void k_vs(DrawArgs a) {
+template <int KIND, class G>
+FGL_D void vs_body(const DrawArgs &a, const G &g) {
     __shared__ uint64_t wsum[VS_THREADS / 64];
-    int64_t i = (int64_t)blockIdx.x * VS_THREADS + threadIdx.x;
+    int64_t i = (int64_t)g.bid() * VS_THREADS + threadIdx.x;
     uint32_t nr = 0, nt = 0, ss = 0;
     if (i < a.in.n) {
         if (KIND == PRIM_TRI) vs_triangle(a, i, nr, nt, ss);
@@ -557,8 +562,17 @@ __global__ __launch_bounds__(VS_THREADS) void k_vs(DrawArgs a) {
     if (threadIdx.x == 0) {
         uint64_t s = 0;
         for (int w = 0; w < VS_THREADS / 64; w++) s += wsum[w];
-        a.blocksum[blockIdx.x] = s;
+        a.blocksum[g.bid()] = s;
     }
+}
+template <int KIND>
+__global__ __launch_bounds__(VS_THREADS) void k_vs(DrawArgs a) {
+    vs_body<KIND>(a, OneDraw{});
+}
+template <int KIND>
+__global__ __launch_bounds__(VS_THREADS) void k_vs_m(MultiArgs m) {
+    const ManyDraws g = many_draws(m);
+    vs_body<KIND>(draw_of(g), g);
 }
 
 hipError_t launch_vs(const DrawArgs &a, hipStream_t s) {
@@ -571,11 +585,32 @@ hipError_t launch_vs(const DrawArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_vs_deferred(const DrawArgs &a, hipStream_t s) {
+hipError_t launch_vs_multi(const MultiArgs &m, uint32_t nb, const DrawArgs &rep, hipStream_t s) {
+    if (rep.kind == PRIM_TRI && !rep.st.wireframe)
+        hipLaunchKernelGGL(k_vs_fast_m, dim3(nb), dim3(VS_THREADS), 0, s, m);
+    else if (rep.kind == PRIM_TRI)
+        hipLaunchKernelGGL(k_vs_m<PRIM_TRI>, dim3(nb), dim3(VS_THREADS), 0, s, m);
+    else
+        hipLaunchKernelGGL(k_vs_m<PRIM_LINE>, dim3(nb), dim3(VS_THREADS), 0, s, m);
+    return hipGetLastError();
+}
+
+// The queued triangles are taken 64 per block in a grid-stride loop.  A
+// launch for many draws gives each fewer blocks: most draws queue few
+// triangles, and every block holds 40 KB of LDS while it runs.
+uint32_t blocks_vs_deferred(const DrawArgs &a, bool many) {
     int64_t blocks = (a.in.n + 63) / 64;
-    if (blocks > 4096) blocks = 4096;
+    const int64_t cap = many ? 16 : 4096;
+    if (blocks > cap) blocks = cap;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_vs_deferred, dim3((unsigned)blocks), dim3(64), 0, s, a);
+    return (uint32_t)blocks;
+}
+hipError_t launch_vs_deferred(const DrawArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_vs_deferred, dim3(blocks_vs_deferred(a, false)), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+hipError_t launch_vs_deferred_multi(const MultiArgs &m, uint32_t nb, const DrawArgs &, hipStream_t s) {
+    hipLaunchKernelGGL(k_vs_deferred_m, dim3(nb), dim3(64), 0, s, m);
     return hipGetLastError();
 }
 
