@@ -2609,6 +2609,7 @@ struct fgl_batch {
         fgl_shader sh;
         uint32_t clear_color;
         double clear_depth;
+        uint32_t ndeferred = 0;  // triangles its last direct draw queued for clipping
     };
     std::vector<View> views;
     std::vector<fgl_ctx *> ctxs;  // the distinct contexts, in first-use order
@@ -2697,6 +2698,7 @@ int batch_eager(fgl_batch *b, size_t i, uint64_t info[2]) {
     mesh_prims(v.mesh, true, l);
     uint64_t i1[2] = {0, 0}, i2[2] = {0, 0};
     int r = run_draw(v.ctx, &v.st, &v.sh, PRIM_TRI, t, i1);
+    if (!r && t.n > 0) v.ndeferred = v.ctx->last.ndeferred;
     if (!r) r = run_draw(v.ctx, &v.st, &v.sh, PRIM_LINE, l, i2);
     if (!r && t.n == 0 && l.n == 0) r = materialize(v.ctx);
     if (r) {
@@ -2789,6 +2791,7 @@ int batch_record_multi(fgl_batch *b) {
         const TileOps *T;
         DrawArgs a;
         int round;
+        uint32_t ndeferred;
     };
     const uint64_t epoch0 = g_dev_epoch.load();
     std::vector<Item> items;
@@ -2814,6 +2817,7 @@ int batch_record_multi(fgl_batch *b) {
                 return r;
             }
             it.round = used[v.ctx]++;
+            it.ndeferred = v.ndeferred;
             items.push_back(it);
             v.ctx->pend_color = v.ctx->pend_depth = false;  // consumed by this draw
         }
@@ -2864,9 +2868,11 @@ int batch_record_multi(fgl_batch *b) {
         G.T = keys[g].T;
         G.rep = items[mem[0]].a;
         std::vector<DrawArgs> draws;
+        std::vector<uint32_t> ndef;
         int maxpass = 0;
         for (size_t k : mem) {
             draws.push_back(items[k].a);
+            ndef.push_back(items[k].ndeferred);
             maxpass = std::max(maxpass, (items[k].a.key_bits + 7) / 8);
         }
         G.draws = put(draws.data(), draws.size() * sizeof(DrawArgs));
@@ -2876,7 +2882,10 @@ int batch_record_multi(fgl_batch *b) {
                 const DrawArgs &a = draws[v];
                 const bool in_pass = (a.key_bits + 7) / 8 > pass;
                 const bool radix = stage == ST_RADIX_HIST || stage == ST_RADIX_SCAN || stage == ST_RADIX_SCATTER;
-                const uint32_t nb = (!radix || in_pass) ? G.T->stage_blocks(stage, a, true) : 0u;
+                uint32_t nb = (!radix || in_pass) ? G.T->stage_blocks(stage, a, true) : 0u;
+                // the clipping stage's grid-stride loop takes any count; its
+                // grid follows the count of the view's direct draw
+                if (stage == ST_VS_DEFERRED) nb = std::max(nb, std::min<uint32_t>((ndef[v] + ndef[v] / 4 + 63) / 64, 4096u));
                 first[v] = (uint32_t)view_of.size();
                 view_of.insert(view_of.end(), nb, (uint32_t)v);
             }
