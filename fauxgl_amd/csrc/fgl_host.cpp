@@ -152,7 +152,7 @@ struct fgl_ctx {
     DevBuf ctr, ntl, nrast, sstart, blocksum, blockoff, rec_main, side_rec, side_vtx;
     HostBuf hctr;  // the draw counters as the host reads them back
     DevBuf keys0, vals0, keys1, vals1, hist, dtot, tstart, tend, deferred;
-    DevBuf pspans, span_bsum, span_boff, poff, spans;
+    DevBuf pspans, prows, span_bsum, span_boff, poff, spans;
     DevBuf frag_bsum, frag_boff, frag_gsum, foff, frags;
     uint32_t span_cap = 0, frag_cap = 0;
     DevBuf in_pos, in_nrm, in_uv, in_col;
@@ -623,11 +623,13 @@ int draw_buffers(fgl_ctx *c, DrawArgs &a) {
     HIPCHK(c, c->hist.ensure((size_t)256 * a.nrsblocks * 4));
     const size_t nsb = ((size_t)c->pair_cap + SPAN_THREADS - 1) / SPAN_THREADS + 1;
     HIPCHK(c, c->pspans.ensure((size_t)c->pair_cap * 4));
+    HIPCHK(c, c->prows.ensure((size_t)c->pair_cap * 2));
     HIPCHK(c, c->span_bsum.ensure(nsb * 8));
     HIPCHK(c, c->span_boff.ensure(nsb * 8));
     HIPCHK(c, c->poff.ensure(((size_t)c->pair_cap + 1) * 4));
     HIPCHK(c, c->spans.ensure((size_t)c->span_cap * sizeof(Span)));
     a.pspans = c->pspans.as<uint32_t>();
+    a.prows = c->prows.as<uint16_t>();
     a.span_blocksum = c->span_bsum.as<uint64_t>();
     a.span_blockoff = c->span_boff.as<uint64_t>();
     a.poff = c->poff.as<uint32_t>();

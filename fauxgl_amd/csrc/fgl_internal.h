@@ -58,6 +58,7 @@ struct DrawArgs {
 
     // coverage spans
     uint32_t *pspans;           // spans per sorted pair
+    uint16_t *prows;            // per sorted pair: its rows that have spans (ROWS_ALL: every row)
     uint64_t *span_blocksum;    // per SPAN_THREADS pairs
     uint64_t *span_blockoff;
     uint32_t *poff;             // first span of each sorted pair (pair_cap + 1 entries)

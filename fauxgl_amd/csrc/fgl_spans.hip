@@ -1,6 +1,7 @@
 // fgl_spans.hip -- coverage spans.  After binning, every (tile, raster
 // triangle) pair is expanded into one span per row the triangle's box
-// reaches inside the tile: the reference's row walk (context.go:184-221) is
+// reaches inside the tile (for a small box, per row whose pixel centers are
+// not all surely outside the triangle): the reference's row walk (context.go:184-221) is
 // run for that row, with the row recurrence carried from row to row and the
 // per-triangle setup (edge values, steps, reciprocals, context.go:162-181)
 // done once per pair.  A span records the first covered pixel inside the
@@ -28,11 +29,56 @@ FGL_D void tile_rect(const DrawArgs &a, uint32_t t, int32_t &cx0, int32_t &cx1, 
     cy1 = min(cy0 + TILE_H, a.H) - 1;
 }
 
-FGL_D uint32_t pair_spans(const DrawArgs &a, const uint32_t *keys, const uint32_t *vals, uint32_t p) {
+// The rows of a pair that get spans.  A triangle whose box is at most 24
+// pixels square and whose rows in the tile do not wrap (nk == 1, at most 15
+// rows) skips every row whose pixel centers inside the tile are all surely
+// outside it (edge_margin): a row whose center line misses the triangle's
+// vertical extent, and, in a box at most 8 pixels wide, a row whose centers
+// each fail the edge test.  Such a row's span would cover nothing.  Bit r
+// stands for row ya + r; ROWS_ALL keeps all nY * nk rows.
+constexpr uint16_t ROWS_ALL = 0x8000;
+
+FGL_D uint16_t pair_rows(const PrimRec *pr, const ItemGeo &g, int32_t W, int32_t cx0, int32_t cx1) {
+    if (g.nk != 1 || g.nY > 15) return ROWS_ALL;
+    const double s0x = pr->sx[0], s0y = pr->sy[0], s1x = pr->sx[1], s1y = pr->sy[1], s2x = pr->sx[2],
+                 s2y = pr->sy[2];
+    const EdgeMargin m = edge_margin(s0x, s0y, s1x, s1y, s2x, s2y, pr->x0, pr->y0, pr->x1, pr->y1, 23);
+    if (!m.ok) return ROWS_ALL;
+    const double lo = __builtin_fmin(s0y, __builtin_fmin(s1y, s2y));
+    const double hi = __builtin_fmax(s0y, __builtin_fmax(s1y, s2y));
+    const int32_t kw = g.kmin * W;
+    const int32_t xa = pr->x0 > kw + cx0 ? pr->x0 : kw + cx0;
+    const int32_t xb = pr->x1 < kw + cx1 ? pr->x1 : kw + cx1;
+    const bool narrow = xb - xa <= 7;
+    // the three edge functions (signed by sg) at the center of (xa, ya), and
+    // their steps per pixel in x and in y; stepping adds at most 40 roundings
+    // of values below 700, far inside tol
+    const double px = (double)xa + 0.5, py = (double)g.ya + 0.5;
+    double e0 = m.sg * ((s2x - px) * (s1y - py) - (s2y - py) * (s1x - px));  // edge(s1, s2, p)
+    double e1 = m.sg * ((s0x - px) * (s2y - py) - (s0y - py) * (s2x - px));  // edge(s2, s0, p)
+    double e2 = m.sg * ((s1x - px) * (s0y - py) - (s1y - py) * (s0x - px));  // edge(s0, s1, p)
+    const double dx0 = m.sg * (s2y - s1y), dx1 = m.sg * (s0y - s2y), dx2 = m.sg * (s1y - s0y);
+    const double dy0 = m.sg * (s1x - s2x), dy1 = m.sg * (s2x - s0x), dy2 = m.sg * (s0x - s1x);
+    uint16_t rows = 0;
+    for (int32_t r = 0; r < g.nY; r++, e0 += dy0, e1 += dy1, e2 += dy2) {
+        if (row_surely_outside(m, lo, hi, g.ya + r)) continue;
+        bool any = !narrow;
+        double f0 = e0, f1 = e1, f2 = e2;
+        for (int32_t x = xa; x <= xb && !any; x++, f0 += dx0, f1 += dx1, f2 += dx2)
+            any = !(f0 < -m.tol || f1 < -m.tol || f2 < -m.tol);
+        if (any) rows |= (uint16_t)(1u << r);
+    }
+    return rows;
+}
+
+// Spans of pair p: the rows pair_rows keeps (times the wraps).
+FGL_D uint32_t pair_spans(const DrawArgs &a, const uint32_t *keys, const uint32_t *vals, uint32_t p, uint16_t &rows) {
     int32_t cx0, cx1, cy0, cy1;
     tile_rect(a, keys[p], cx0, cx1, cy0, cy1);
     const PrimRec *pr = prim_rec(a, vals[p]);
     ItemGeo g = item_geo(pr->x0, pr->y0, pr->x1, pr->y1, a.W, cx0, cx1, cy0, cy1);
+    rows = pair_rows(pr, g, a.W, cx0, cx1);
+    if (rows != ROWS_ALL) return (uint32_t)__builtin_popcount(rows);
     uint64_t n = (uint64_t)g.nY * (uint64_t)g.nk;
     return n > 0xffffffull ? 0xffffffu : (uint32_t)n;
 }
@@ -51,9 +97,11 @@ FGL_D void span_count_body(const DrawArgs &a, const G &g) {
     const uint32_t *keys = a.keys[a.sorted_buf];
     const uint32_t *vals = a.vals[a.sorted_buf];
     uint32_t p = g.bid() * SPAN_THREADS + threadIdx.x;
-    uint32_t n = p < E ? pair_spans(a, keys, vals, p) : 0u;
+    uint16_t rows = ROWS_ALL;
+    uint32_t n = p < E ? pair_spans(a, keys, vals, p, rows) : 0u;
     if (p < E) {
         a.pspans[p] = n;
+        a.prows[p] = rows;
         // the tile's [start, end) in the sorted pairs
         const uint32_t t = keys[p];
         if (p == 0 || keys[p - 1] != t) a.tstart[t] = p;
@@ -119,6 +167,7 @@ struct PairSetup {
     int32_t x0, x1, ya, kmin, nk;
     int32_t cx0, cx1, cy0, cy1;
     uint32_t ref;
+    uint32_t rows;  // pair_rows of the pair
 };
 
 FGL_D void pair_setup(const DrawArgs &a, const uint32_t *keys, const uint32_t *vals, uint32_t p, PairSetup &q) {
@@ -164,15 +213,23 @@ FGL_D void pair_setup(const DrawArgs &a, const uint32_t *keys, const uint32_t *v
     q.cy0 = cy0;
     q.cy1 = cy1;
     q.ref = ref;
+    q.rows = a.prows[p];
 }
 
-// Span m of a pair (row yy = m / nk, wrapped column kk = m % nk): the row's
+// Span m of a pair (row yy = m / nk, wrapped column kk = m % nk, or the m-th
+// row pair_rows kept): the row's
 // edge values (yy row steps from ya), the skip to the first pixel that can
 // be inside, then the pixel walk of the reference's row loop
 // (context.go:185-221) through the tile's columns.  Returns the span and
 // its covered-pixel count.
 FGL_D Span pair_span(const PairSetup &q, uint32_t m, int32_t W, uint32_t &cnt_out) {
-    const int32_t yy = (int32_t)(m / (uint32_t)q.nk), kk = (int32_t)(m % (uint32_t)q.nk);
+    int32_t yy = (int32_t)(m / (uint32_t)q.nk), kk = (int32_t)(m % (uint32_t)q.nk);
+    if (q.rows != ROWS_ALL) {  // the m-th kept row (nk == 1)
+        uint32_t r = q.rows;
+        for (uint32_t k = 0; k < m; k++) r &= r - 1;
+        yy = __builtin_ctz(r);
+        kk = 0;
+    }
     double w00 = q.w0, w01 = q.w1, w02 = q.w2;
     for (int32_t r = 0; r < yy; r++) {
         w00 += q.b12;

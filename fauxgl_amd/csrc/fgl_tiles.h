@@ -65,4 +65,64 @@ FGL_D ItemGeo item_geo(int32_t x0, int32_t y0, int32_t x1, int32_t y1, int32_t W
     return g;
 }
 
+// A conservative form of the reference's inside test (context.go:184-228),
+// used to drop work that provably yields no pixel.  For a triangle whose box
+// is at most `lim` + 1 pixels square (lim <= 23), a pixel center p is surely
+// outside when one edge function at p has the sign opposite to
+// edge(s0, s1, s2) (the sign of ra) by more than tol = (M + 16) * 1e-8, M
+// bounding the coordinates: every way rasterize() evaluates that edge
+// function there (the corner value, the row and pixel steps, the skip) stays
+// within 1e-10 + 1e-13 * M of the exact value for a box this small, as does
+// the direct evaluation here, so the reference's b for that edge is negative
+// at p.  ok is false for a degenerate or non-finite triangle, which is left
+// to the walk.
+struct EdgeMargin {
+    bool ok;
+    double sg, tol, area;
+};
+
+FGL_D EdgeMargin edge_margin(double s0x, double s0y, double s1x, double s1y, double s2x, double s2y, int64_t x0,
+                             int64_t y0, int64_t x1, int64_t y1, int64_t lim) {
+    EdgeMargin m;
+    m.ok = false;
+    m.sg = 1;
+    m.tol = 0;
+    m.area = 0;
+    if (x1 - x0 > lim || y1 - y0 > lim) return m;
+    const double area = (s1x - s2x) * (s0y - s2y) - (s1y - s2y) * (s0x - s2x);
+    if (!(__builtin_fabs(area) > 0) || !__builtin_isfinite(area)) return m;
+    double M = __builtin_fmax(__builtin_fabs((double)x0), __builtin_fabs((double)x1));
+    M = __builtin_fmax(M, __builtin_fmax(__builtin_fabs((double)y0), __builtin_fabs((double)y1)));
+    M = __builtin_fmax(M, __builtin_fmax(__builtin_fabs(s0x), __builtin_fabs(s0y)));
+    M = __builtin_fmax(M, __builtin_fmax(__builtin_fabs(s1x), __builtin_fabs(s1y)));
+    M = __builtin_fmax(M, __builtin_fmax(__builtin_fabs(s2x), __builtin_fabs(s2y)));
+    m.ok = true;
+    m.sg = area > 0 ? 1.0 : -1.0;
+    m.tol = (M + 16.0) * 1e-8;
+    m.area = __builtin_fabs(area);
+    return m;
+}
+
+FGL_D bool surely_outside(const EdgeMargin &m, double s0x, double s0y, double s1x, double s1y, double s2x,
+                          double s2y, int64_t x, int64_t y) {
+    const double px = (double)x + 0.5, py = (double)y + 0.5;
+    const double e0 = m.sg * ((s2x - px) * (s1y - py) - (s2y - py) * (s1x - px));  // edge(s1, s2, p)
+    const double e1 = m.sg * ((s0x - px) * (s2y - py) - (s0y - py) * (s2x - px));  // edge(s2, s0, p)
+    const double e2 = m.sg * ((s1x - px) * (s0y - py) - (s1y - py) * (s0x - px));  // edge(s0, s1, p)
+    return e0 < -m.tol || e1 < -m.tol || e2 < -m.tol;
+}
+
+// Whether every pixel center of row y is surely outside because the row's
+// center line y + 0.5 misses the triangle's vertical extent [lo, hi] by
+// delta > 0.  With P such a center, P = sum b_k s_k and sum b_k = 1 give
+// sum over negative b_k of |b_k| >= delta / (delta + h), h = hi - lo, so one
+// edge function has magnitude >= area * delta / (2 (delta + h)) on the
+// outside; asking 4 (delta + h) tol of it leaves the rounding of delta, h
+// and the area far behind.
+FGL_D bool row_surely_outside(const EdgeMargin &m, double lo, double hi, int64_t y) {
+    const double py = (double)y + 0.5;
+    const double delta = py > hi ? py - hi : lo - py;
+    return delta > 0 && m.area * delta > 4.0 * (delta + (hi - lo)) * m.tol;
+}
+
 }  // namespace fgl

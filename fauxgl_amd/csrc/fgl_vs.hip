@@ -46,34 +46,14 @@ FGL_D V3 load_pos(const DevPrims &in, int64_t vid) {
 // Raster triangle records
 
 // True for a small box none of whose pixel centers the reference's inside
-// test can accept (context.go:184-228): at each center (x + 0.5, y + 0.5) of
-// [x0, x1] x [y0, y1] one edge function has the sign opposite to
-// edge(s0, s1, s2), the sign of ra, by a margin (M + 16) * 1e-8, where M
-// bounds the coordinates.  Every way rasterize() evaluates that edge function
-// at that center (the corner value, the row and pixel steps, the skip) stays
-// within about 1e-14 * M of the exact value for a box this small, and so
-// does the direct evaluation here; so the reference's b for that edge is
-// negative there and the pixel is skipped.  A degenerate or non-finite
-// triangle is left to the walk.
+// test can accept: every center of [x0, x1] x [y0, y1] is surely outside
+// (edge_margin).  Such a rasterize() call touches no pixel.
 FGL_D bool surely_empty(V3 s0, V3 s1, V3 s2, int64_t x0, int64_t y0, int64_t x1, int64_t y1) {
-    if (x1 - x0 > 3 || y1 - y0 > 3) return false;
-    const double area = (s1.x - s2.x) * (s0.y - s2.y) - (s1.y - s2.y) * (s0.x - s2.x);
-    if (!(__builtin_fabs(area) > 0) || !__builtin_isfinite(area)) return false;
-    const double sg = area > 0 ? 1.0 : -1.0;
-    double M = __builtin_fmax(__builtin_fabs((double)x0), __builtin_fabs((double)x1));
-    M = __builtin_fmax(M, __builtin_fmax(__builtin_fabs((double)y0), __builtin_fabs((double)y1)));
-    M = __builtin_fmax(M, __builtin_fmax(__builtin_fabs(s0.x), __builtin_fabs(s0.y)));
-    M = __builtin_fmax(M, __builtin_fmax(__builtin_fabs(s1.x), __builtin_fabs(s1.y)));
-    M = __builtin_fmax(M, __builtin_fmax(__builtin_fabs(s2.x), __builtin_fabs(s2.y)));
-    const double tol = (M + 16.0) * 1e-8;
+    const EdgeMargin m = edge_margin(s0.x, s0.y, s1.x, s1.y, s2.x, s2.y, x0, y0, x1, y1, 3);
+    if (!m.ok) return false;
     for (int64_t y = y0; y <= y1; y++)
-        for (int64_t x = x0; x <= x1; x++) {
-            const double px = (double)x + 0.5, py = (double)y + 0.5;
-            const double e0 = sg * ((s2.x - px) * (s1.y - py) - (s2.y - py) * (s1.x - px));  // edge(s1, s2, p)
-            const double e1 = sg * ((s0.x - px) * (s2.y - py) - (s0.y - py) * (s2.x - px));  // edge(s2, s0, p)
-            const double e2 = sg * ((s1.x - px) * (s0.y - py) - (s1.y - py) * (s0.x - px));  // edge(s0, s1, p)
-            if (!(e0 < -tol || e1 < -tol || e2 < -tol)) return false;
-        }
+        for (int64_t x = x0; x <= x1; x++)
+            if (!surely_outside(m, s0.x, s0.y, s1.x, s1.y, s2.x, s2.y, x, y)) return false;
     return true;
 }
 
