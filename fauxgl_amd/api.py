@@ -618,8 +618,10 @@ class DeviceMesh:
 class Batch:
     """Many views rendered per step (fgl_batch): each view is a context, its
     clear color, its current state and shader, and a resident mesh.  The
-    first Run draws them one by one and records them into one HIP graph;
-    later Runs replay it.  Run returns [RasterizeInfo] per view."""
+    first Run draws them one by one (sizing every context's buffers) and
+    prepares the views' draws for replay; later Runs launch each pipeline
+    stage once per group of views with one tile shape and kernel variant.
+    Run returns [RasterizeInfo] per view."""
 
     def __init__(self):
         h = ctypes.c_void_p()

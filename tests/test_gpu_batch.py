@@ -1,8 +1,9 @@
-"""Batched views (fgl_batch): a batch of frames recorded into one HIP graph
-and replayed gives, on every replay, the same pixels, depth and
-RasterizeInfo as drawing each view on its own (the path the parity tests
-check against the oracle).  Each case prints its name before touching the
-GPU."""
+"""Batched views (fgl_batch): a batch of frames prepared once and replayed
+(one launch per pipeline stage for each group of views; one recorded HIP
+graph when a view's mesh is empty) gives, on every replay, the same pixels,
+depth and RasterizeInfo as drawing each view on its own (the path the parity
+tests check against the oracle).  Each case prints its name before touching
+the GPU."""
 import math
 import sys
 
