@@ -212,10 +212,12 @@ int fgl_tune_clear(void);
  * A batch is a list of views: (context, clear color and depth, state,
  * shader, resident mesh) = ClearColorBufferWith + ClearDepthBufferWith +
  * DrawMesh on that context.  The first fgl_batch_run draws every view one
- * by one, which sizes each context's buffers, and records the whole batch
- * into one HIP graph (views on different contexts run as independent
- * branches); later runs replay the graph.  A view whose frame no longer fits
- * is redrawn directly and the graph recorded again.  info: 2 per view
+ * by one, which sizes each context's buffers, and prepares the views' draws;
+ * later runs launch each pipeline stage once for all views of one tile shape
+ * and kernel variant (a context's views in successive rounds, in order).  A
+ * batch with a view whose mesh is empty is recorded into one HIP graph of
+ * per-view launches instead.  A view whose frame no longer fits is redrawn
+ * directly and the batch prepared again.  info: 2 per view
  * (TotalPixels, UpdatedPixels), may be NULL.  A view's context must not be
  * used by other calls while a run is in flight. */
 typedef struct fgl_batch fgl_batch;
