@@ -29,7 +29,10 @@ constexpr int TILE_H = FGL_TILE_H;
 #define FGL_TNS1(w, h) FGL_TNS2(w, h)
 #define FGL_TILE_NS FGL_TNS1(FGL_TILE_W, FGL_TILE_H)
 constexpr int TILE_PIX = TILE_W * TILE_H;
-constexpr int TILE_THREADS = 256;
+// Threads of the framebuffer-update workgroup (one per tile): a 128 x 64
+// tile holds 128 KB of LDS, one workgroup per CU, so it gets 16 waves to hide
+// its loads; small tiles share a CU among several workgroups of 4 waves.
+constexpr int TILE_THREADS = TILE_PIX >= 8192 ? 1024 : (TILE_PIX >= 4096 ? 512 : 256);
 // Raster triangles whose integer bounding box leaves [-BBOX_LIM, BBOX_LIM]
 // are dropped: the reference only produces such boxes from NaN/Inf screen
 // coordinates, where its loops never end or wrap int64.
