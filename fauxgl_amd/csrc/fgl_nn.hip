@@ -150,6 +150,16 @@ FGL_D uint32_t pack_bf2(float a, float b) {
     return __builtin_bit_cast(uint32_t, v);
 }
 FGL_D float relu(float v) { return v > 0.0f ? v : 0.0f; }
+// pack_bf2(relu(a), relu(b)) for every a, b but NaN, in fewer instructions:
+// rounding to BF16 keeps the sign, and a BF16 is negative (or -0) exactly
+// when its bits are negative as a 16-bit integer
+typedef short i16x2 __attribute__((ext_vector_type(2)));
+FGL_D uint32_t pack_relu_bf2(float a, float b) {
+    uint32_t u = pack_bf2(a, b);
+    asm("" : "+v"(u));  // keeps the two conversions one v_cvt_pk_bf16_f32
+    const i16x2 v = __builtin_bit_cast(i16x2, u);
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(v, i16x2{0, 0}));
+}
 FGL_D float keep_if(uint32_t m16, float v) { return ((m16 & 0x8000u) == 0 && (m16 & 0x7fffu) != 0) ? v : 0.0f; }
 
 template <int CIN, int COUT, int EPI, bool FEAT>
@@ -855,10 +865,8 @@ __global__ __launch_bounds__(512) void k_nn_stream(const uint32_t *color, const 
                         const int c = 16 * b + pm, gx = x0 + c;
                         uint4 o = make_uint4(0, 0, 0, 0);
                         if (gx >= 0 && gx < W)
-                            o = make_uint4(pack_bf2(relu(acc[0][0]), relu(acc[0][1])),
-                                           pack_bf2(relu(acc[0][2]), relu(acc[0][3])),
-                                           pack_bf2(relu(acc[1][0]), relu(acc[1][1])),
-                                           pack_bf2(relu(acc[1][2]), relu(acc[1][3])));
+                            o = make_uint4(pack_relu_bf2(acc[0][0], acc[0][1]), pack_relu_bf2(acc[0][2], acc[0][3]),
+                                           pack_relu_bf2(acc[1][0], acc[1][1]), pack_relu_bf2(acc[1][2], acc[1][3]));
                         *reinterpret_cast<uint4 *>(dst + ns_chunk(c + 1, kg)) = o;
                     }
                 }
@@ -902,6 +910,10 @@ __global__ __launch_bounds__(512) void k_nn_stream(const uint32_t *color, const 
                     // between block b's MFMAs
                     bf16x8 fa[9], fb[9];
                     ns_load9(rows, 0, pm, kg, fa);
+                    // block 0's nine loads go out back to back; without this
+                    // group the interleave below would pair each of them with
+                    // the two MFMAs that wait for it
+                    __builtin_amdgcn_sched_group_barrier(0x100, 9, 0);
 #pragma unroll
                     for (int b = 0; b < NS_NB; b++) {
                         bf16x8(&cur)[9] = (b & 1) ? fb : fa;
@@ -917,10 +929,8 @@ __global__ __launch_bounds__(512) void k_nn_stream(const uint32_t *color, const 
                         const int c = 16 * b + pm, gx = x0 + c;
                         uint4 o = make_uint4(0, 0, 0, 0);
                         if (gx >= 0 && gx < W)
-                            o = make_uint4(pack_bf2(relu(acc[0][0]), relu(acc[0][1])),
-                                           pack_bf2(relu(acc[0][2]), relu(acc[0][3])),
-                                           pack_bf2(relu(acc[1][0]), relu(acc[1][1])),
-                                           pack_bf2(relu(acc[1][2]), relu(acc[1][3])));
+                            o = make_uint4(pack_relu_bf2(acc[0][0], acc[0][1]), pack_relu_bf2(acc[0][2], acc[0][3]),
+                                           pack_relu_bf2(acc[1][0], acc[1][1]), pack_relu_bf2(acc[1][2], acc[1][3]));
                         *reinterpret_cast<uint4 *>(dst + ns_chunk(c + 1, kg)) = o;
                     }
                 }
@@ -947,6 +957,7 @@ __global__ __launch_bounds__(512) void k_nn_stream(const uint32_t *color, const 
             const uint32_t *crow = reinterpret_cast<const uint32_t *>(color_ring + (r & 15) * NS_COLOR_ROW);
             bf16x8 fa[9], fb[9];
             ns_load9(rows, blk0, pm, kg, fa);
+            __builtin_amdgcn_sched_group_barrier(0x100, 9, 0);  // the first block's loads back to back
 #pragma unroll
             for (int k = 0; k < NS_NB / 2; k++) {
                 const int b = blk0 + k;
