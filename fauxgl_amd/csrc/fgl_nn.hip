@@ -836,6 +836,18 @@ __global__ __launch_bounds__(512) void k_nn_stream(const uint32_t *color, const 
         for (int t2 = 0; t2 < 2; t2++)
             bias[t2] = f32x4{b0[8 * kg + 4 * t2], b0[8 * kg + 4 * t2 + 1], b0[8 * kg + 4 * t2 + 2],
                              b0[8 * kg + 4 * t2 + 3]};
+        // lane (pm, kg) reads tap 4 g + kg of its pixel.  Taps 9..11 (g = 2,
+        // kg > 0) have zero weights; they read tap kg - 1 instead (row r - 1,
+        // complete and finite), so each product is an exact zero, the sums
+        // are those of a zero operand up to the sign of a zero, and the ReLU
+        // makes every zero +0.
+        int trow[3], tcol[3];
+#pragma unroll
+        for (int g = 0; g < 3; g++) {
+            const int tap = 4 * g + kg < 9 ? 4 * g + kg : kg - 1;
+            trow[g] = tap / 3;
+            tcol[g] = (pm + tap % 3) * 16;
+        }
         __syncthreads();  // rings zeroed
         for (int t = 0; t < nsteps; t++) {
             const int r = y0 - 8 + t;
@@ -845,23 +857,27 @@ __global__ __launch_bounds__(512) void k_nn_stream(const uint32_t *color, const 
                     for (int q = lane; q < NS_CW * 4; q += 64)
                         *reinterpret_cast<uint4 *>(dst + ns_chunk(1 + (q >> 2), q & 3)) = make_uint4(0, 0, 0, 0);
                 } else {
+                    const uint8_t *base[3];
+#pragma unroll
+                    for (int g = 0; g < 3; g++) base[g] = feat_ring + ((r - 1 + trow[g]) & 3) * NS_FEAT_ROW + tcol[g];
+                    // block b + 1's three loads go out before block b's MFMAs
+                    bf16x8 fa[3], fb[3];
+#pragma unroll
+                    for (int g = 0; g < 3; g++) fa[g] = *reinterpret_cast<const bf16x8 *>(base[g]);
 #pragma unroll
                     for (int b = 0; b < NS_NB; b++) {
-                        bf16x8 f[3];
+                        bf16x8(&cur)[3] = (b & 1) ? fb : fa;
+                        bf16x8(&nxt)[3] = (b & 1) ? fa : fb;
+                        if (b + 1 < NS_NB)
 #pragma unroll
-                        for (int g = 0; g < 3; g++) {
-                            const int tap = 4 * g + kg;
-                            const int dy = tap / 3, dx = tap - 3 * (tap / 3);
-                            f[g] = tap < 9 ? *reinterpret_cast<const bf16x8 *>(
-                                                 feat_ring + ((r - 1 + dy) & 3) * NS_FEAT_ROW + (16 * b + pm + dx) * 16)
-                                           : bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
-                        }
+                            for (int g = 0; g < 3; g++)
+                                nxt[g] = *reinterpret_cast<const bf16x8 *>(base[g] + 256 * (b + 1));
                         f32x4 acc[2] = {bias[0], bias[1]};
 #pragma unroll
                         for (int g = 0; g < 3; g++)
 #pragma unroll
                             for (int t2 = 0; t2 < 2; t2++)
-                                acc[t2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[g][t2], f[g], acc[t2], 0, 0, 0);
+                                acc[t2] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[g][t2], cur[g], acc[t2], 0, 0, 0);
                         const int c = 16 * b + pm, gx = x0 + c;
                         uint4 o = make_uint4(0, 0, 0, 0);
                         if (gx >= 0 && gx < W)
