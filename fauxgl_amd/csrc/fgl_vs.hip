@@ -220,6 +220,55 @@ __device__ __noinline__ int sutherland_hodgman(V4 (*buf)[10], int *which) {
     return n;
 }
 
+// sutherlandHodgman (clipping.go:28-52) for the clipping kernel: the same
+// point tests, intersections and output order as sutherland_hodgman, with
+// fewer dependent steps per point.  Each plane reads the polygon's points
+// into registers at once and computes every intersection without branching
+// (the ones used are those the reference computes); the outputs are written
+// without branches, a write the reference does not make going to slot 10.
+// A plane that no lane of the wave clips is skipped (it would copy every
+// polygon unchanged).  buf[0][0..2] holds the triangle on entry.
+constexpr int SH_SLOTS = 11;
+FGL_D int sutherland_hodgman_ilp(V4 (*buf)[SH_SLOTS], int *which) {
+    int n = 3;
+    int cur = 0;
+    for (int p = 0; p < 6; p++) {
+        const V4 *input = buf[cur];
+        V4 *output = buf[cur ^ 1];
+        V4 pt[9];
+        bool fr[9];
+        bool clips = false;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            pt[k] = input[k];
+            fr[k] = point_in_front(p, pt[k]);
+            clips |= k < n && !fr[k];
+        }
+        if (__ballot(clips) == 0) continue;
+        const V4 s0 = input[n > 0 ? n - 1 : 0];
+        V4 x[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++) x[k] = intersect_segment(p, k == 0 ? s0 : pt[k - 1], pt[k]);
+        bool sf = point_in_front(p, s0);
+        int m = 0;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            const bool act = k < n, ef = fr[k];
+            const bool wx = act && ef != sf && m < 10;
+            output[wx ? m : 10] = x[k];
+            m += wx ? 1 : 0;
+            const bool we = act && ef && m < 10;
+            output[we ? m : 10] = pt[k];
+            m += we ? 1 : 0;
+            sf = act ? ef : sf;
+        }
+        n = m;
+        cur ^= 1;
+    }
+    *which = cur;
+    return n;
+}
+
 // Barycentric (vertex.go:81-95)
 FGL_D V4 barycentric(V3 p1, V3 p2, V3 p3, V3 p) {
     V3 v0 = vsub(p2, p1);
@@ -451,20 +500,26 @@ FGL_D void vs_fast_body(const DrawArgs &a, const G &g) {
 }
 FGL_DRAW_KERNELS(k_vs_fast, vs_fast_body, __launch_bounds__(VS_THREADS))
 
-// Exclusive position of this lane's `want` items in a block of the counter
-// reserved for the whole wave (one atomic per wave).  All 64 lanes call it.
-FGL_D uint32_t wave_reserve(uint32_t *counter, uint32_t want) {
+// Exclusive positions of this lane's nrec side records and 3 nrec side
+// vertices in blocks of the two counters reserved for the whole wave (one
+// atomic on each, both in flight together).  All 64 lanes call it.
+FGL_D void wave_reserve_fans(const DrawArgs &a, uint32_t nrec, uint32_t &rb, uint32_t &vb) {
     const int lane = threadIdx.x & 63;
-    uint32_t incl = want;
+    uint32_t incl = nrec;
     for (int off = 1; off < 64; off <<= 1) {
         const uint32_t y = __shfl_up(incl, off);
         if (lane >= off) incl += y;
     }
     const uint32_t total = __shfl(incl, 63);
-    uint32_t base = 0;
-    if (lane == 63 && total) base = atomicAdd(counter, total);
-    base = __shfl(base, 63);
-    return base + incl - want;
+    uint32_t r0 = 0, v0 = 0;
+    if (lane == 63 && total) {
+        r0 = atomicAdd(&a.ctr->side_rec, total);
+        v0 = atomicAdd(&a.ctr->side_vtx, 3 * total);
+    }
+    r0 = __shfl(r0, 63);
+    v0 = __shfl(v0, 63);
+    rb = r0 + incl - nrec;
+    vb = v0 + 3 * (incl - nrec);
 }
 
 // The triangles k_vs_fast queued, one per lane: clip (polygon in LDS),
@@ -472,9 +527,9 @@ FGL_D uint32_t wave_reserve(uint32_t *counter, uint32_t want) {
 // build and emit the fans.
 template <class G>
 FGL_D void vs_deferred_body(const DrawArgs &a, const G &g) {
-    __shared__ V4 clipbuf[64][2][10];
+    __shared__ V4 clipbuf[64][2][SH_SLOTS];
     const uint32_t n = a.ctr->ndeferred;
-    V4(*buf)[10] = clipbuf[threadIdx.x];
+    V4(*buf)[SH_SLOTS] = clipbuf[threadIdx.x];
     for (uint32_t q0 = g.bid() * 64; q0 < n; q0 += g.nblk() * 64) {
         const uint32_t q = q0 + threadIdx.x;
         const bool act = q < n;
@@ -489,11 +544,11 @@ FGL_D void vs_deferred_body(const DrawArgs &a, const G &g) {
             buf[0][0] = o0;
             buf[0][1] = o1;
             buf[0][2] = o2;
-            np = sutherland_hodgman(buf, &which);
+            np = sutherland_hodgman_ilp(buf, &which);
         }
         const uint32_t nfan = np >= 3 ? (uint32_t)(np - 2) : 0u;
-        const uint32_t rb = wave_reserve(&a.ctr->side_rec, nfan);
-        const uint32_t vb = wave_reserve(&a.ctr->side_vtx, 3 * nfan);
+        uint32_t rb, vb;
+        wave_reserve_fans(a, nfan, rb, vb);
         uint32_t nr = 0, nt = 0, ss = 0;
         if (nfan) {
             bool ok = true;
@@ -611,7 +666,7 @@ hipError_t launch_vs_multi(const MultiArgs &m, uint32_t nb, const DrawArgs &rep,
 
 // The queued triangles are taken 64 per block in a grid-stride loop.  A
 // launch for many draws gives each fewer blocks: most draws queue few
-// triangles, and every block holds 40 KB of LDS while it runs.
+// triangles, and every block holds 44 KB of LDS while it runs.
 uint32_t blocks_vs_deferred(const DrawArgs &a, bool many) {
     int64_t blocks = (a.in.n + 63) / 64;
     const int64_t cap = many ? 16 : 4096;
