@@ -189,9 +189,11 @@ FGL_D V4 intersect_segment(int p, V4 v0, V4 v1) {                               
 
 // sutherlandHodgman (clipping.go:28-52); a triangle yields at most 9
 // points.  buf[0][0..2] holds the triangle on entry; the polygon is left in
-// buf[*which] and its point count returned.  buf is the caller's (LDS in
-// the clipping kernel).
-__device__ __noinline__ int sutherland_hodgman(V4 (*buf)[10], int *which) {
+// buf[*which] and its point count returned.  buf is the caller's (rows of
+// S >= 10 points; LDS in the clipping kernel).  sutherland_hodgman is the
+// out-of-line form the generic vertex kernel calls.
+template <int S>
+FGL_D int sh_passes(V4 (*buf)[S], int *which) {
     int n = 3;
     int cur = 0;
     for (int p = 0; p < 6; p++) {
@@ -220,45 +222,53 @@ __device__ __noinline__ int sutherland_hodgman(V4 (*buf)[10], int *which) {
     return n;
 }
 
+__device__ __noinline__ int sutherland_hodgman(V4 (*buf)[10], int *which) { return sh_passes(buf, which); }
+
 // sutherlandHodgman (clipping.go:28-52) for the clipping kernel: the same
 // point tests, intersections and output order as sutherland_hodgman, with
 // fewer dependent steps per point.  Each plane reads the polygon's points
 // into registers at once and computes every intersection without branching
 // (the ones used are those the reference computes); the outputs are written
-// without branches, a write the reference does not make going to slot 10.
-// A plane that no lane of the wave clips is skipped (it would copy every
-// polygon unchanged).  buf[0][0..2] holds the triangle on entry.
-constexpr int SH_SLOTS = 11;
+// without branches, a write the reference does not make going to slot
+// SH_MAX.  A plane that no lane of the wave clips is skipped (it would copy
+// every polygon unchanged).  buf[0][0..2] holds the triangle on entry.
+// Returns -1 for a lane whose polygon outgrows the SH_IN points a pass
+// reads (a triangle has at most 9 after six planes; only rounding can add
+// a tenth): the caller runs the plain passes (sh_passes) for it.
+constexpr int SH_IN = 9;    // points a pass reads into registers
+constexpr int SH_MAX = 10;  // points a pass keeps, as in sutherland_hodgman
+constexpr int SH_SLOTS = SH_MAX + 1;
 FGL_D int sutherland_hodgman_ilp(V4 (*buf)[SH_SLOTS], int *which) {
     int n = 3;
     int cur = 0;
     for (int p = 0; p < 6; p++) {
+        if (n > SH_IN) return -1;
         const V4 *input = buf[cur];
         V4 *output = buf[cur ^ 1];
-        V4 pt[9];
-        bool fr[9];
+        V4 pt[SH_IN];
+        bool fr[SH_IN];
         bool clips = false;
 #pragma unroll
-        for (int k = 0; k < 9; k++) {
+        for (int k = 0; k < SH_IN; k++) {
             pt[k] = input[k];
             fr[k] = point_in_front(p, pt[k]);
             clips |= k < n && !fr[k];
         }
         if (__ballot(clips) == 0) continue;
         const V4 s0 = input[n > 0 ? n - 1 : 0];
-        V4 x[9];
+        V4 x[SH_IN];
 #pragma unroll
-        for (int k = 0; k < 9; k++) x[k] = intersect_segment(p, k == 0 ? s0 : pt[k - 1], pt[k]);
+        for (int k = 0; k < SH_IN; k++) x[k] = intersect_segment(p, k == 0 ? s0 : pt[k - 1], pt[k]);
         bool sf = point_in_front(p, s0);
         int m = 0;
 #pragma unroll
-        for (int k = 0; k < 9; k++) {
+        for (int k = 0; k < SH_IN; k++) {
             const bool act = k < n, ef = fr[k];
-            const bool wx = act && ef != sf && m < 10;
-            output[wx ? m : 10] = x[k];
+            const bool wx = act && ef != sf && m < SH_MAX;
+            output[wx ? m : SH_MAX] = x[k];
             m += wx ? 1 : 0;
-            const bool we = act && ef && m < 10;
-            output[we ? m : 10] = pt[k];
+            const bool we = act && ef && m < SH_MAX;
+            output[we ? m : SH_MAX] = pt[k];
             m += we ? 1 : 0;
             sf = act ? ef : sf;
         }
@@ -545,6 +555,12 @@ FGL_D void vs_deferred_body(const DrawArgs &a, const G &g) {
             buf[0][1] = o1;
             buf[0][2] = o2;
             np = sutherland_hodgman_ilp(buf, &which);
+            if (np < 0) {
+                buf[0][0] = o0;
+                buf[0][1] = o1;
+                buf[0][2] = o2;
+                np = sh_passes(buf, &which);
+            }
         }
         const uint32_t nfan = np >= 3 ? (uint32_t)(np - 2) : 0u;
         uint32_t rb, vb;
