@@ -838,9 +838,10 @@ __global__ __launch_bounds__(512) void k_nn_stream(const uint32_t *color, const 
                              b0[8 * kg + 4 * t2 + 3]};
         // lane (pm, kg) reads tap 4 g + kg of its pixel.  Taps 9..11 (g = 2,
         // kg > 0) have zero weights; they read tap kg - 1 instead (row r - 1,
-        // complete and finite), so each product is an exact zero, the sums
-        // are those of a zero operand up to the sign of a zero, and the ReLU
-        // makes every zero +0.
+        // complete before this step), so each product is an exact zero, the
+        // sums are those of a zero operand up to the sign of a zero, and the
+        // ReLU makes every zero +0.  (Tap kg - 1 is in the same 3x3 window: a
+        // non-finite feature there reaches the sum through its own tap too.)
         int trow[3], tcol[3];
 #pragma unroll
         for (int g = 0; g < 3; g++) {
