@@ -235,6 +235,31 @@ def test_clipping_near_far_and_behind_camera():
     check("clipping_lines", W, H, [("clear_color", BG), ("shader", sh), ("lines", lines)])
 
 
+def test_clipping_vertices_on_planes_and_corners():
+    """Clip-space vertices (identity matrix, w = 1) on a lattice that puts
+    many of them exactly on the planes x, y, z = +-1 and makes triangles
+    cross two or three planes at once (polygons of up to 9 points, several
+    fans each), plus triangles with a vertex behind the eye (w <= 0 through
+    a perspective matrix): the clipping kernel's register form and its
+    fallback must give the oracle's polygons, fans and records."""
+    W, H = 96, 80
+    rng = np.random.Generator(np.random.PCG64(97))
+    lattice = np.array([-3.0, -2.0, -1.0, -0.5, 0.0, 0.5, 1.0, 2.0, 3.0])
+    pos = lattice[rng.integers(0, len(lattice), size=(3000, 3, 3))]
+    p = A.Prims(pos, M.with_face_normals(pos), None, rng.uniform(0, 1, size=(3000, 3, 4)))
+    sh = A.NewSolidColorShader(ORTHO, A.Color(0.2, 0.6, 0.3, 1))
+    check("clipping_lattice_on_planes", W, H, [("clear_color", BG), ("set", {"Cull": A.CullNone}), ("shader", sh),
+                                              ("tris", p)])
+    # big triangles around a perspective eye: vertices behind it and on the
+    # near plane's side of it
+    mat = A.LookAt(A.V(0, 0, 0.5), A.V(0, 0, -1), A.V(0, 1, 0)).Perspective(75, W / H, 0.25, 4)
+    big = rng.uniform(-6, 6, size=(1500, 3, 3))
+    q = A.Prims(big, M.with_face_normals(big), None, rng.uniform(0, 1, size=(1500, 3, 4)))
+    ph = A.NewPhongShader(mat, A.V(0.3, 0.5, 1), A.V(0, 0, 0.5))
+    check("clipping_behind_eye", W, H, [("clear_color", BG), ("set", {"Cull": A.CullNone}), ("shader", ph),
+                                       ("tris", q)])
+
+
 def test_mesh_triangles_then_lines():
     W, H = 128, 128
     t = rand_tris(500, 101, size=0.4, colors=True)
