@@ -99,6 +99,20 @@ FGL_D void emit_body(const DrawArgs &a, const G &g) {
     uint32_t cnt = nr & ~REF_SIDE;
     uint32_t *keys = a.keys[0];
     uint32_t *vals = a.vals[0];
+    if (!side) {
+        // the main record (a triangle drawn without clipping): its box from
+        // the compact copy the vertex stage wrote, not from the record
+        const int4 bx = a.tbox[i];
+        TileRect r;
+        if (!prim_tile_rect(bx.x, bx.y, bx.z, bx.w, a.W, a.H, r)) return;
+        for (int32_t ty = r.ty0; ty <= r.ty1; ty++)
+            for (int32_t tx = r.tx0; tx <= r.tx1; tx++) {
+                keys[pos] = (uint32_t)(ty * a.ntx + tx);
+                vals[pos] = (uint32_t)i;
+                pos++;
+            }
+        return;
+    }
     for (uint32_t j = 0; j < cnt; j++) {
         const PrimRec *rec;
         uint32_t ref;

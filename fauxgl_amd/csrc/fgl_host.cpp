@@ -149,7 +149,7 @@ struct fgl_ctx {
     std::map<int64_t, Tex> tex;
     int64_t next_tex = 1;
 
-    DevBuf ctr, ntl, nrast, sstart, blocksum, blockoff, rec_main, side_rec, side_vtx;
+    DevBuf ctr, ntl, nrast, sstart, blocksum, blockoff, rec_main, tbox, side_rec, side_vtx;
     HostBuf hctr;  // the draw counters as the host reads them back
     DevBuf keys0, vals0, keys1, vals1, hist, dtot, tstart, tend, deferred;
     DevBuf pspans, prows, span_bsum, span_boff, poff, spans;
@@ -603,6 +603,7 @@ int draw_begin(fgl_ctx *c, const fgl_state *st, const fgl_shader *sh, int kind, 
     HIPCHK(c, c->blocksum.ensure((size_t)a.nvsblocks * 8));
     HIPCHK(c, c->blockoff.ensure((size_t)a.nvsblocks * 8));
     if (kind == PRIM_TRI) HIPCHK(c, c->rec_main.ensure((size_t)n * sizeof(PrimRec)));
+    if (kind == PRIM_TRI) HIPCHK(c, c->tbox.ensure((size_t)n * sizeof(int4)));
     if (kind == PRIM_TRI) HIPCHK(c, c->deferred.ensure((size_t)n * 4));
     HIPCHK(c, c->dtot.ensure(256 * 4));
     HIPCHK(c, c->tstart.ensure((size_t)a.ntiles * 4));
@@ -649,6 +650,7 @@ int draw_buffers(fgl_ctx *c, DrawArgs &a) {
     a.frag_cap = c->frag_cap;
 
     a.rec_main = c->rec_main.as<PrimRec>();
+    a.tbox = c->tbox.as<int4>();
     a.side_rec = c->side_rec.as<PrimRec>();
     a.side_rec_cap = c->side_rec_cap;
     a.side_vtx = c->side_vtx.as<double>();

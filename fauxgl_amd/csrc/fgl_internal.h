@@ -41,6 +41,7 @@ struct DrawArgs {
     uint32_t *ntl;         // per input primitive: (tile, triangle) pairs it emits
     uint32_t *nrast;       // per input primitive: raster triangles (bit 31: side records)
     uint32_t *sstart;      // per input primitive: first side record
+    int4 *tbox;            // per input triangle: its main record's box (x0, y0, x1, y1), for the pair emitter
     uint64_t *blocksum;    // per vertex-stage block: sum of ntl
     uint64_t *blockoff;    // exclusive prefix of blocksum
     int32_t nvsblocks;

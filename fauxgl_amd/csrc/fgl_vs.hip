@@ -63,7 +63,7 @@ FGL_D bool surely_empty(V3 s0, V3 s1, V3 s2, int64_t x0, int64_t y0, int64_t x1,
 // framebuffer, or a small box has no center the inside test can accept);
 // such a call has no effect in the reference either.
 FGL_D bool make_raster(const DrawArgs &a, V3 s0, V3 s1, V3 s2, double W0, double W1, double W2, uint64_t attr,
-                       PrimRec *out, uint32_t &ntiles) {
+                       PrimRec *out, uint32_t &ntiles, int4 *box = nullptr) {
     // integer bounding box (context.go:155-160)
     double mnx = __builtin_floor(go_min(s0.x, go_min(s1.x, s2.x)));
     double mny = __builtin_floor(go_min(s0.y, go_min(s1.y, s2.y)));
@@ -89,19 +89,22 @@ FGL_D bool make_raster(const DrawArgs &a, V3 s0, V3 s1, V3 s2, double W0, double
         out->x1 = (int32_t)x1; out->y1 = (int32_t)y1;
         // ra = 1 / edge(s0, s1, s2) (context.go:175)
         out->ra = 1.0 / ((s1.x - s2.x) * (s0.y - s2.y) - (s1.y - s2.y) * (s0.x - s2.x));
+        if (box) *box = make_int4((int32_t)x0, (int32_t)y0, (int32_t)x1, (int32_t)y1);
     }
     return true;
 }
 
 // Collects the raster triangles of one input primitive, counting on the
-// first pass and writing on the second.
+// first pass and writing on the second; box (main records only: one raster
+// triangle) also receives the record's box, compact for the pair emitter.
 struct Emitter {
     PrimRec *base;  // nullptr on the counting pass
     uint32_t n;
     uint32_t ntiles;
+    int4 *box;
     FGL_D void add(const DrawArgs &a, V3 s0, V3 s1, V3 s2, double W0, double W1, double W2, uint64_t attr) {
         uint32_t t = 0;
-        if (make_raster(a, s0, s1, s2, W0, W1, W2, attr, base ? base + n : nullptr, t)) {
+        if (make_raster(a, s0, s1, s2, W0, W1, W2, attr, base ? base + n : nullptr, t, box)) {
             n++;
             ntiles += t;
         }
@@ -428,7 +431,7 @@ FGL_D void vs_triangle(const DrawArgs &a, int64_t i, uint32_t &nr, uint32_t &nt,
     const uint32_t ident[3] = {0, 1, 2};
     if (!(woutside(o0) || woutside(o1) || woutside(o2))) {
         if (!a.st.wireframe) {
-            Emitter em{a.rec_main + i, 0, 0};
+            Emitter em{a.rec_main + i, 0, 0, a.tbox + i};
             emit_clipped_triangle(a, em, ATTR_TRI, (uint64_t)i, ident, o0, o1, o2);
             nr = em.n;
             nt = em.ntiles;
@@ -480,14 +483,15 @@ FGL_D void vs_fast_body(const DrawArgs &a, const G &g) {
             defer = !rejected;
         } else {
             const uint32_t ident[3] = {0, 1, 2};
-            Emitter em{a.rec_main + i, 0, 0};
+            Emitter em{a.rec_main + i, 0, 0, a.tbox + i};
             emit_clipped_triangle(a, em, ATTR_TRI, (uint64_t)i, ident, o0, o1, o2);
             nr = em.n;
             nt = em.ntiles;
         }
+        // (sstart is read only for side records; k_vs_deferred writes it
+        // for the triangles queued here)
         a.nrast[i] = nr;
         a.ntl[i] = nt;
-        a.sstart[i] = 0;
     }
     // queue the deferred triangles: one counter update per wave
     const uint64_t m = __ballot(defer);
